@@ -1,0 +1,431 @@
+"""Streamline-tracking throughput on one MI355X.
+
+Metric: streamline steps/s (one step = one orientation update of one
+streamline, tracking.py:135) over the Diffusion workload on a 145x174x145
+SH order-8 FOD with 1M seeds, FAST path (persistent lanes, transcendental-free
+model evaluation, FP64 arc length) with the PLAIN FP32 path (the reference's
+integrator, bit-exact with the host restatement) timed beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--seeds N]
+
+Prints one JSON line. Inputs (field, mask, seeds) are resident in HBM before
+timing; each timed step is the tracking kernel plus the on-device compaction
+of all saved points into one ragged array.
+
+The module also holds the synthetic workload generators (configs 0-4 of the
+benchmark definition) used by the tests.
+"""
+
+from __future__ import annotations
+
+import argparse
+import concurrent.futures
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+SHAPE_145 = (145, 174, 145)
+
+# --------------------------------------------------------------------------
+# Synthetic FOD fields
+# --------------------------------------------------------------------------
+
+
+def real_sh_basis(xyz: np.ndarray, lmax: int, transpose: bool = False) -> np.ndarray:
+    """Real symmetric SH (MRtrix order, orthonormal) of unit vectors, FP64.
+
+    Fully normalised associated Legendre functions by the standard three-term
+    recurrence in l, Condon-Shortley phase included; index l(l+1)/2 + m over
+    even l, m = -l..l, sqrt(2) Im for m < 0 and sqrt(2) Re for m > 0.
+    Returns (n, K), or (K, n) with transpose=True.
+    """
+    xyz = np.asarray(xyz, dtype=np.float64)
+    x, y, z = xyz[:, 0], xyz[:, 1], xyz[:, 2]
+    s = np.sqrt(x * x + y * y)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        e1 = np.where(s > 0, (x + 1j * y) / s, 1.0 + 0j)
+    n = len(xyz)
+    K = (lmax + 1) * (lmax + 2) // 2
+    out = np.zeros((K, n))
+    p_mm = np.full(n, 1.0 / math.sqrt(4 * math.pi))
+    em = np.ones(n, dtype=np.complex128)
+    for m in range(lmax + 1):
+        if m > 0:
+            p_mm = -math.sqrt((2 * m + 1) / (2 * m)) * s * p_mm
+            em = em * e1
+        p_prev2 = None
+        p_prev = p_mm
+        for l in range(m, lmax + 1):
+            if l == m:
+                p = p_mm
+            elif l == m + 1:
+                p = math.sqrt(2 * m + 3) * z * p_mm
+            else:
+                a = math.sqrt((4 * l * l - 1) / (l * l - m * m))
+                b = math.sqrt(((l - 1) ** 2 - m * m) / (4 * (l - 1) ** 2 - 1))
+                p = a * (z * p_prev - b * p_prev2)
+            if l > m:
+                p_prev2, p_prev = p_prev, p
+            if l % 2 == 0:
+                centre = l * (l + 1) // 2
+                if m == 0:
+                    out[centre] = p
+                else:
+                    sp = math.sqrt(2) * p
+                    out[centre + m] = sp * em.real
+                    out[centre - m] = sp * em.imag
+    return out if transpose else out.T
+
+
+def axial_kernel_sh(lmax: int = 8, sharpness: float = 4.0) -> np.ndarray:
+    """Funk-Hecke weights k_l = 2 pi int k(t) P_l(t) dt of k(t) = exp(s (t^2 - 1)):
+    the SH coefficients of k(u . d) are k_l Y_lm(d)."""
+    t, w = np.polynomial.legendre.leggauss(200)
+    k = np.exp(sharpness * (t * t - 1.0))
+    weights = np.zeros(lmax + 1)
+    for l in range(0, lmax + 1, 2):
+        weights[l] = 2 * math.pi * np.sum(w * k * np.polynomial.legendre.Legendre.basis(l)(t))
+    return weights
+
+
+def _coefficient_degrees(K: int) -> np.ndarray:
+    degrees = []
+    l = 0
+    while len(degrees) < K:
+        degrees.extend([l] * (2 * l + 1))
+        l += 2
+    return np.array(degrees[:K])
+
+
+def fod_from_fibres(directions: np.ndarray, weights: np.ndarray, lmax: int = 8, chunk: int = 1 << 17) -> np.ndarray:
+    """Sum of weighted axially symmetric fibre kernels as SH coefficients,
+    normalised to unit integral per voxel (utils.normalize_odf).
+
+    directions (V, F, 3) unit vectors, weights (V, F) -> (V, K) float32.
+    """
+    V, F, _ = directions.shape
+    K = (lmax + 1) * (lmax + 2) // 2
+    kernel = axial_kernel_sh(lmax)[_coefficient_degrees(K)][:, None]
+    out = np.zeros((V, K), dtype=np.float32)
+
+    def work(start):
+        stop = min(V, start + chunk)
+        acc = np.zeros((K, stop - start))
+        for f in range(F):
+            basis = real_sh_basis(directions[start:stop, f], lmax, transpose=True)
+            basis *= kernel
+            basis *= weights[start:stop, f][None, :]
+            acc += basis
+        c0 = acc[:1]
+        scale = np.zeros_like(c0)
+        np.divide(1.0 / np.sqrt(4 * np.pi), c0, out=scale, where=c0 != 0)
+        out[start:stop] = (acc * scale).T
+
+    workers = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 4)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, workers)) as pool:
+        list(pool.map(work, range(0, V, chunk)))
+    return out
+
+
+def _random_fibres(rng, n_voxels: int, base_directions=None):
+    """1-3 fibres per voxel with Dirichlet(1) weights."""
+    n = rng.integers(1, 4, n_voxels)
+    raw = rng.gamma(1.0, size=(n_voxels, 3))
+    raw[np.arange(3)[None, :] >= n[:, None]] = 0
+    weights = raw / raw.sum(axis=1, keepdims=True)
+    if base_directions is None:
+        d = rng.standard_normal((n_voxels, 3, 3))
+    else:
+        d = base_directions
+    d = d / np.linalg.norm(d, axis=-1, keepdims=True)
+    return d, weights
+
+
+def smooth_direction_fields(rng, shape, sigma: float = 4.0, n_fields: int = 3) -> np.ndarray:
+    """n_fields unit vector fields: three N(0,1) components per field,
+    Gaussian-filtered (sigma voxels, reflect) and normalised. (V, n, 3)."""
+    from scipy.ndimage import gaussian_filter
+
+    fields = np.empty((int(np.prod(shape)), n_fields, 3), dtype=np.float64)
+    for f in range(n_fields):
+        for c in range(3):
+            noise = rng.standard_normal(shape, dtype=np.float32)
+            fields[:, f, c] = gaussian_filter(noise, sigma, mode="reflect").reshape(-1)
+    norm = np.sqrt(np.einsum("vfc,vfc->vf", fields, fields))
+    fields /= norm[..., None]
+    return fields
+
+
+def ellipsoid_mask(shape, semi_axes) -> np.ndarray:
+    centre = (np.array(shape) - 1) / 2.0
+    idx = np.indices(shape).reshape(3, -1).T.astype(np.float64)
+    r = np.sum(((idx - centre) / np.array(semi_axes)) ** 2, axis=1)
+    return (r <= 1.0).reshape(shape)
+
+
+def seeds_in_mask(rng, mask: np.ndarray, n_seeds: int) -> np.ndarray:
+    """from_mask semantics (seeds.py:114-142) with a seeded generator:
+    uniform nonzero voxel, U[-0.5, 0.5)^3 jitter, normalised Gaussian
+    direction; identity affine."""
+    voxels = np.argwhere(mask)
+    index = rng.integers(0, len(voxels), n_seeds)
+    locations = voxels[index] + rng.random((n_seeds, 3)) - 0.5
+    directions = rng.standard_normal((n_seeds, 3))
+    directions /= np.linalg.norm(directions, axis=1, keepdims=True)
+    seeds = np.zeros((n_seeds, 8), dtype=np.float32)
+    seeds[:, :3] = locations
+    seeds[:, 3] = 1
+    seeds[:, 4:7] = directions
+    return seeds
+
+
+def config0(seed: int = 0, n_seeds: int = 10_000):
+    """32^3 SH order-8 FOD from 1-3 random fibres per voxel, sphere mask of
+    radius 14, seeds in the mask; Transport, step 0.1, save 0.25, max 300."""
+    rng = np.random.default_rng(seed)
+    shape = (32, 32, 32)
+    d, w = _random_fibres(rng, int(np.prod(shape)))
+    fod = fod_from_fibres(d, w).reshape(*shape, 45)
+    idx = np.indices(shape).reshape(3, -1).T
+    mask = (np.linalg.norm(idx - 15.5, axis=1) <= 14).reshape(shape)
+    fod[~mask] = 0
+    seeds = seeds_in_mask(rng, mask, n_seeds)
+    return {"fod": fod, "mask": mask.astype(np.uint8), "seeds": seeds, "algorithm": "transport",
+            "step_size": 0.1, "gamma": 1.0, "noise_variance": 0.0, "save_at": 0.25, "max_length": 300.0}
+
+
+def config1(seed: int = 0, n_seeds: int = 1_000_000, shape=SHAPE_145):
+    """145x174x145 SH order-8 FOD whose 1-3 fibres per voxel follow smooth
+    random direction fields (sigma 4), ellipsoid mask 60/75/60; Diffusion
+    gamma 1, noise variance 0.1, step 0.1, save 0.25, max 300."""
+    rng = np.random.default_rng(seed)
+    directions = smooth_direction_fields(rng, shape)
+    _, weights = _random_fibres(rng, int(np.prod(shape)), base_directions=directions)
+    fod = fod_from_fibres(directions, weights).reshape(*shape, 45)
+    semi = np.array([60, 75, 60]) * np.array(shape) / np.array(SHAPE_145)
+    mask = ellipsoid_mask(shape, semi)
+    fod[~mask] = 0
+    seeds = seeds_in_mask(rng, mask, n_seeds)
+    return {"fod": fod, "mask": mask.astype(np.uint8), "seeds": seeds, "algorithm": "diffusion",
+            "step_size": 0.1, "gamma": 1.0, "noise_variance": 0.1, "save_at": 0.25, "max_length": 300.0}
+
+
+# --------------------------------------------------------------------------
+# Measurement
+# --------------------------------------------------------------------------
+
+MI355X_HBM_PEAK_GBS = 8000.0
+
+
+def bytes_per_step(K: int, step: float, save_at: float) -> float:
+    """Algorithmic HBM bytes per step: K f32 coefficients, one mask byte
+    counted as the reference's 4-byte int, and 12 of the 16 bytes of a saved
+    point times saves per step."""
+    return 4.0 * K + 4.0 + 16.0 * step / save_at
+
+
+def make_algorithm(tg, cfg, seed: int = 0):
+    model = tg.local.RealSymmetricSphericalHarmonics(cfg["fod"].shape[-1])
+    if cfg["algorithm"] == "transport":
+        return tg.algorithm.Transport(model, step_size=cfg["step_size"], gamma=cfg["gamma"])
+    return tg.algorithm.Diffusion(model, step_size=cfg["step_size"], gamma=cfg["gamma"],
+                                  noise_variance=cfg["noise_variance"], seed=seed)
+
+
+class DeviceRun:
+    """Device-resident inputs and outputs of one workload for repeated timing."""
+
+    def __init__(self, tg, torch, runtime, cfg, batch_size: int = 50_000):
+        from tractography_amd import _native
+
+        self.tg, self.torch, self.runtime, self.cfg = tg, torch, runtime, cfg
+        self.native = _native
+        fod_image = tg.Nifti1Image(cfg["fod"], np.eye(4))
+        self.field = tg.field.Image(fod_image)
+        self.domain = tg.domain.Image(tg.Nifti1Image(cfg["mask"], np.eye(4)))
+        self.algorithm = make_algorithm(tg, cfg)
+        self.n_points = tg.tracking.point_capacity(cfg["max_length"], cfg["save_at"])
+        seeds = np.ascontiguousarray(cfg["seeds"], dtype=np.float32)
+        self.n = len(seeds)
+        self.batch_size = batch_size
+        n_batches = (self.n + batch_size - 1) // batch_size
+        rng_seeds = np.array([self.algorithm.get_rng_seed() for _ in range(n_batches)], dtype=np.uint32)
+        dev = torch.device("cuda", runtime.device)
+        self.d_seeds = torch.from_numpy(seeds).to(dev)
+        self.d_rng = torch.from_numpy(rng_seeds.view(np.int32)).to(dev)
+        self.d_lengths = torch.empty(self.n, dtype=torch.int32, device=dev)
+        self.d_steps = torch.empty(self.n, dtype=torch.int32, device=dev)
+        self.d_hits = torch.empty(self.n, dtype=torch.int32, device=dev)
+        self.d_dense = torch.empty(self.n * self.n_points * 3, dtype=torch.float32, device=dev)
+        self.d_offsets = torch.empty(self.n + 1, dtype=torch.int64, device=dev)
+        self.d_points = torch.empty(self.n * self.n_points * 3, dtype=torch.float32, device=dev)
+        self.field_handle = self.field.upload(runtime)
+        self.domain_handle = self.domain.upload(runtime)
+        torch.cuda.synchronize()
+        self.stream = torch.cuda.ExternalStream(runtime.stream(), device=dev)
+
+    def track(self, path: int):
+        lib = self.native.lib()
+        if path == self.native.PATH_FAST:
+            self.native.check(lib.tgk_field_set_fast_precision(self.field_handle.handle, 0))
+        self.native.check(lib.tgk_track_device(
+            self.runtime.handle, self.domain_handle.handle, self.field_handle.handle, self.algorithm.descriptor(),
+            path, self.d_seeds.data_ptr(), 0, self.n, self.d_rng.data_ptr(), self.batch_size, self.n_points,
+            float(np.float32(self.cfg["save_at"])), self.d_lengths.data_ptr(), self.d_steps.data_ptr(),
+            self.d_hits.data_ptr(), self.d_dense.data_ptr()))
+
+    def compact(self, path: int):
+        lib = self.native.lib()
+        self.native.check(lib.tgk_compact_device(
+            self.runtime.handle, path, self.d_dense.data_ptr(), self.n, self.n_points, self.d_lengths.data_ptr(),
+            self.d_offsets.data_ptr(), self.d_points.data_ptr()))
+
+    def timed(self, path: int, warmup: int, steps: int):
+        """(ms per step, track-kernel ms per step, total orientation updates)."""
+        torch = self.torch
+        for _ in range(warmup):
+            self.track(path)
+            self.compact(path)
+        self.runtime.finish()
+        events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        torch.cuda.synchronize()
+        self.runtime.finish()
+        t0 = time.perf_counter()
+        for start, stop in events:
+            start.record(self.stream)
+            self.track(path)
+            stop.record(self.stream)
+            self.compact(path)
+        self.runtime.finish()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / steps
+        kernel = float(np.mean([a.elapsed_time(b) for a, b in events]))
+        total_steps = int(self.d_steps.to(torch.int64).sum().item())
+        return wall, kernel, total_steps
+
+    def results(self):
+        torch = self.torch
+        lengths = self.d_lengths.cpu().numpy().view(np.uint32)
+        total = int(self.d_offsets[-1].item())
+        points = self.d_points[: total * 3].cpu().numpy().reshape(-1, 3)
+        return lengths, self.d_steps.cpu().numpy().view(np.uint32), points
+
+
+def cpu_baseline(cfg, seconds: float = 15.0):
+    """FP32 host restatement (OpenMP over seeds, all host cores) on a leading
+    sample of the workload's seeds, sized to ~`seconds` of wall time."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle  # noqa: E402  (the host restatement; CPU baseline only)
+
+    import tractography_amd as tg
+
+    field = tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4)))
+    domain = tg.domain.Image(tg.Nifti1Image(cfg["mask"], np.eye(4)))
+    algorithm = make_algorithm(tg, cfg)
+    rng_seeds = [algorithm.get_rng_seed() for _ in range((len(cfg["seeds"]) + 49_999) // 50_000)]
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    sample = max(64, threads * 8)
+    while True:
+        seeds = cfg["seeds"][:sample]
+        t0 = time.perf_counter()
+        r = oracle.track(domain, field, algorithm, seeds, rng_seeds, 50_000, cfg["save_at"], cfg["max_length"],
+                         threads=threads)
+        dt = time.perf_counter() - t0
+        if dt > seconds / 4 or sample >= len(cfg["seeds"]):
+            break
+        sample = min(len(cfg["seeds"]), int(sample * max(2.0, seconds / max(dt, 1e-3))))
+    steps = int(r["steps"].astype(np.int64).sum())
+    return {"value": steps / dt, "unit": "streamline steps/s", "cores": threads, "kind": "port",
+            "sample": f"first {len(seeds)} seeds of the workload, {steps} steps in {dt:.1f} s "
+                      "(FP32 host restatement, OpenMP)"}
+
+
+def main():
+    parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=3)
+    parser.add_argument("--warmup", type=int, default=1)
+    parser.add_argument("--config", type=int, default=1, choices=[0, 1])
+    parser.add_argument("--seeds", type=int, default=None)
+    parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--paths", default="fast,plain")
+    args = parser.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (seeds shard with no "
+              "communication, see the multi-GPU section of the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import tractography_amd as tg
+    from tractography_amd import _native
+
+    t0 = time.perf_counter()
+    if args.config == 0:
+        cfg = config0(n_seeds=args.seeds or 10_000)
+        workload = "config0: 32^3 SH45 Transport, 10k seeds, step 0.1, save 0.25, max 300"
+    else:
+        cfg = config1(n_seeds=args.seeds or 1_000_000)
+        workload = ("config1: 145x174x145 SH45 (lmax 8) Diffusion gamma 1 sigma^2 0.1, "
+                    f"{len(cfg['seeds'])} seeds, step 0.1, save 0.25, max 300")
+    print(f"[bench] workload generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+
+    runtime = tg.HipRuntime()
+    run = DeviceRun(tg, torch, runtime, cfg)
+    paths = [p for p in args.paths.split(",") if p]
+    results = {}
+    for name in paths:
+        path = _native.PATH_FAST if name == "fast" else _native.PATH_PLAIN
+        try:
+            wall, kernel, total = run.timed(path, args.warmup, args.steps)
+        except NotImplementedError as e:
+            print(f"[bench] {name} path unavailable: {e}", file=sys.stderr, flush=True)
+            continue
+        results[name] = {"ms_per_step": wall, "kernel_ms": kernel, "steps": total}
+        print(f"[bench] {name}: {wall:.1f} ms/step, kernel {kernel:.1f} ms, {total} orientation updates, "
+              f"{total / wall * 1e3:.3e} steps/s", file=sys.stderr, flush=True)
+    headline = "fast" if "fast" in results else "plain"
+    h = results[headline]
+    K = cfg["fod"].shape[-1]
+    bps = bytes_per_step(K, cfg["step_size"], cfg["save_at"])
+    achieved = bps * h["steps"] / (h["kernel_ms"] * 1e-3) / 1e9
+    line = {
+        "metric": "streamline steps/s on 1 MI355X, mixed vs FP32 path, at <=1e-3 voxel max deviation",
+        "value": h["steps"] / (h["ms_per_step"] * 1e-3),
+        "unit": "streamline steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": h["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded numpy generators)",
+        "config": {"workload": workload, "path": headline, "seeds": len(cfg["seeds"]),
+                   "orientation_updates_per_step": h["steps"]},
+        "paths": {k: {"steps_per_s": v["steps"] / (v["ms_per_step"] * 1e-3), "ms_per_step": v["ms_per_step"],
+                      "kernel_ms": v["kernel_ms"]} for k, v in results.items()},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / MI355X_HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"track_{headline}", "bytes_per_step": bps},
+    }
+    if "fast" in results and "plain" in results:
+        line["speedup_fast_over_plain"] = (results["fast"]["steps"] / results["fast"]["ms_per_step"]) / (
+            results["plain"]["steps"] / results["plain"]["ms_per_step"])
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(cfg)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,191 @@
+/*
+ * tgk.h -- C ABI of the MI355X tracking engine (libtgk.so).
+ *
+ * The library replaces the OpenCL backend of the reference's tracking path.
+ * Each entry point below names the reference interface it stands in for
+ * (paths relative to the reference repository's tractography/ package).
+ * Python binds these with ctypes (tractography_amd/_native.py); any FFI that
+ * can pass plain pointers and sizes can do the same.
+ *
+ * Conventions
+ *   - Every function returns an int status: TGK_OK (0) or a negative
+ *     TGK_ERR_* category; tgk_last_error() then describes the failure
+ *     (thread-local, valid until the next call on the same thread).
+ *   - Host arrays are C-contiguous and owned by the caller. Device memory is
+ *     owned by the runtime/field/domain/tracks handles and freed by their
+ *     destroy functions.
+ *   - Functions taking `d_` pointers operate on device memory and enqueue on
+ *     the runtime's stream without synchronising; all others are blocking.
+ *   - One runtime is one GPU and one in-order HIP stream. A handle is not
+ *     re-entrant; distinct runtimes may be used from distinct threads.
+ */
+#ifndef TGK_H
+#define TGK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TGK_OK = 0,
+    TGK_ERR_INVALID = -1,     /* bad argument, shape or size */
+    TGK_ERR_NO_MEMORY = -2,   /* device or host allocation failed */
+    TGK_ERR_DEVICE = -3,      /* HIP runtime or kernel failure */
+    TGK_ERR_UNSUPPORTED = -4  /* valid request this build cannot serve */
+};
+
+enum { TGK_MODEL_SH = 0, TGK_MODEL_DTI = 1 };
+enum { TGK_ALGO_TRANSPORT = 0, TGK_ALGO_DIFFUSION = 1 };
+
+/* Tracking paths. PLAIN is the reference's integrator, bit-exact with the
+ * FP32 host restatement. FAST is the persistent, lane-refilling integrator
+ * with a transcendental-free Cartesian model evaluation and FP64 arc length;
+ * it is judged against the FP64 oracle. */
+enum { TGK_PATH_PLAIN = 0, TGK_PATH_FAST = 1 };
+
+/* Storage of the coefficients read by the FAST path. */
+enum { TGK_PRECISION_FP32 = 0, TGK_PRECISION_BF16X2 = 1, TGK_PRECISION_FP16 = 2, TGK_PRECISION_BF16 = 3 };
+
+/* Seeding-from-FOD methods: VALU reproduces seeds.cl bit for bit; MFMA
+ * evaluates all 1000 directions as one BF16 (hi + lo split) GEMM. */
+enum { TGK_SEEDS_VALU = 0, TGK_SEEDS_MFMA = 1 };
+
+typedef struct tgk_runtime tgk_runtime;
+typedef struct tgk_field tgk_field;
+typedef struct tgk_domain tgk_domain;
+typedef struct tgk_tracks tgk_tracks;
+
+/* Orientation-update algorithm and its local model: the kernel arguments of
+ * Transport (algorithm.py:115-117) and Diffusion (algorithm.py:231-237) plus
+ * the model selection of local.from_shape (local.py:265-282). */
+typedef struct {
+    int32_t model;        /* TGK_MODEL_* */
+    int32_t algorithm;    /* TGK_ALGO_* */
+    int32_t n_parameters; /* K: 1..45 for SH, 6 for DTI */
+    float step_size;
+    float gamma;
+    float noise_variance; /* Diffusion only */
+} tgk_algorithm;
+
+const char *tgk_last_error(void);
+int tgk_version(void);
+
+/* ------------------------------------------------------------ runtime --- */
+
+/* Replaces OpenCLRuntime.__init__ / finish (opencl.py:30-84, 163). */
+int tgk_runtime_create(int32_t device, tgk_runtime **out);
+int tgk_runtime_destroy(tgk_runtime *rt);
+int tgk_runtime_synchronize(tgk_runtime *rt);
+/* Device name (NUL-terminated, truncated to name_len), compute units, HBM. */
+int tgk_runtime_info(tgk_runtime *rt, char *name, int32_t name_len, int32_t *compute_units, int64_t *total_bytes);
+/* The runtime's HIP stream (hipStream_t) for callers sharing device buffers. */
+int tgk_runtime_stream(tgk_runtime *rt, void **stream);
+
+/* ------------------------------------------------------ field, domain --- */
+
+/* Replaces field.Image.get_args (field.py:117-122): uploads the (X, Y, Z, K)
+ * float32 image once; `inverse_affine` is the row-major float32 4x4. */
+int tgk_field_create(tgk_runtime *rt, const float *data, const uint32_t dims[4], const float inverse_affine[16],
+                     tgk_field **out);
+int tgk_field_destroy(tgk_field *field);
+
+/* Replaces domain.Image.get_args (domain.py:385-398): mask (X, Y, Z), any
+ * nonzero byte is inside. */
+int tgk_mask_domain_create(tgk_runtime *rt, const uint8_t *mask, const uint32_t dims[3],
+                           const float inverse_affine[16], tgk_domain **out);
+
+/* Replaces BaseDomain.get_args's boundary and BVH buffers
+ * (domain.py:170-199) for a domain bounded by triangle meshes: tracking stops
+ * when a step's segment hits a boundary triangle (bvh.cl:83-131).
+ * vertices (V, 4) f32, triangles (T, 4) u32, cumulative (T,) f32 normalised
+ * areas, BVH node bounds (N, 4) f32, nodes (N, 4) i32 (left, right, first,
+ * count), order (M,) u32. The culling grid (near, dims, inverse affine) flags
+ * cells within `reach` of a triangle bounding box; it may be NULL. */
+int tgk_mesh_domain_create(tgk_runtime *rt, const float *vertices, uint32_t n_vertices, const uint32_t *triangles,
+                           const float *cumulative, uint32_t n_triangles, const float *bvh_min, const float *bvh_max,
+                           const int32_t *bvh_nodes, uint32_t n_nodes, const uint32_t *bvh_order, uint32_t n_order,
+                           const uint8_t *near, const uint32_t near_dims[3], const float near_inverse_affine[16],
+                           float reach, tgk_domain **out);
+
+/* Attaches the boundary surface used by boundary seeding (domain.py:48-70)
+ * to a mask domain; mesh domains already carry theirs. */
+int tgk_domain_set_boundary(tgk_domain *domain, const float *vertices, uint32_t n_vertices,
+                            const uint32_t *triangles, const float *cumulative, uint32_t n_triangles);
+int tgk_domain_destroy(tgk_domain *domain);
+
+/* ------------------------------------------------------------ tracking --- */
+
+/* Replaces tracking.tractogram (tracking.py:22-178) and the batch loop of
+ * tractography.tractogram (__init__.py:524-539): seeds (n, 8) f32 rows
+ * (x, y, z, 1, dx, dy, dz, 0); seed g draws its noise stream from
+ * rng_init(rng_seeds[g / batch_size], g % batch_size) like the reference's
+ * per-batch get_rng_seed(); n_points = floor(max_length / save_at) is the
+ * point capacity including the seed. Results stay in the returned handle. */
+int tgk_track(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field, const tgk_algorithm *algorithm,
+              int32_t path, const float *seeds, int64_t n_seeds, const uint32_t *rng_seeds, int64_t batch_size,
+              uint32_t n_points, float save_at, tgk_tracks **out);
+/* Number of streamlines and of saved points over all of them. */
+int tgk_tracks_size(const tgk_tracks *tracks, int64_t *n_seeds, int64_t *total_points);
+/* Copies per-seed saved-point counts, orientation-update counts, crossed
+ * boundary triangle (0xffffffff when none) and the ragged points
+ * (total_points, 3) in seed order. Any output pointer may be NULL. */
+int tgk_tracks_copy(const tgk_tracks *tracks, uint32_t *lengths, uint32_t *steps, uint32_t *hits, float *points);
+int tgk_tracks_destroy(tgk_tracks *tracks);
+
+/* Device-resident tracking of one chunk, asynchronous on the runtime stream.
+ * d_seeds (n, 8); first_seed is the global index of d_seeds[0] (noise
+ * streams); d_dense holds n * n_points * 3 floats (layout private to the
+ * path); then tgk_compact_device turns it into ragged points. */
+int tgk_track_device(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field,
+                     const tgk_algorithm *algorithm, int32_t path, const float *d_seeds, int64_t first_seed,
+                     int64_t n_seeds, const uint32_t *d_rng_seeds, int64_t batch_size, uint32_t n_points,
+                     float save_at, uint32_t *d_lengths, uint32_t *d_steps, uint32_t *d_hits, float *d_dense);
+/* d_offsets (n + 1) receives the exclusive prefix sum of d_lengths (the last
+ * entry is the total); d_points receives the ragged (total, 3) points and
+ * must hold up to n * n_points * 3 floats. */
+int tgk_compact_device(tgk_runtime *rt, int32_t path, const float *d_dense, int64_t n_seeds, uint32_t n_points,
+                       const uint32_t *d_lengths, int64_t *d_offsets, float *d_points);
+
+/* Storage precision of the coefficients the FAST path reads from `field`
+ * (default FP32); converts once on the device and caches. */
+int tgk_field_set_fast_precision(tgk_field *field, int32_t precision);
+
+/* -------------------------------------------------------------- seeds --- */
+
+/* Replaces domain.seed (domain.py:494-575): n boundary seeds (n, 8). */
+int tgk_domain_seed(tgk_runtime *rt, const tgk_domain *domain, uint32_t rng_seed, int64_t n, float *out);
+
+/* Replaces the OpenCL branch of seeds.from_fod (seeds.py:174-216): sparse
+ * FOD (nnz, K), voxel indices (nnz, 4) with w = 1, forward affine 4x4,
+ * states (n, 2) u32 (updated in place), out (n, 8). method TGK_SEEDS_*. */
+int tgk_seeds_from_fod(tgk_runtime *rt, int32_t model, int32_t n_parameters, const float *fod, const float *voxels,
+                       uint32_t nnz, const float affine[16], uint32_t *states, int64_t n, int32_t method, float *out);
+
+/* ------------------------------------------------------ one-op drivers --- */
+
+/* Replaces local.evaluate (local.py:285-362): params (n, K), orientations
+ * (n, 3), out (n, 3) value, d/dtheta, (d/dphi)/sin(theta). */
+int tgk_evaluate(tgk_runtime *rt, int32_t model, int32_t n_parameters, const float *params, const float *orientations,
+                 int64_t n, float *out);
+/* Replaces algorithm.update_orientation (algorithm.py:701-790). */
+int tgk_update_orientation(tgk_runtime *rt, const tgk_algorithm *algorithm, uint32_t rng_seed, const float *params,
+                           const float *orientations, int64_t n, float *out);
+/* Replaces field.evaluate (field.py:144-197): out (n, K). */
+int tgk_field_evaluate(tgk_runtime *rt, const tgk_field *field, const float *points, int64_t n, float *out);
+/* Replaces domain.contains (domain.py:433-491) for mask domains: out (n,). */
+int tgk_domain_contains(tgk_runtime *rt, const tgk_domain *domain, const float *points, int64_t n, uint8_t *out);
+/* Closest boundary hit per segment (bvh.cl:83-131) of a mesh domain. */
+int tgk_bvh_segment_hit(tgk_runtime *rt, const tgk_domain *domain, const float *starts, const float *ends, int64_t n,
+                        uint8_t *found, uint32_t *hit, float *t);
+/* Evaluates the device elementary functions of the numeric contract:
+ * fn 0 sin, 1 cos, 2 atan2(x, y), 3 acos, 4 exp, 5 log, 6 sqrt, 7 x / y. */
+int tgk_math(tgk_runtime *rt, int32_t fn, const float *x, const float *y, int64_t n, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TGK_H */

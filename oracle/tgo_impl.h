@@ -1,0 +1,593 @@
+/*
+ * tgo_impl.h -- body of the host restatement, instantiated twice by tgo.c.
+ *
+ * TEST INFRASTRUCTURE ONLY (see tgo.h).
+ *
+ * Expected macros:
+ *   R          real type (float or double)
+ *   FN(x)      name mangling, tgo32_##x or tgo64_##x
+ *   K(c)       literal of type R written as in the reference (c##f or c)
+ *   PI_R       the reference's PI (core.cl:4) in FP32; exact pi in FP64
+ *   SQRT SINCOS ATAN2 ACOS EXP LOG FLOOR RINT FMAX FMIN FABS
+ *
+ * All vector expressions are restated per component, left to right, as the
+ * OpenCL C source reads, with no fused multiply-add (compile with
+ * -ffp-contract=off). The fourth (w) lane of the reference's float4 values is
+ * dropped: it is 1 for points and 0 for directions throughout the path.
+ */
+
+typedef struct {
+    R x, y, z;
+} FN(vec);
+#define VEC FN(vec)
+
+static inline VEC FN(mkvec)(R x, R y, R z)
+{
+    VEC v;
+    v.x = x;
+    v.y = y;
+    v.z = z;
+    return v;
+}
+
+/* OpenCL length() over xyz. */
+static inline R FN(length)(VEC v) { return SQRT((v.x * v.x + v.y * v.y) + v.z * v.z); }
+
+/* core.cl:142-153 */
+static inline void FN(cart2sph_one)(VEC c, R *azimuth, R *colatitude)
+{
+    const R radius = FN(length)(c);
+    if (radius <= K(0.0)) {
+        *azimuth = K(0.0);
+        *colatitude = K(0.0);
+        return;
+    }
+    R az = ATAN2(c.y, c.x);
+    if (az < K(0.0))
+        az += K(2.0) * PI_R;
+    *azimuth = az;
+    *colatitude = ACOS(FMIN(FMAX(c.z / radius, K(-1.0)), K(1.0)));
+}
+
+/* core.cl:101-109 */
+static inline VEC FN(exps2)(VEC p, VEC x, R t)
+{
+    const R n = FN(length)(x);
+    if (n <= K(0.0))
+        return p;
+    R s, c;
+    SINCOS(t * n, &s, &c);
+    const R q = s / n;
+    return FN(mkvec)(c * p.x + x.x * q, c * p.y + x.y * q, c * p.z + x.z * q);
+}
+
+/* core.cl:111-120 with SOFTMAX_SCALE = 100 (local.py:17) */
+static inline R FN(softmax)(R x, R k)
+{
+    if (x * k < K(30.0))
+        return LOG(1 + EXP(k * x)) / k;
+    return x;
+}
+
+static inline R FN(dsoftmax)(R x, R k) { return K(1.0) / (1 + EXP(-k * x)); }
+
+/* core.cl:240-245 */
+static inline R FN(randn)(uint32_t state[2])
+{
+    R n = (R)tgo_randu(state);
+    while (n <= K(0.0))
+        n = (R)tgo_randu(state);
+    return SQRT(K(-2.0) * LOG(n)) * COS_OF((2 * PI_R) * (R)tgo_randu(state));
+}
+
+/* spharm.cl:216-413: real symmetric SH basis l <= 8 in MRtrix order, its
+ * colatitude derivative and its azimuth derivative divided by sin(theta),
+ * restated index by index. */
+static void FN(sh45_one)(R phi, R theta, R ylm[45], R dp[45], R dt[45])
+{
+    R st, ct, sp, cp, s2p, c2p;
+    SINCOS(theta, &st, &ct);
+    SINCOS(phi, &sp, &cp);
+    SINCOS(K(2.0) * phi, &s2p, &c2p);
+    const R s3p = 2 * cp * s2p - sp, c3p = 2 * cp * c2p - cp;
+    const R s4p = 2 * cp * s3p - s2p, c4p = 2 * cp * c3p - c2p;
+    const R s5p = 2 * cp * s4p - s3p, c5p = 2 * cp * c4p - c3p;
+    const R s6p = 2 * cp * s5p - s4p, c6p = 2 * cp * c5p - c4p;
+    const R s7p = 2 * cp * s6p - s5p, c7p = 2 * cp * c6p - c5p;
+    const R s8p = 2 * cp * s7p - s6p, c8p = 2 * cp * c7p - c6p;
+
+    const R st2 = st * st, st3 = st2 * st, st4 = st3 * st, st5 = st4 * st;
+    const R st6 = st5 * st, st7 = st6 * st, st8 = st7 * st;
+    const R ct2 = ct * ct, ct3 = ct2 * ct, ct4 = ct3 * ct, ct5 = ct4 * ct;
+    const R ct6 = ct5 * ct, ct7 = ct6 * ct, ct8 = ct7 * ct;
+    const R stct = st * ct;
+
+    ylm[0] = K(0.28209479177387814);
+    ylm[1] = K(0.54627421529) * st2 * s2p;
+    ylm[2] = K(-1.0925484305920792) * stct * sp;
+    ylm[3] = K(0.31539156525252005) * (K(3.0) * ct2 - K(1.0));
+    ylm[4] = K(-1.0925484305920792) * stct * cp;
+    ylm[5] = K(0.54627421529) * st2 * c2p;
+    ylm[6] = K(0.62583573544) * st4 * s4p;
+    ylm[7] = K(-1.77013076978) * st3 * ct * s3p;
+    ylm[8] = K(0.47308734787) * st2 * (K(7.0) * ct2 - K(1.0)) * s2p;
+    ylm[9] = K(-0.66904654355) * st * (K(7.0) * ct2 - K(3.0)) * ct * sp;
+    ylm[10] = K(0.10578554691520431) * (K(35.0) * ct2 * ct2 - K(30.0) * ct2 + K(3.0));
+    ylm[11] = K(-0.66904654355) * st * (K(7.0) * ct2 - K(3.0)) * ct * cp;
+    ylm[12] = K(0.47308734787) * st2 * (K(7.0) * ct2 - K(1.0)) * c2p;
+    ylm[13] = K(-1.77013076978) * st3 * ct * c3p;
+    ylm[14] = K(0.62583573544) * st4 * c4p;
+    ylm[15] = K(0.68318410519) * st6 * s6p;
+    ylm[16] = K(-2.36661916223) * st5 * ct * s5p;
+    ylm[17] = K(0.50456490072) * st4 * (K(11.0) * ct2 - K(1.0)) * s4p;
+    ylm[18] = K(-0.92120525951) * st3 * (K(11.0) * ct3 - K(3.0) * ct) * s3p;
+    ylm[19] = K(0.46060262975) * st2 * (K(33.0) * ct4 - K(18.0) * ct2 + K(1.0)) * s2p;
+    ylm[20] = K(-0.58262136251) * st * (K(33.0) * ct5 - K(30.0) * ct3 + K(5.0) * ct) * sp;
+    ylm[21] = K(0.06356920226) * (K(231.0) * ct6 - K(315.0) * ct4 + K(105.0) * ct2 - K(5.0));
+    ylm[22] = K(-0.58262136251) * st * (K(33.0) * ct5 - K(30.0) * ct3 + K(5.0) * ct) * cp;
+    ylm[23] = K(0.46060262975) * st2 * (K(33.0) * ct4 - K(18.0) * ct2 + K(1.0)) * c2p;
+    ylm[24] = K(-0.92120525951) * st3 * (K(11.0) * ct3 - K(3.0) * ct) * c3p;
+    ylm[25] = K(0.50456490072) * st4 * (K(11.0) * ct2 - K(1.0)) * c4p;
+    ylm[26] = K(-2.36661916223) * st5 * ct * c5p;
+    ylm[27] = K(0.68318410519) * st6 * c6p;
+    ylm[28] = K(0.72892666017) * st8 * s8p;
+    ylm[29] = K(-2.9157066407) * st7 * ct * s7p;
+    ylm[30] = K(0.53233276606) * st6 * (K(15.0) * ct2 - K(1.0)) * s6p;
+    ylm[31] = K(-3.4499106221) * st5 * (K(5.0) * ct3 - ct) * s5p;
+    ylm[32] = K(0.47841652475) * st4 * (K(65.0) * ct4 - K(26.0) * ct2 + K(1.0)) * s4p;
+    ylm[33] = K(-1.2352661553) * st3 * (K(39.0) * ct5 - K(26.0) * ct3 + 3 * ct) * s3p;
+    ylm[34] = K(0.45615225843) * st2 * (K(143.0) * ct6 - K(143.0) * ct4 + K(33.0) * ct2 - K(1.0)) * s2p;
+    ylm[35] = K(-0.10904124589) * st * (K(715.0) * ct7 - K(1001.0) * ct5 + K(385.0) * ct3 - K(35.0) * ct) * sp;
+    ylm[36] = K(0.00908677049) * (K(6435.0) * ct8 - K(12012.0) * ct6 + K(6930.0) * ct4 - K(1260.0) * ct2 + K(35.0));
+    ylm[37] = K(-0.10904124589) * st * (K(715.0) * ct7 - K(1001.0) * ct5 + K(385.0) * ct3 - K(35.0) * ct) * cp;
+    ylm[38] = K(0.45615225843) * st2 * (K(143.0) * ct6 - K(143.0) * ct4 + K(33.0) * ct2 - K(1.0)) * c2p;
+    ylm[39] = K(-1.2352661553) * st3 * (K(39.0) * ct5 - K(26.0) * ct3 + 3 * ct) * c3p;
+    ylm[40] = K(0.47841652475) * st4 * (K(65.0) * ct4 - K(26.0) * ct2 + K(1.0)) * c4p;
+    ylm[41] = K(-3.4499106221) * st5 * (K(5.0) * ct3 - ct) * c5p;
+    ylm[42] = K(0.53233276606) * st6 * (K(15.0) * ct2 - K(1.0)) * c6p;
+    ylm[43] = K(-2.9157066407) * st7 * ct * c7p;
+    ylm[44] = K(0.72892666017) * st8 * c8p;
+
+    dp[0] = K(0.0);
+    dp[1] = K(0.54627421529) * st * K(2.0) * c2p;
+    dp[2] = K(-1.0925484305920792) * ct * cp;
+    dp[3] = K(0.0);
+    dp[4] = K(1.0925484305920792) * ct * sp;
+    dp[5] = K(-0.54627421529) * st * K(2.0) * s2p;
+    dp[6] = K(0.62583573544) * st3 * K(4.0) * c4p;
+    dp[7] = K(-1.77013076978) * st2 * ct * K(3.0) * c3p;
+    dp[8] = K(0.47308734787) * st * (K(7.0) * ct2 - K(1.0)) * K(2.0) * c2p;
+    dp[9] = K(-0.66904654355) * ct * (K(7.0) * ct2 - K(3.0)) * cp;
+    dp[10] = K(0.0);
+    dp[11] = K(0.66904654355) * ct * (K(7.0) * ct2 - K(3.0)) * sp;
+    dp[12] = K(-0.47308734787) * st * (K(7.0) * ct2 - K(1.0)) * K(2.0) * s2p;
+    dp[13] = K(1.77013076978) * st2 * ct * K(3.0) * s3p;
+    dp[14] = K(-0.62583573544) * st3 * K(4.0) * s4p;
+    dp[15] = K(0.68318410519) * st5 * K(6.0) * c6p;
+    dp[16] = K(-2.36661916223) * st4 * ct * K(5.0) * c5p;
+    dp[17] = K(0.50456490072) * st3 * (K(11.0) * ct2 - K(1.0)) * K(4.0) * c4p;
+    dp[18] = K(-0.92120525951) * st2 * (K(11.0) * ct3 - K(3.0) * ct) * K(3.0) * c3p;
+    dp[19] = K(0.46060262975) * st * (K(33.0) * ct4 - K(18.0) * ct2 + K(1.0)) * K(2.0) * c2p;
+    dp[20] = K(-0.58262136251) * (K(33.0) * ct5 - K(30.0) * ct3 + K(5.0) * ct) * cp;
+    dp[21] = K(0.0);
+    dp[22] = K(0.58262136251) * (K(33.0) * ct5 - K(30.0) * ct3 + K(5.0) * ct) * sp;
+    dp[23] = K(-0.46060262975) * st * (K(33.0) * ct4 - K(18.0) * ct2 + K(1.0)) * K(2.0) * s2p;
+    dp[24] = K(0.92120525951) * st2 * (K(11.0) * ct3 - K(3.0) * ct) * K(3.0) * s3p;
+    dp[25] = K(-0.50456490072) * st3 * (K(11.0) * ct2 - K(1.0)) * K(4.0) * s4p;
+    dp[26] = K(2.36661916223) * st4 * ct * K(5.0) * s5p;
+    dp[27] = K(-0.68318410519) * st5 * K(6.0) * s6p;
+    dp[28] = K(0.72892666017) * st7 * K(8.0) * c8p;
+    dp[29] = K(-2.9157066407) * st6 * ct * K(7.0) * c7p;
+    dp[30] = K(0.53233276606) * st5 * (K(15.0) * ct2 - K(1.0)) * K(6.0) * c6p;
+    dp[31] = K(-3.4499106221) * st4 * (K(5.0) * ct3 - ct) * K(5.0) * c5p;
+    dp[32] = K(0.47841652475) * st3 * (K(65.0) * ct4 - K(26.0) * ct2 + K(1.0)) * K(4.0) * c4p;
+    dp[33] = K(-1.2352661553) * st2 * (K(39.0) * ct5 - K(26.0) * ct3 + 3 * ct) * K(3.0) * c3p;
+    dp[34] = K(0.45615225843) * st * (K(143.0) * ct6 - K(143.0) * ct4 + K(33.0) * ct2 - K(1.0)) * K(2.0) * c2p;
+    dp[35] = K(-0.10904124589) * (K(715.0) * ct7 - K(1001.0) * ct5 + K(385.0) * ct3 - K(35.0) * ct) * cp;
+    dp[36] = K(0.0);
+    dp[37] = K(0.10904124589) * (K(715.0) * ct7 - K(1001.0) * ct5 + K(385.0) * ct3 - K(35.0) * ct) * sp;
+    dp[38] = K(-0.45615225843) * st * (K(143.0) * ct6 - K(143.0) * ct4 + K(33.0) * ct2 - K(1.0)) * K(2.0) * s2p;
+    dp[39] = K(1.2352661553) * st2 * (K(39.0) * ct5 - K(26.0) * ct3 + 3 * ct) * K(3.0) * s3p;
+    dp[40] = K(-0.47841652475) * st3 * (K(65.0) * ct4 - K(26.0) * ct2 + K(1.0)) * K(4.0) * s4p;
+    dp[41] = K(3.4499106221) * st4 * (K(5.0) * ct3 - ct) * K(5.0) * s5p;
+    dp[42] = K(-0.53233276606) * st5 * (K(15.0) * ct2 - K(1.0)) * K(6.0) * s6p;
+    dp[43] = K(2.9157066407) * st6 * ct * K(7.0) * s7p;
+    dp[44] = K(-0.72892666017) * st7 * K(8.0) * s8p;
+
+    dt[0] = K(0.0);
+    dt[1] = K(0.54627421529) * K(2.0) * stct * s2p;
+    dt[2] = K(-1.0925484305920792) * (ct2 - st2) * sp;
+    dt[3] = K(-0.31539156525252005) * K(6.0) * ct * st;
+    dt[4] = K(-1.0925484305920792) * (ct2 - st2) * cp;
+    dt[5] = K(0.54627421529) * K(2.0) * stct * c2p;
+    dt[6] = K(0.62583573544) * K(4.0) * st3 * ct * s4p;
+    dt[7] = K(-1.77013076978) * (K(3.0) * st2 * ct2 - st4) * s3p;
+    dt[8] = K(0.47308734787) * (K(2.0) * stct * (K(7.0) * ct2 - K(1.0)) - K(14.0) * st3 * ct) * s2p;
+    dt[9] = K(-0.66904654355) * ((ct2 - st2) * (K(7.0) * ct2 - K(3.0)) - (K(14.0) * st2 * ct2)) * sp;
+    dt[10] = K(0.10578554691520431) * (K(-35.0) * K(4.0) * ct2 * stct + K(30.0) * K(2.0) * stct);
+    dt[11] = K(-0.66904654355) * ((ct2 - st2) * (K(7.0) * ct2 - K(3.0)) - (K(14.0) * st2 * ct2)) * cp;
+    dt[12] = K(0.47308734787) * (K(2.0) * stct * (K(7.0) * ct2 - K(1.0)) - K(14.0) * st3 * ct) * c2p;
+    dt[13] = K(-1.77013076978) * (K(3.0) * st2 * ct2 - st4) * c3p;
+    dt[14] = K(0.62583573544) * K(4.0) * st3 * ct * c4p;
+    dt[15] = K(0.68318410519) * K(6.0) * st5 * ct * s6p;
+    dt[16] = K(-2.36661916223) * (K(5.0) * st4 * ct2 - st6) * s5p;
+    dt[17] = K(0.50456490072) * (K(4.0) * st3 * ct * (K(11.0) * ct2 - K(1.0)) - K(22.0) * st5 * ct) * s4p;
+    dt[18] = K(-0.92120525951) * (K(3.0) * st2 * ct * (K(11.0) * ct3 - K(3.0) * ct) - st4 * (K(33.0) * ct2 - K(3.0))) * s3p;
+    dt[19] = K(0.46060262975) * (K(2.0) * st * ct * (K(33.0) * ct4 - K(18.0) * ct2 + K(1.0)) - st3 * (K(33.0) * K(4.0) * ct3 - K(36.0) * ct)) * s2p;
+    dt[20] = K(-0.58262136251) * (ct * (K(33.0) * ct5 - K(30.0) * ct3 + K(5.0) * ct) - st2 * (K(33.0) * K(5.0) * ct4 - K(90.0) * ct2 + K(5.0))) * sp;
+    dt[21] = K(-0.06356920226) * (K(1386.0) * ct5 - K(1260.0) * ct3 + K(210.0) * ct) * st;
+    dt[22] = K(-0.58262136251) * (ct * (K(33.0) * ct5 - K(30.0) * ct3 + K(5.0) * ct) - st2 * (K(33.0) * K(5.0) * ct4 - K(90.0) * ct2 + K(5.0))) * cp;
+    dt[23] = K(0.46060262975) * (K(2.0) * st * ct * (K(33.0) * ct4 - K(18.0) * ct2 + K(1.0)) - st3 * (K(33.0) * K(4.0) * ct3 - K(36.0) * ct)) * c2p;
+    dt[24] = K(-0.92120525951) * (K(3.0) * st2 * ct * (K(11.0) * ct3 - K(3.0) * ct) - st4 * (K(33.0) * ct2 - K(3.0))) * c3p;
+    dt[25] = K(0.50456490072) * (K(4.0) * st3 * ct * (K(11.0) * ct2 - K(1.0)) - K(22.0) * st5 * ct) * c4p;
+    dt[26] = K(-2.36661916223) * (K(5.0) * st4 * ct2 - st6) * c5p;
+    dt[27] = K(0.68318410519) * K(6.0) * st5 * ct * c6p;
+    dt[28] = K(0.72892666017) * K(8.0) * st7 * ct * s8p;
+    dt[29] = K(-2.9157066407) * (K(7.0) * st6 * ct2 - st8) * s7p;
+    dt[30] = K(0.53233276606) * (K(6.0) * st5 * ct * (K(15.0) * ct2 - K(1.0)) - st7 * K(30.0) * ct) * s6p;
+    dt[31] = K(-3.4499106221) * (K(5.0) * st4 * ct * (K(5.0) * ct3 - ct) - st6 * (K(15.0) * ct2 - K(1.0))) * s5p;
+    dt[32] = K(0.47841652475) * (K(4.0) * st3 * ct * (K(65.0) * ct4 - K(26.0) * ct2 + K(1.0)) - st5 * (K(260.0) * ct3 - K(52.0) * ct)) * s4p;
+    dt[33] = K(-1.2352661553) * (K(3.0) * st2 * ct * (K(39.0) * ct5 - K(26.0) * ct3 + 3 * ct) - st4 * (K(195.0) * ct4 - K(78.0) * ct2 + K(3.0))) * s3p;
+    dt[34] = K(0.45615225843) * (K(2.0) * st * ct * (K(143.0) * ct6 - K(143.0) * ct4 + K(33.0) * ct2 - K(1.0)) - st3 * (K(858.0) * ct5 - K(572.0) * ct3 + K(66.0) * ct)) * s2p;
+    dt[35] = K(-0.10904124589) * (ct * (K(715.0) * ct7 - K(1001.0) * ct5 + K(385.0) * ct3 - K(35.0) * ct) - st2 * (K(5005.0) * ct6 - K(5005.0) * ct4 + K(1155.0) * ct2 - K(35.0))) * sp;
+    dt[36] = K(-0.00908677049) * (K(8.0) * K(6435.0) * ct7 - K(6.0) * K(12012.0) * ct5 + K(4.0) * K(6930.0) * ct3 - K(2.0) * K(1260.0) * ct) * st;
+    dt[37] = K(-0.10904124589) * (ct * (K(715.0) * ct7 - K(1001.0) * ct5 + K(385.0) * ct3 - K(35.0) * ct) - st2 * (K(5005.0) * ct6 - K(5005.0) * ct4 + K(1155.0) * ct2 - K(35.0))) * cp;
+    dt[38] = K(0.45615225843) * (K(2.0) * st * ct * (K(143.0) * ct6 - K(143.0) * ct4 + K(33.0) * ct2 - K(1.0)) - st3 * (K(858.0) * ct5 - K(572.0) * ct3 + K(66.0) * ct)) * c2p;
+    dt[39] = K(-1.2352661553) * (K(3.0) * st2 * ct * (K(39.0) * ct5 - K(26.0) * ct3 + 3 * ct) - st4 * (K(195.0) * ct4 - K(78.0) * ct2 + K(3.0))) * c3p;
+    dt[40] = K(0.47841652475) * (K(4.0) * st3 * ct * (K(65.0) * ct4 - K(26.0) * ct2 + K(1.0)) - st5 * (K(260.0) * ct3 - K(52.0) * ct)) * c4p;
+    dt[41] = K(-3.4499106221) * (K(5.0) * st4 * ct * (K(5.0) * ct3 - ct) - st6 * (K(15.0) * ct2 - K(1.0))) * c5p;
+    dt[42] = K(0.53233276606) * (K(6.0) * st5 * ct * (K(15.0) * ct2 - K(1.0)) - st7 * K(30.0) * ct) * c6p;
+    dt[43] = K(-2.9157066407) * (K(7.0) * st6 * ct2 - st8) * c7p;
+    dt[44] = K(0.72892666017) * K(8.0) * st7 * ct * c8p;
+}
+
+typedef struct {
+    R value, dtheta, dphi;
+} FN(model_value);
+#define MV FN(model_value)
+
+/* local.py:91-120 (SH, K <= 45) */
+static inline MV FN(sh_model)(const R *model, int n_parameters, VEC o)
+{
+    R ylm[45], dp[45], dt[45];
+    R az, col;
+    FN(cart2sph_one)(o, &az, &col);
+    FN(sh45_one)(az, col, ylm, dp, dt);
+    MV r;
+    r.value = K(0.0);
+    r.dtheta = K(0.0);
+    r.dphi = K(0.0);
+    for (int i = 0; i < n_parameters; i++) {
+        const R c = model[i];
+        r.value += c * ylm[i];
+        r.dtheta += c * dt[i];
+        r.dphi += c * dp[i];
+    }
+    const R d = FN(dsoftmax)(r.value, K(100.0));
+    r.value = FN(softmax)(r.value, K(100.0));
+    r.dtheta = r.dtheta * d;
+    r.dphi = r.dphi * d;
+    return r;
+}
+
+/* local.py:212-230 (DTI, coefficients Dxx Dyy Dzz Dxy Dxz Dyz) */
+static inline MV FN(dti_model)(const R *m, VEC o)
+{
+    const R t0 = (m[0] * o.x + m[3] * o.y) + m[4] * o.z;
+    const R t1 = (m[3] * o.x + m[1] * o.y) + m[5] * o.z;
+    const R t2 = (m[4] * o.x + m[5] * o.y) + m[2] * o.z;
+    R az, col, sp, cp, st, ct;
+    FN(cart2sph_one)(o, &az, &col);
+    SINCOS(az, &sp, &cp);
+    SINCOS(col, &st, &ct);
+    const R etx = ct * cp, ety = ct * sp, etz = -st;
+    const R epx = -sp, epy = cp, epz = K(0.0);
+    MV r;
+    r.value = (o.x * t0 + o.y * t1) + o.z * t2;
+    r.dtheta = K(2.0) * ((t0 * etx + t1 * ety) + t2 * etz);
+    r.dphi = K(2.0) * ((t0 * epx + t1 * epy) + t2 * epz);
+    return r;
+}
+
+static inline MV FN(model_evaluate)(int model, int n_parameters, const R *params, VEC o)
+{
+    if (model == TGO_MODEL_DTI)
+        return FN(dti_model)(params, o);
+    return FN(sh_model)(params, n_parameters, o);
+}
+
+/* algorithm.py:83-112 (Transport) and 198-228 (Diffusion). */
+static inline VEC FN(update_one)(const tgo_algorithm *a, const R *params, VEC o, uint32_t rng[2])
+{
+    const MV mv = FN(model_evaluate)(a->model, a->n_parameters, params, o);
+    R az, col, sp, cp, st, ct;
+    FN(cart2sph_one)(o, &az, &col);
+    SINCOS(az, &sp, &cp);
+    SINCOS(col, &st, &ct);
+    const R etx = ct * cp, ety = ct * sp, etz = -st;
+    const R epx = -sp, epy = cp, epz = K(0.0);
+    const R m = FMAX(mv.value, K(1e-6));
+    const R dx = (mv.dtheta * etx + mv.dphi * epx) / m;
+    const R dy = (mv.dtheta * ety + mv.dphi * epy) / m;
+    const R dz = (mv.dtheta * etz + mv.dphi * epz) / m;
+    const R step = (R)a->step_size;
+    const R gamma = (R)a->gamma;
+    const R scale_drift = gamma * step;
+    if (a->algorithm == TGO_ALGO_TRANSPORT) {
+        const VEC tangent = FN(mkvec)(scale_drift * dx, scale_drift * dy, scale_drift * dz);
+        return FN(exps2)(o, tangent, K(1.0));
+    }
+    const R nv = (R)a->noise_variance;
+    R nx = K(0.0), ny = K(0.0), nz = K(0.0);
+    if (nv > K(0.0)) {
+        const R r1 = FN(randn)(rng); /* left operand first, as clang orders it */
+        const R r2 = FN(randn)(rng);
+        nx = r1 * etx + r2 * epx;
+        ny = r1 * ety + r2 * epy;
+        nz = r1 * etz + r2 * epz;
+    }
+    const R scale_noise = SQRT(nv * gamma * step);
+    const VEC tangent = FN(mkvec)(scale_drift * dx + scale_noise * nx, scale_drift * dy + scale_noise * ny,
+                                  scale_drift * dz + scale_noise * nz);
+    return FN(exps2)(o, tangent, K(1.0));
+}
+
+/* World to voxel: row dots of the float32 inverse affine with (p, 1)
+ * (field.py:91-96, domain.py:361-366). */
+static inline void FN(to_voxel)(const float A[16], VEC p, R v[3])
+{
+    for (int i = 0; i < 3; i++)
+        v[i] = (((R)A[4 * i] * p.x + (R)A[4 * i + 1] * p.y) + (R)A[4 * i + 2] * p.z) + (R)A[4 * i + 3];
+}
+
+/* field.py:88-114: nearest voxel by rint (ties to even); false outside. */
+static inline int FN(field_index)(const tgo_scene *s, VEC p, int64_t *base, R *margin)
+{
+    R v[3];
+    FN(to_voxel)(s->field_inverse_affine, p, v);
+    R r[3];
+    for (int i = 0; i < 3; i++) {
+        r[i] = RINT(v[i]);
+        if (margin) {
+            const R d = K(0.5) - fabs((double)(v[i] - r[i]));
+            if (d < *margin)
+                *margin = d;
+        }
+    }
+    for (int i = 0; i < 3; i++)
+        if (!(r[i] >= K(0.0) && r[i] < (R)s->field_dims[i]))
+            return 0;
+    *base = (((int64_t)r[0] * s->field_dims[1] + (int64_t)r[1]) * s->field_dims[2] + (int64_t)r[2]) *
+            (int64_t)s->field_dims[3];
+    return 1;
+}
+
+/* domain.py:358-382: half-open cells via floor(v + 0.5). */
+static inline int FN(mask_contains)(const tgo_scene *s, VEC p, R *margin)
+{
+    R v[3];
+    FN(to_voxel)(s->mask_inverse_affine, p, v);
+    R r[3];
+    for (int i = 0; i < 3; i++) {
+        r[i] = FLOOR(v[i] + K(0.5));
+        if (margin) {
+            const R d = K(0.5) - fabs((double)(v[i] - r[i]));
+            if (d < *margin)
+                *margin = d;
+        }
+    }
+    for (int i = 0; i < 3; i++)
+        if (!(r[i] >= K(0.0) && r[i] < (R)s->mask_dims[i]))
+            return 0;
+    const int64_t idx = (int64_t)r[0] * ((int64_t)s->mask_dims[1] * s->mask_dims[2]) +
+                        (int64_t)r[1] * s->mask_dims[2] + (int64_t)r[2];
+    return s->mask[idx] != 0;
+}
+
+/* bvh.cl:16-46 */
+static inline int FN(bvh_bounds)(const tgo_scene *s, uint32_t node, const R start[3], const R dir[3], R max_t)
+{
+    R t_min = K(0.0), t_max = max_t;
+    for (int axis = 0; axis < 3; axis++) {
+        const R lo = (R)s->bvh_min[4 * node + axis];
+        const R hi = (R)s->bvh_max[4 * node + axis];
+        if (FABS(dir[axis]) < K(1e-12)) {
+            if (start[axis] < lo || start[axis] > hi)
+                return 0;
+        } else {
+            const R inv = K(1.0) / dir[axis];
+            R near_t = (lo - start[axis]) * inv;
+            R far_t = (hi - start[axis]) * inv;
+            if (near_t > far_t) {
+                const R tmp = near_t;
+                near_t = far_t;
+                far_t = tmp;
+            }
+            t_min = FMAX(t_min, near_t);
+            t_max = FMIN(t_max, far_t);
+            if (t_min > t_max)
+                return 0;
+        }
+    }
+    return 1;
+}
+
+/* bvh.cl:48-80 (Moller-Trumbore) */
+static inline int FN(bvh_triangle)(const tgo_scene *s, uint32_t tri, const R start[3], const R dir[3], R max_t,
+                                   R *hit_t)
+{
+    const uint32_t *ix = s->triangles + 4 * (size_t)tri;
+    R a[3], b[3], c[3];
+    for (int i = 0; i < 3; i++) {
+        a[i] = (R)s->vertices[4 * (size_t)ix[0] + i];
+        b[i] = (R)s->vertices[4 * (size_t)ix[1] + i];
+        c[i] = (R)s->vertices[4 * (size_t)ix[2] + i];
+    }
+    const R e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const R e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    const R p[3] = {dir[1] * e2[2] - dir[2] * e2[1], dir[2] * e2[0] - dir[0] * e2[2], dir[0] * e2[1] - dir[1] * e2[0]};
+    const R det = (e1[0] * p[0] + e1[1] * p[1]) + e1[2] * p[2];
+    if (FABS(det) < K(1e-10))
+        return 0;
+    const R inv = K(1.0) / det;
+    const R off[3] = {start[0] - a[0], start[1] - a[1], start[2] - a[2]};
+    const R u = ((off[0] * p[0] + off[1] * p[1]) + off[2] * p[2]) * inv;
+    if (u < K(-1e-6) || u > K(1.0) + K(1e-6))
+        return 0;
+    const R q[3] = {off[1] * e1[2] - off[2] * e1[1], off[2] * e1[0] - off[0] * e1[2], off[0] * e1[1] - off[1] * e1[0]};
+    const R v = ((dir[0] * q[0] + dir[1] * q[1]) + dir[2] * q[2]) * inv;
+    if (v < K(-1e-6) || u + v > K(1.0) + K(1e-6))
+        return 0;
+    const R t = ((e2[0] * q[0] + e2[1] * q[1]) + e2[2] * q[2]) * inv;
+    if (t <= K(1e-6) || t > max_t)
+        return 0;
+    *hit_t = t;
+    return 1;
+}
+
+/* bvh.cl:83-131: closest hit with t in (1e-6, 1]; a later-visited triangle
+ * at an equal t replaces the earlier one; left child visited first. */
+static inline int FN(bvh_hit)(const tgo_scene *s, VEC start, VEC end, uint32_t *hit, R *hit_t)
+{
+    const R st[3] = {start.x, start.y, start.z};
+    const R dir[3] = {end.x - start.x, end.y - start.y, end.z - start.z};
+    int32_t stack[64];
+    uint32_t size = 1;
+    stack[0] = 0;
+    *hit_t = K(1.0);
+    *hit = 0;
+    int found = 0;
+    while (size > 0) {
+        const uint32_t node = (uint32_t)stack[--size];
+        if (!FN(bvh_bounds)(s, node, st, dir, *hit_t))
+            continue;
+        const int32_t *nd = s->bvh_nodes + 4 * (size_t)node;
+        if (nd[3] > 0) {
+            for (int i = 0; i < nd[3]; i++) {
+                const uint32_t tri = s->bvh_order[nd[2] + i];
+                R t;
+                if (FN(bvh_triangle)(s, tri, st, dir, *hit_t, &t)) {
+                    *hit_t = t;
+                    *hit = tri;
+                    found = 1;
+                }
+            }
+        } else {
+            if (size + 2 > 64)
+                return found;
+            stack[size++] = nd[1];
+            stack[size++] = nd[0];
+        }
+    }
+    return found;
+}
+
+/* tracking.py:112-148 for one seed. Mesh domains: the loop's domain test is
+ * "the segment just traversed crossed no boundary triangle", checked at the
+ * top of each iteration after the first. */
+static void FN(track_one)(const tgo_scene *s, const tgo_algorithm *a, const float *seed, uint32_t rng_seed,
+                          uint32_t stream_id, uint32_t n_points, float save_at, uint32_t *length_out,
+                          uint32_t *steps_out, uint32_t *hit_out, R *points, R *margins)
+{
+    VEC point = FN(mkvec)((R)seed[0], (R)seed[1], (R)seed[2]);
+    VEC orientation = FN(mkvec)((R)seed[4], (R)seed[5], (R)seed[6]);
+    VEC previous = point;
+    uint32_t rng[2];
+    tgo_rng_init(rng_seed, stream_id, rng);
+    const int K_ = a->n_parameters;
+    R params[128];
+    const R step = (R)a->step_size;
+    const R save = (R)save_at;
+    R margin = K(0.5);
+
+    points[0] = point.x;
+    points[1] = point.y;
+    points[2] = point.z;
+    if (margins)
+        margins[0] = margin;
+    uint32_t n = 1, steps = 0, hit = 0xffffffffu;
+    R time = K(0.0);
+    int first = 1;
+    while (n < n_points) {
+        if (s->domain_kind == TGO_DOMAIN_MASK) {
+            if (!FN(mask_contains)(s, point, margins ? &margin : 0))
+                break;
+        } else if (!first) {
+            uint32_t tri;
+            R t;
+            if (FN(bvh_hit)(s, previous, point, &tri, &t)) {
+                hit = tri;
+                break;
+            }
+        }
+        int64_t base;
+        if (!FN(field_index)(s, point, &base, margins ? &margin : 0))
+            break;
+        for (int i = 0; i < K_; i++)
+            params[i] = (R)s->field[base + i];
+        orientation = FN(update_one)(a, params, orientation, rng);
+        steps++;
+        previous = point;
+        point.x += step * orientation.x;
+        point.y += step * orientation.y;
+        point.z += step * orientation.z;
+        first = 0;
+        time += step;
+        if (time >= save) {
+            time -= save;
+            points[3 * n + 0] = point.x - time * orientation.x;
+            points[3 * n + 1] = point.y - time * orientation.y;
+            points[3 * n + 2] = point.z - time * orientation.z;
+            if (margins)
+                margins[n] = margin;
+            n++;
+        }
+    }
+    *length_out = n;
+    *steps_out = steps;
+    *hit_out = hit;
+}
+
+void FN(track)(const tgo_scene *s, const tgo_algorithm *a, const float *seeds, int64_t n, const uint32_t *seed_rng,
+               const uint32_t *seed_stream, uint32_t n_points, float save_at, uint32_t *lengths, uint32_t *steps,
+               uint32_t *hits, R *points TRACK_EXTRA_PARAM, int n_threads)
+{
+#pragma omp parallel for schedule(dynamic, 16) num_threads(n_threads > 0 ? n_threads : 1)
+    for (int64_t g = 0; g < n; g++)
+        FN(track_one)(s, a, seeds + 8 * g, seed_rng[g], seed_stream[g], n_points, save_at, lengths + g, steps + g,
+                      hits + g, points + (size_t)3 * n_points * (size_t)g, TRACK_MARGINS(g));
+}
+
+void FN(sh45)(const R *phi, const R *theta, int64_t n, R *ylm, R *dp, R *dt)
+{
+    for (int64_t i = 0; i < n; i++)
+        FN(sh45_one)(phi[i], theta[i], ylm + 45 * i, dp + 45 * i, dt + 45 * i);
+}
+
+void FN(evaluate)(int model, int n_parameters, const R *params, const R *orientations, int64_t n, R *out)
+{
+    for (int64_t i = 0; i < n; i++) {
+        const VEC o = FN(mkvec)(orientations[3 * i], orientations[3 * i + 1], orientations[3 * i + 2]);
+        const MV mv = FN(model_evaluate)(model, n_parameters, params + (size_t)n_parameters * i, o);
+        out[3 * i] = mv.value;
+        out[3 * i + 1] = mv.dtheta;
+        out[3 * i + 2] = mv.dphi;
+    }
+}
+
+void FN(update)(const tgo_algorithm *a, uint32_t rng_seed, const R *params, const R *orientations, int64_t n,
+                     R *out)
+{
+    for (int64_t i = 0; i < n; i++) {
+        uint32_t rng[2];
+        tgo_rng_init(rng_seed, (uint32_t)i, rng);
+        const VEC o = FN(mkvec)(orientations[3 * i], orientations[3 * i + 1], orientations[3 * i + 2]);
+        const VEC r = FN(update_one)(a, params + (size_t)a->n_parameters * i, o, rng);
+        out[3 * i] = r.x;
+        out[3 * i + 1] = r.y;
+        out[3 * i + 2] = r.z;
+    }
+}
+
+#undef VEC
+#undef MV

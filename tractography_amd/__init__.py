@@ -1,0 +1,167 @@
+"""MI355X-native streamline tracking, a drop-in for the OpenCL backend of the
+reference ``tractography`` package on its tracking path.
+
+``import tractography_amd as tg`` then ``tg.tractogram(...)``,
+``tg.algorithm.Transport/Diffusion``, ``tg.local.*``, ``tg.field.Image``,
+``tg.domain.Image`` and ``tg.seeds.*`` with the reference's signatures. The
+device work runs in hand-written HIP kernels (libtgk.so) on gfx950.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import numpy.typing as npt
+
+from . import algorithm as algorithm_module
+from . import core, local, seeds, tracking, utils
+from . import domain as domain_module
+from . import field as field_module
+from .algorithm import BaseAlgorithm
+from .image import Nifti1Image, Streamlines, Tractogram, is_image
+from .runtime import HipRuntime
+
+algorithm = algorithm_module
+domain = domain_module
+field = field_module
+seeds_module = seeds
+OpenCLRuntime = HipRuntime
+
+
+def _validate_algorithm(tracking_algorithm: BaseAlgorithm) -> None:
+    if not isinstance(tracking_algorithm, BaseAlgorithm):
+        raise TypeError("algorithm must be a BaseAlgorithm instance.")
+
+
+def tractogram(
+    field,
+    seeds: npt.NDArray[np.float32] | int,
+    domain=None,
+    algorithm: BaseAlgorithm | None = None,
+    *,
+    save_at: float = 1.0,
+    maximum_streamline_length: float = 300.0,
+    minimum_streamline_length: float = 0.0,
+    batch_size: int = 50000,
+    runtime: HipRuntime | None = None,
+) -> Tractogram:
+    """Generate streamlines from a 4-D image or a prepared field
+    (__init__.py:369-540).
+
+    ``field`` is a 4-D image of 6 DTI or 45 SH coefficients per voxel (the
+    local model and an image field are built from it) or a ``BaseField``.
+    ``seeds`` is an (n, 8) array or a number of boundary seeds to sample.
+    ``domain`` defaults to the nonzero voxels of an image field. Streamlines
+    with ceil(minimum_streamline_length / save_at) points or fewer are
+    dropped. ``batch_size`` only sets which seeds share a noise-stream draw
+    (one ``algorithm.get_rng_seed()`` per batch, as the reference's batch
+    loop); all seeds are tracked in one device call.
+    """
+    if not np.isfinite(save_at) or save_at <= 0:
+        raise ValueError("save_at must be finite and positive.")
+    if not np.isfinite(maximum_streamline_length) or maximum_streamline_length < 0:
+        raise ValueError("maximum_streamline_length must be finite and non-negative.")
+    if not np.isfinite(minimum_streamline_length) or minimum_streamline_length < 0:
+        raise ValueError("minimum_streamline_length must be finite and non-negative.")
+    if minimum_streamline_length > maximum_streamline_length:
+        raise ValueError("minimum_streamline_length must not exceed maximum_streamline_length.")
+    if not isinstance(batch_size, (int, np.integer)) or batch_size <= 0:
+        raise ValueError("batch_size must be a positive integer.")
+    if is_image(field):
+        if field.ndim != 4:
+            raise ValueError("The tracking image must be four-dimensional.")
+        tracking_field = field_module.Image(field)
+        domain_image = Nifti1Image(
+            np.any(np.asanyarray(field.dataobj) != 0, axis=-1).astype(np.uint8), field.affine
+        )
+        local_model = local.from_shape(field.shape)
+    elif isinstance(field, field_module.BaseField):
+        tracking_field = field
+        local_model = local.from_shape((field.n_parameters,)) if algorithm is None else None
+        if isinstance(field, field_module.Image):
+            affine = np.linalg.inv(field.inverse_affine)
+            domain_image = Nifti1Image(np.any(field.values != 0, axis=-1).astype(np.uint8), affine)
+        else:
+            domain_image = None
+    else:
+        raise TypeError("field must be a Nifti1Image or a BaseField instance.")
+
+    if algorithm is None:
+        assert local_model is not None
+        tracking_algorithm = algorithm_module.Transport(local_model)
+    else:
+        _validate_algorithm(algorithm)
+        tracking_algorithm = algorithm
+    if tracking_algorithm.local_model.n_parameters != tracking_field.n_parameters:
+        raise ValueError(
+            "Algorithm local model parameter count does not match the field: "
+            f"{tracking_algorithm.local_model.n_parameters} != "
+            f"{tracking_field.n_parameters}"
+        )
+
+    if isinstance(seeds, (int, np.integer)):
+        if seeds < 0:
+            raise ValueError("seeds must be a non-negative integer or an (n_seeds, 8) array.")
+        n_seeds = int(seeds)
+    elif isinstance(seeds, np.ndarray):
+        seeds = seeds_module.validate_seed_array(seeds)
+        n_seeds = len(seeds)
+    else:
+        raise TypeError("seeds must be a non-negative integer or an (n_seeds, 8) array.")
+
+    if domain is not None and not (is_image(domain) or isinstance(domain, domain_module.BaseDomain)):
+        raise TypeError("domain must be a BaseDomain, Nifti1Image, or None.")
+    if n_seeds == 0:
+        return Tractogram([], affine_to_rasmm=np.eye(4))
+
+    if is_image(domain):
+        tracking_domain = domain_module.Image(domain)
+    elif domain is not None:
+        tracking_domain = domain
+    elif domain_image is not None:
+        tracking_domain = domain_module.Image(domain_image)
+    else:
+        raise ValueError(
+            "A domain is required when the field is not an image field and "
+            "does not provide an image-derived nonzero domain."
+        )
+
+    if isinstance(seeds, (int, np.integer)):
+        if runtime is None:
+            runtime = HipRuntime()
+        seeds = domain_module.seed(tracking_domain, n_seeds, runtime=runtime)
+    if len(seeds) == 0:
+        return Tractogram([], affine_to_rasmm=np.eye(4))
+    if runtime is None:
+        runtime = HipRuntime()
+
+    n_batches = (len(seeds) + batch_size - 1) // batch_size
+    rng_seeds = [tracking_algorithm.get_rng_seed() for _ in range(n_batches)]
+    result = tracking.track(
+        tracking_algorithm, tracking_domain, tracking_field, seeds, rng_seeds, int(batch_size), save_at,
+        maximum_streamline_length, runtime,
+    )
+    lengths = result.lengths.astype(np.int64)
+    keep = lengths > int(np.ceil(minimum_streamline_length / save_at))
+    if np.all(keep):
+        streamlines = result.streamlines()
+    else:
+        streamlines = Streamlines(result.points[np.repeat(keep, lengths)], lengths[keep])
+    return Tractogram(streamlines, affine_to_rasmm=np.eye(4))
+
+
+__all__ = [
+    "HipRuntime",
+    "Nifti1Image",
+    "OpenCLRuntime",
+    "Streamlines",
+    "Tractogram",
+    "algorithm",
+    "core",
+    "domain",
+    "field",
+    "local",
+    "seeds",
+    "tracking",
+    "tractogram",
+    "utils",
+]

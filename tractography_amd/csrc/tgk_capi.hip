@@ -1,0 +1,964 @@
+// tgk_capi.hip -- the extern "C" boundary of libtgk.so (declared in
+// include/tgk.h): runtime and buffer ownership, argument validation, chunked
+// host-array tracking and the one-op drivers.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <deque>
+#include <vector>
+
+#include "../../include/tgk.h"
+#include "tgk_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buffer[512];
+    va_list args;
+    va_start(args, fmt);
+    vsnprintf(buffer, sizeof buffer, fmt, args);
+    va_end(args);
+    g_last_error = buffer;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    const int code = (e == hipErrorOutOfMemory) ? TGK_ERR_NO_MEMORY : TGK_ERR_DEVICE;
+    return fail(code, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define TGK_HIP(expr, what)                   \
+    do {                                      \
+        const hipError_t e_ = (expr);         \
+        if (e_ != hipSuccess)                 \
+            return hip_fail(e_, what);        \
+    } while (0)
+
+// Owning device allocation.
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer() { release(); }
+    void release()
+    {
+        if (ptr)
+            (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    hipError_t allocate(size_t n)
+    {
+        if (n <= bytes && ptr)
+            return hipSuccess;
+        release();
+        if (n == 0)
+            return hipSuccess;
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e == hipSuccess)
+            bytes = n;
+        else
+            ptr = nullptr;
+        return e;
+    }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+}  // namespace
+
+struct tgk_runtime {
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+    DeviceBuffer workspace;
+};
+
+struct tgk_field {
+    tgk_runtime *rt = nullptr;
+    DeviceBuffer data;
+    uint32_t dims[4] = {0, 0, 0, 0};
+    uint32_t stride = 0;
+    float inverse_affine[16];
+    int32_t fast_precision = TGK_PRECISION_FP32;
+    DeviceBuffer fast;              // transformed coefficients for the fast path
+    int32_t fast_cached = -1;       // precision held in `fast`
+    uint32_t fast_stride = 0;
+};
+
+struct tgk_domain {
+    tgk_runtime *rt = nullptr;
+    int32_t kind = tgk::DOMAIN_MASK;
+    DeviceBuffer mask;
+    uint32_t mask_dims[3] = {0, 0, 0};
+    float mask_inverse_affine[16];
+    // boundary surface (seeding) and BVH (mesh domains)
+    DeviceBuffer vertices, triangles, cumulative;
+    uint32_t n_vertices = 0, n_triangles = 0;
+    DeviceBuffer bvh_min, bvh_max, bvh_nodes, bvh_order, near;
+    uint32_t n_nodes = 0;
+    uint32_t near_dims[3] = {0, 0, 0};
+    float near_inverse_affine[16];
+    float reach = 0.0f;
+};
+
+struct tgk_tracks {
+    std::vector<uint32_t> lengths, steps, hits;
+    std::vector<float> points;
+};
+
+namespace {
+
+void copy_rows3(const float m[16], float out[12])
+{
+    for (int i = 0; i < 12; i++)
+        out[i] = m[i];
+}
+
+tgk::FieldView field_view(const tgk_field *f)
+{
+    tgk::FieldView v;
+    v.data = f->data.as<const float>();
+    v.dims[0] = f->dims[0];
+    v.dims[1] = f->dims[1];
+    v.dims[2] = f->dims[2];
+    v.n_parameters = f->dims[3];
+    v.stride = f->stride;
+    copy_rows3(f->inverse_affine, v.inv);
+    return v;
+}
+
+tgk::MaskView mask_view(const tgk_domain *d)
+{
+    tgk::MaskView v;
+    v.mask = d->mask.as<const uint8_t>();
+    for (int i = 0; i < 3; i++)
+        v.dims[i] = d->mask_dims[i];
+    copy_rows3(d->mask_inverse_affine, v.inv);
+    return v;
+}
+
+tgk::MeshView mesh_view(const tgk_domain *d)
+{
+    tgk::MeshView v;
+    v.vertices = d->vertices.as<const float4>();
+    v.triangles = d->triangles.as<const uint4>();
+    v.bvh_min = d->bvh_min.as<const float4>();
+    v.bvh_max = d->bvh_max.as<const float4>();
+    v.bvh_nodes = d->bvh_nodes.as<const int4>();
+    v.bvh_order = d->bvh_order.as<const uint32_t>();
+    v.n_nodes = d->n_nodes;
+    v.near = d->near.as<const uint8_t>();
+    for (int i = 0; i < 3; i++)
+        v.near_dims[i] = d->near_dims[i];
+    copy_rows3(d->near_inverse_affine, v.near_inv);
+    v.reach = d->near.ptr ? d->reach : -1.0f;
+    return v;
+}
+
+int check_algorithm(const tgk_algorithm *a, const tgk_field *f)
+{
+    if (!a)
+        return fail(TGK_ERR_INVALID, "algorithm is NULL");
+    if (a->model != TGK_MODEL_SH && a->model != TGK_MODEL_DTI)
+        return fail(TGK_ERR_INVALID, "unknown model %d", a->model);
+    if (a->algorithm != TGK_ALGO_TRANSPORT && a->algorithm != TGK_ALGO_DIFFUSION)
+        return fail(TGK_ERR_INVALID, "unknown algorithm %d", a->algorithm);
+    if (a->model == TGK_MODEL_DTI && a->n_parameters != 6)
+        return fail(TGK_ERR_INVALID, "DTI needs 6 parameters, got %d", a->n_parameters);
+    if (a->model == TGK_MODEL_SH && (a->n_parameters < 1 || a->n_parameters > 45))
+        return fail(TGK_ERR_UNSUPPORTED, "SH with %d coefficients: 1..45 supported", a->n_parameters);
+    if (f && (uint32_t)a->n_parameters != f->dims[3])
+        return fail(TGK_ERR_INVALID, "incompatible local model and field: %d != %u", a->n_parameters, f->dims[3]);
+    return TGK_OK;
+}
+
+tgk::AlgoParams algo_params(const tgk_algorithm *a)
+{
+    tgk::AlgoParams p;
+    p.model = a->model;
+    p.algorithm = a->algorithm;
+    p.n_parameters = a->n_parameters;
+    p.step_size = a->step_size;
+    p.gamma = a->gamma;
+    p.noise_variance = a->noise_variance;
+    return p;
+}
+
+template <class T> int upload(DeviceBuffer &buf, const T *host, size_t count, hipStream_t stream, const char *what)
+{
+    TGK_HIP(buf.allocate(count * sizeof(T)), what);
+    if (count)
+        TGK_HIP(hipMemcpyAsync(buf.ptr, host, count * sizeof(T), hipMemcpyHostToDevice, stream), what);
+    return TGK_OK;
+}
+
+int select_device(tgk_runtime *rt)
+{
+    TGK_HIP(hipSetDevice(rt->device), "hipSetDevice");
+    return TGK_OK;
+}
+
+int ensure_workspace(tgk_runtime *rt, size_t bytes)
+{
+    TGK_HIP(rt->workspace.allocate(bytes), "workspace allocation");
+    return TGK_OK;
+}
+
+}  // namespace
+
+namespace tgk {
+
+// Workspace: widened lengths (n + 1 int64, 256-byte aligned) followed by
+// the scan's temporary storage.
+size_t offsets_workspace_bytes(int64_t n)
+{
+    size_t bytes = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int64_t *)nullptr, (int64_t *)nullptr, (int)(n + 1));
+    const size_t widened = (((size_t)(n + 1) * sizeof(int64_t)) + 255) & ~(size_t)255;
+    return widened + bytes;
+}
+
+namespace {
+__global__ void widen_lengths(const uint32_t *__restrict__ lengths, int64_t n, int64_t *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n)
+        out[i] = i < n ? (int64_t)lengths[i] : 0;
+}
+}  // namespace
+
+hipError_t launch_offsets(const uint32_t *lengths, int64_t n, int64_t *offsets, void *workspace,
+                          size_t workspace_bytes, hipStream_t stream)
+{
+    const size_t widened_bytes = (((size_t)(n + 1) * sizeof(int64_t)) + 255) & ~(size_t)255;
+    if (workspace_bytes < widened_bytes)
+        return hipErrorInvalidValue;
+    int64_t *widened = static_cast<int64_t *>(workspace);
+    widen_lengths<<<(unsigned)((n + 1 + 255) / 256), 256, 0, stream>>>(lengths, n, widened);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    size_t bytes = workspace_bytes - widened_bytes;
+    return hipcub::DeviceScan::ExclusiveSum(static_cast<char *>(workspace) + widened_bytes, bytes,
+                                            (const int64_t *)widened, offsets, (int)(n + 1), stream);
+}
+
+}  // namespace tgk
+
+extern "C" {
+
+const char *tgk_last_error(void) { return g_last_error.c_str(); }
+
+int tgk_version(void) { return 1; }
+
+int tgk_runtime_create(int32_t device, tgk_runtime **out)
+{
+    if (!out)
+        return fail(TGK_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+        return fail(TGK_ERR_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= count)
+        return fail(TGK_ERR_INVALID, "device %d out of range (have %d)", device, count);
+    std::unique_ptr<tgk_runtime> rt(new (std::nothrow) tgk_runtime());
+    if (!rt)
+        return fail(TGK_ERR_NO_MEMORY, "runtime allocation");
+    rt->device = device;
+    TGK_HIP(hipSetDevice(device), "hipSetDevice");
+    TGK_HIP(hipStreamCreateWithFlags(&rt->stream, hipStreamNonBlocking), "hipStreamCreate");
+    *out = rt.release();
+    return TGK_OK;
+}
+
+int tgk_runtime_destroy(tgk_runtime *rt)
+{
+    if (!rt)
+        return TGK_OK;
+    (void)hipSetDevice(rt->device);
+    if (rt->stream) {
+        (void)hipStreamSynchronize(rt->stream);
+        (void)hipStreamDestroy(rt->stream);
+    }
+    delete rt;
+    return TGK_OK;
+}
+
+int tgk_runtime_synchronize(tgk_runtime *rt)
+{
+    if (!rt)
+        return fail(TGK_ERR_INVALID, "runtime is NULL");
+    TGK_HIP(hipStreamSynchronize(rt->stream), "hipStreamSynchronize");
+    return TGK_OK;
+}
+
+int tgk_runtime_info(tgk_runtime *rt, char *name, int32_t name_len, int32_t *compute_units, int64_t *total_bytes)
+{
+    if (!rt)
+        return fail(TGK_ERR_INVALID, "runtime is NULL");
+    hipDeviceProp_t prop;
+    TGK_HIP(hipGetDeviceProperties(&prop, rt->device), "hipGetDeviceProperties");
+    if (name && name_len > 0) {
+        snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (compute_units)
+        *compute_units = prop.multiProcessorCount;
+    if (total_bytes)
+        *total_bytes = (int64_t)prop.totalGlobalMem;
+    return TGK_OK;
+}
+
+int tgk_runtime_stream(tgk_runtime *rt, void **stream)
+{
+    if (!rt || !stream)
+        return fail(TGK_ERR_INVALID, "NULL argument");
+    *stream = (void *)rt->stream;
+    return TGK_OK;
+}
+
+int tgk_field_create(tgk_runtime *rt, const float *data, const uint32_t dims[4], const float inverse_affine[16],
+                     tgk_field **out)
+{
+    if (!rt || !data || !dims || !inverse_affine || !out)
+        return fail(TGK_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    if (dims[0] == 0 || dims[1] == 0 || dims[2] == 0 || dims[3] == 0)
+        return fail(TGK_ERR_INVALID, "empty field (%u, %u, %u, %u)", dims[0], dims[1], dims[2], dims[3]);
+    if (int r = select_device(rt))
+        return r;
+    std::unique_ptr<tgk_field> f(new (std::nothrow) tgk_field());
+    if (!f)
+        return fail(TGK_ERR_NO_MEMORY, "field allocation");
+    f->rt = rt;
+    memcpy(f->dims, dims, sizeof f->dims);
+    memcpy(f->inverse_affine, inverse_affine, sizeof f->inverse_affine);
+    const uint32_t K = dims[3];
+    f->stride = (K + 3u) & ~3u;
+    const size_t n_voxels = (size_t)dims[0] * dims[1] * dims[2];
+    TGK_HIP(f->data.allocate(n_voxels * f->stride * sizeof(float)), "field allocation");
+    // Strided copy: K floats per voxel into 16-byte aligned rows, pad zeroed.
+    TGK_HIP(hipMemsetAsync(f->data.ptr, 0, n_voxels * f->stride * sizeof(float), rt->stream), "field clear");
+    TGK_HIP(hipMemcpy2DAsync(f->data.ptr, f->stride * sizeof(float), data, K * sizeof(float), K * sizeof(float),
+                             n_voxels, hipMemcpyHostToDevice, rt->stream),
+            "field upload");
+    TGK_HIP(hipStreamSynchronize(rt->stream), "field upload");
+    *out = f.release();
+    return TGK_OK;
+}
+
+int tgk_field_destroy(tgk_field *field)
+{
+    if (!field)
+        return TGK_OK;
+    (void)hipSetDevice(field->rt->device);
+    delete field;
+    return TGK_OK;
+}
+
+int tgk_field_set_fast_precision(tgk_field *field, int32_t precision)
+{
+    if (!field)
+        return fail(TGK_ERR_INVALID, "field is NULL");
+    if (precision < TGK_PRECISION_FP32 || precision > TGK_PRECISION_BF16)
+        return fail(TGK_ERR_INVALID, "unknown precision %d", precision);
+    field->fast_precision = precision;
+    return TGK_OK;
+}
+
+int tgk_mask_domain_create(tgk_runtime *rt, const uint8_t *mask, const uint32_t dims[3],
+                           const float inverse_affine[16], tgk_domain **out)
+{
+    if (!rt || !mask || !dims || !inverse_affine || !out)
+        return fail(TGK_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    if (dims[0] == 0 || dims[1] == 0 || dims[2] == 0)
+        return fail(TGK_ERR_INVALID, "empty mask");
+    if (int r = select_device(rt))
+        return r;
+    std::unique_ptr<tgk_domain> d(new (std::nothrow) tgk_domain());
+    if (!d)
+        return fail(TGK_ERR_NO_MEMORY, "domain allocation");
+    d->rt = rt;
+    d->kind = tgk::DOMAIN_MASK;
+    memcpy(d->mask_dims, dims, sizeof d->mask_dims);
+    memcpy(d->mask_inverse_affine, inverse_affine, sizeof d->mask_inverse_affine);
+    if (int r = upload(d->mask, mask, (size_t)dims[0] * dims[1] * dims[2], rt->stream, "mask upload"))
+        return r;
+    TGK_HIP(hipStreamSynchronize(rt->stream), "mask upload");
+    *out = d.release();
+    return TGK_OK;
+}
+
+int tgk_domain_set_boundary(tgk_domain *d, const float *vertices, uint32_t n_vertices, const uint32_t *triangles,
+                            const float *cumulative, uint32_t n_triangles)
+{
+    if (!d || !vertices || !triangles || !cumulative)
+        return fail(TGK_ERR_INVALID, "NULL argument");
+    if (n_triangles == 0 || n_vertices == 0)
+        return fail(TGK_ERR_INVALID, "empty boundary");
+    if (int r = select_device(d->rt))
+        return r;
+    hipStream_t s = d->rt->stream;
+    if (int r = upload(d->vertices, vertices, (size_t)n_vertices * 4, s, "boundary upload"))
+        return r;
+    if (int r = upload(d->triangles, triangles, (size_t)n_triangles * 4, s, "boundary upload"))
+        return r;
+    if (int r = upload(d->cumulative, cumulative, (size_t)n_triangles, s, "boundary upload"))
+        return r;
+    d->n_vertices = n_vertices;
+    d->n_triangles = n_triangles;
+    TGK_HIP(hipStreamSynchronize(s), "boundary upload");
+    return TGK_OK;
+}
+
+int tgk_mesh_domain_create(tgk_runtime *rt, const float *vertices, uint32_t n_vertices, const uint32_t *triangles,
+                           const float *cumulative, uint32_t n_triangles, const float *bvh_min, const float *bvh_max,
+                           const int32_t *bvh_nodes, uint32_t n_nodes, const uint32_t *bvh_order, uint32_t n_order,
+                           const uint8_t *near, const uint32_t near_dims[3], const float near_inverse_affine[16],
+                           float reach, tgk_domain **out)
+{
+    if (!rt || !out || !bvh_min || !bvh_max || !bvh_nodes || !bvh_order)
+        return fail(TGK_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    if (n_nodes == 0 || n_order == 0)
+        return fail(TGK_ERR_INVALID, "empty BVH");
+    for (uint32_t i = 0; i < n_nodes; i++) {
+        const int32_t *nd = bvh_nodes + 4 * (size_t)i;
+        if (nd[3] > 0) {
+            if (nd[2] < 0 || (uint64_t)nd[2] + (uint64_t)nd[3] > n_order)
+                return fail(TGK_ERR_INVALID, "BVH leaf %u out of range", i);
+        } else if (nd[0] < 0 || nd[1] < 0 || (uint32_t)nd[0] >= n_nodes || (uint32_t)nd[1] >= n_nodes) {
+            return fail(TGK_ERR_INVALID, "BVH node %u has invalid children", i);
+        }
+    }
+    for (uint32_t i = 0; i < n_order; i++)
+        if (bvh_order[i] >= n_triangles)
+            return fail(TGK_ERR_INVALID, "BVH triangle slot out of range");
+    for (uint32_t i = 0; i < n_triangles; i++)
+        for (int k = 0; k < 3; k++)
+            if (triangles[4 * (size_t)i + k] >= n_vertices)
+                return fail(TGK_ERR_INVALID, "triangle %u references vertex out of range", i);
+    if (int r = select_device(rt))
+        return r;
+    std::unique_ptr<tgk_domain> d(new (std::nothrow) tgk_domain());
+    if (!d)
+        return fail(TGK_ERR_NO_MEMORY, "domain allocation");
+    d->rt = rt;
+    d->kind = tgk::DOMAIN_MESH;
+    hipStream_t s = rt->stream;
+    if (int r = upload(d->vertices, vertices, (size_t)n_vertices * 4, s, "mesh upload"))
+        return r;
+    if (int r = upload(d->triangles, triangles, (size_t)n_triangles * 4, s, "mesh upload"))
+        return r;
+    if (cumulative) {
+        if (int r = upload(d->cumulative, cumulative, (size_t)n_triangles, s, "mesh upload"))
+            return r;
+    }
+    if (int r = upload(d->bvh_min, bvh_min, (size_t)n_nodes * 4, s, "BVH upload"))
+        return r;
+    if (int r = upload(d->bvh_max, bvh_max, (size_t)n_nodes * 4, s, "BVH upload"))
+        return r;
+    if (int r = upload(d->bvh_nodes, bvh_nodes, (size_t)n_nodes * 4, s, "BVH upload"))
+        return r;
+    if (int r = upload(d->bvh_order, bvh_order, (size_t)n_order, s, "BVH upload"))
+        return r;
+    d->n_vertices = n_vertices;
+    d->n_triangles = n_triangles;
+    d->n_nodes = n_nodes;
+    if (near && near_dims && near_inverse_affine) {
+        memcpy(d->near_dims, near_dims, sizeof d->near_dims);
+        memcpy(d->near_inverse_affine, near_inverse_affine, sizeof d->near_inverse_affine);
+        if (int r = upload(d->near, near, (size_t)near_dims[0] * near_dims[1] * near_dims[2], s, "grid upload"))
+            return r;
+        d->reach = reach;
+    }
+    TGK_HIP(hipStreamSynchronize(s), "mesh upload");
+    *out = d.release();
+    return TGK_OK;
+}
+
+int tgk_domain_destroy(tgk_domain *domain)
+{
+    if (!domain)
+        return TGK_OK;
+    (void)hipSetDevice(domain->rt->device);
+    delete domain;
+    return TGK_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int check_track_args(const tgk_domain *domain, const tgk_field *field, const tgk_algorithm *algorithm, int32_t path,
+                     int64_t n_seeds, int64_t batch_size, uint32_t n_points, float save_at)
+{
+    if (!domain || !field)
+        return fail(TGK_ERR_INVALID, "domain and field are required");
+    if (int r = check_algorithm(algorithm, field))
+        return r;
+    if (path != TGK_PATH_PLAIN && path != TGK_PATH_FAST)
+        return fail(TGK_ERR_INVALID, "unknown path %d", path);
+    if (n_seeds < 0)
+        return fail(TGK_ERR_INVALID, "negative seed count");
+    if (batch_size <= 0)
+        return fail(TGK_ERR_INVALID, "batch_size must be positive");
+    if (n_points == 0)
+        return fail(TGK_ERR_INVALID, "n_points must be positive");
+    if (!(save_at > 0.0f))
+        return fail(TGK_ERR_INVALID, "save_at must be positive");
+    if (domain->kind == tgk::DOMAIN_MESH && domain->n_nodes == 0)
+        return fail(TGK_ERR_INVALID, "mesh domain without BVH");
+    return TGK_OK;
+}
+
+tgk::TrackLaunch make_launch(const tgk_domain *domain, const tgk_field *field, const tgk_algorithm *algorithm,
+                             const float *d_seeds, int64_t first_seed, int64_t n_seeds, const uint32_t *d_rng_seeds,
+                             int64_t batch_size, uint32_t n_points, float save_at, uint32_t *d_lengths,
+                             uint32_t *d_steps, uint32_t *d_hits, float *d_dense)
+{
+    tgk::TrackLaunch p;
+    memset(&p, 0, sizeof p);
+    p.domain_kind = domain->kind;
+    if (domain->kind == tgk::DOMAIN_MASK)
+        p.mask = mask_view(domain);
+    else
+        p.mesh = mesh_view(domain);
+    p.field = field_view(field);
+    p.algo = algo_params(algorithm);
+    p.seeds = d_seeds;
+    p.first_seed = first_seed;
+    p.n_seeds = n_seeds;
+    p.rng_seeds = d_rng_seeds;
+    p.batch_size = batch_size;
+    p.n_points = n_points;
+    p.save_at = save_at;
+    p.lengths = d_lengths;
+    p.steps = d_steps;
+    p.hits = d_hits;
+    p.points = d_dense;
+    return p;
+}
+
+int run_track(tgk_runtime *rt, const tgk_domain *domain, tgk_field *field, const tgk_algorithm *algorithm,
+              int32_t path, const float *d_seeds, int64_t first_seed, int64_t n_seeds, const uint32_t *d_rng_seeds,
+              int64_t batch_size, uint32_t n_points, float save_at, uint32_t *d_lengths, uint32_t *d_steps,
+              uint32_t *d_hits, float *d_dense);
+
+int run_compact(tgk_runtime *rt, int32_t path, const float *d_dense, int64_t n_seeds, uint32_t n_points,
+                const uint32_t *d_lengths, int64_t *d_offsets, float *d_points)
+{
+    const size_t scan_bytes = tgk::offsets_workspace_bytes(n_seeds);
+    if (int r = ensure_workspace(rt, scan_bytes))
+        return r;
+    TGK_HIP(tgk::launch_offsets(d_lengths, n_seeds, d_offsets, rt->workspace.ptr, scan_bytes, rt->stream),
+            "offsets scan");
+    if (path == TGK_PATH_PLAIN)
+        TGK_HIP(tgk::launch_compact_point_major(d_dense, n_seeds, n_points, d_lengths, d_offsets, d_points,
+                                                rt->stream),
+                "compaction");
+    else
+        TGK_HIP(tgk::launch_compact_seed_major(d_dense, n_seeds, n_points, d_lengths, d_offsets, d_points,
+                                               rt->stream),
+                "compaction");
+    return TGK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tgk_track_device(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field,
+                     const tgk_algorithm *algorithm, int32_t path, const float *d_seeds, int64_t first_seed,
+                     int64_t n_seeds, const uint32_t *d_rng_seeds, int64_t batch_size, uint32_t n_points,
+                     float save_at, uint32_t *d_lengths, uint32_t *d_steps, uint32_t *d_hits, float *d_dense)
+{
+    if (!rt)
+        return fail(TGK_ERR_INVALID, "runtime is NULL");
+    if (int r = check_track_args(domain, field, algorithm, path, n_seeds, batch_size, n_points, save_at))
+        return r;
+    if (n_seeds > 0 && (!d_seeds || !d_rng_seeds || !d_lengths || !d_steps || !d_hits || !d_dense))
+        return fail(TGK_ERR_INVALID, "NULL device buffer");
+    if (first_seed < 0)
+        return fail(TGK_ERR_INVALID, "negative first_seed");
+    if (int r = select_device(rt))
+        return r;
+    return run_track(rt, domain, const_cast<tgk_field *>(field), algorithm, path, d_seeds, first_seed, n_seeds,
+                     d_rng_seeds, batch_size, n_points, save_at, d_lengths, d_steps, d_hits, d_dense);
+}
+
+int tgk_compact_device(tgk_runtime *rt, int32_t path, const float *d_dense, int64_t n_seeds, uint32_t n_points,
+                       const uint32_t *d_lengths, int64_t *d_offsets, float *d_points)
+{
+    if (!rt)
+        return fail(TGK_ERR_INVALID, "runtime is NULL");
+    if (path != TGK_PATH_PLAIN && path != TGK_PATH_FAST)
+        return fail(TGK_ERR_INVALID, "unknown path %d", path);
+    if (n_seeds < 0 || (n_seeds > 0 && (!d_dense || !d_lengths || !d_offsets || !d_points)))
+        return fail(TGK_ERR_INVALID, "bad compaction arguments");
+    if (n_seeds == 0)
+        return TGK_OK;
+    if (int r = select_device(rt))
+        return r;
+    return run_compact(rt, path, d_dense, n_seeds, n_points, d_lengths, d_offsets, d_points);
+}
+
+int tgk_track(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field, const tgk_algorithm *algorithm,
+              int32_t path, const float *seeds, int64_t n_seeds, const uint32_t *rng_seeds, int64_t batch_size,
+              uint32_t n_points, float save_at, tgk_tracks **out)
+{
+    if (!rt || !out)
+        return fail(TGK_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    if (int r = check_track_args(domain, field, algorithm, path, n_seeds, batch_size, n_points, save_at))
+        return r;
+    if (n_seeds > 0 && (!seeds || !rng_seeds))
+        return fail(TGK_ERR_INVALID, "NULL seeds");
+    if (int r = select_device(rt))
+        return r;
+    std::unique_ptr<tgk_tracks> tracks(new (std::nothrow) tgk_tracks());
+    if (!tracks)
+        return fail(TGK_ERR_NO_MEMORY, "tracks allocation");
+    tracks->lengths.resize((size_t)n_seeds);
+    tracks->steps.resize((size_t)n_seeds);
+    tracks->hits.resize((size_t)n_seeds);
+    if (n_seeds == 0) {
+        *out = tracks.release();
+        return TGK_OK;
+    }
+    // Bound the dense per-chunk buffer (points are compacted before copy-back).
+    const size_t per_seed = (size_t)n_points * 3 * sizeof(float);
+    size_t budget = (size_t)1 << 31;
+    if (const char *env = getenv("TGK_CHUNK_BYTES"))
+        budget = std::max<size_t>((size_t)strtoull(env, nullptr, 10), per_seed);
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(n_seeds, (int64_t)(budget / per_seed)));
+    const int64_t n_batches = (n_seeds + batch_size - 1) / batch_size;
+
+    DeviceBuffer d_seeds, d_rng, d_lengths, d_steps, d_hits, d_dense, d_offsets, d_points;
+    TGK_HIP(d_seeds.allocate((size_t)chunk * 8 * sizeof(float)), "seed buffer");
+    if (int r = upload(d_rng, rng_seeds, (size_t)n_batches, rt->stream, "rng seeds"))
+        return r;
+    TGK_HIP(d_lengths.allocate((size_t)chunk * 4), "lengths buffer");
+    TGK_HIP(d_steps.allocate((size_t)chunk * 4), "steps buffer");
+    TGK_HIP(d_hits.allocate((size_t)chunk * 4), "hits buffer");
+    TGK_HIP(d_dense.allocate((size_t)chunk * per_seed), "point buffer");
+    TGK_HIP(d_offsets.allocate((size_t)(chunk + 1) * 8), "offsets buffer");
+    TGK_HIP(d_points.allocate((size_t)chunk * per_seed), "point buffer");
+
+    for (int64_t start = 0; start < n_seeds; start += chunk) {
+        const int64_t count = std::min(chunk, n_seeds - start);
+        TGK_HIP(hipMemcpyAsync(d_seeds.ptr, seeds + 8 * start, (size_t)count * 8 * sizeof(float),
+                               hipMemcpyHostToDevice, rt->stream),
+                "seed upload");
+        if (int r = run_track(rt, domain, const_cast<tgk_field *>(field), algorithm, path, d_seeds.as<float>(),
+                              start, count, d_rng.as<uint32_t>(), batch_size, n_points, save_at,
+                              d_lengths.as<uint32_t>(), d_steps.as<uint32_t>(), d_hits.as<uint32_t>(),
+                              d_dense.as<float>()))
+            return r;
+        if (int r = run_compact(rt, path, d_dense.as<float>(), count, n_points, d_lengths.as<uint32_t>(),
+                                d_offsets.as<int64_t>(), d_points.as<float>()))
+            return r;
+        int64_t total = 0;
+        TGK_HIP(hipMemcpyAsync(&total, d_offsets.as<int64_t>() + count, sizeof total, hipMemcpyDeviceToHost,
+                               rt->stream),
+                "offsets copy");
+        TGK_HIP(hipMemcpyAsync(tracks->lengths.data() + start, d_lengths.ptr, (size_t)count * 4,
+                               hipMemcpyDeviceToHost, rt->stream),
+                "lengths copy");
+        TGK_HIP(hipMemcpyAsync(tracks->steps.data() + start, d_steps.ptr, (size_t)count * 4, hipMemcpyDeviceToHost,
+                               rt->stream),
+                "steps copy");
+        TGK_HIP(hipMemcpyAsync(tracks->hits.data() + start, d_hits.ptr, (size_t)count * 4, hipMemcpyDeviceToHost,
+                               rt->stream),
+                "hits copy");
+        TGK_HIP(hipStreamSynchronize(rt->stream), "tracking");
+        const size_t before = tracks->points.size();
+        tracks->points.resize(before + (size_t)total * 3);
+        TGK_HIP(hipMemcpy(tracks->points.data() + before, d_points.ptr, (size_t)total * 3 * sizeof(float),
+                          hipMemcpyDeviceToHost),
+                "points copy");
+    }
+    *out = tracks.release();
+    return TGK_OK;
+}
+
+int tgk_tracks_size(const tgk_tracks *tracks, int64_t *n_seeds, int64_t *total_points)
+{
+    if (!tracks)
+        return fail(TGK_ERR_INVALID, "tracks is NULL");
+    if (n_seeds)
+        *n_seeds = (int64_t)tracks->lengths.size();
+    if (total_points)
+        *total_points = (int64_t)(tracks->points.size() / 3);
+    return TGK_OK;
+}
+
+int tgk_tracks_copy(const tgk_tracks *tracks, uint32_t *lengths, uint32_t *steps, uint32_t *hits, float *points)
+{
+    if (!tracks)
+        return fail(TGK_ERR_INVALID, "tracks is NULL");
+    const size_t n = tracks->lengths.size();
+    if (lengths && n)
+        memcpy(lengths, tracks->lengths.data(), n * 4);
+    if (steps && n)
+        memcpy(steps, tracks->steps.data(), n * 4);
+    if (hits && n)
+        memcpy(hits, tracks->hits.data(), n * 4);
+    if (points && !tracks->points.empty())
+        memcpy(points, tracks->points.data(), tracks->points.size() * sizeof(float));
+    return TGK_OK;
+}
+
+int tgk_tracks_destroy(tgk_tracks *tracks)
+{
+    delete tracks;
+    return TGK_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int run_track(tgk_runtime *rt, const tgk_domain *domain, tgk_field *field, const tgk_algorithm *algorithm,
+              int32_t path, const float *d_seeds, int64_t first_seed, int64_t n_seeds, const uint32_t *d_rng_seeds,
+              int64_t batch_size, uint32_t n_points, float save_at, uint32_t *d_lengths, uint32_t *d_steps,
+              uint32_t *d_hits, float *d_dense)
+{
+    if (n_seeds == 0)
+        return TGK_OK;
+    const tgk::TrackLaunch p = make_launch(domain, field, algorithm, d_seeds, first_seed, n_seeds, d_rng_seeds,
+                                           batch_size, n_points, save_at, d_lengths, d_steps, d_hits, d_dense);
+    if (path == TGK_PATH_PLAIN) {
+        TGK_HIP(tgk::launch_track_plain(p, rt->stream), "plain tracking kernel");
+        return TGK_OK;
+    }
+    return fail(TGK_ERR_UNSUPPORTED, "the fast path is not available in this build");
+}
+
+// Scratch device copies for the one-op drivers.
+struct Staging {
+    std::deque<DeviceBuffer> buffers;
+    template <class T> int in(const T *host, size_t count, hipStream_t s, T **dev)
+    {
+        buffers.emplace_back();
+        if (int r = upload(buffers.back(), host, count, s, "input upload"))
+            return r;
+        *dev = buffers.back().as<T>();
+        return TGK_OK;
+    }
+    template <class T> int out(size_t count, T **dev)
+    {
+        buffers.emplace_back();
+        TGK_HIP(buffers.back().allocate(std::max<size_t>(count, 1) * sizeof(T)), "output allocation");
+        *dev = buffers.back().as<T>();
+        return TGK_OK;
+    }
+};
+
+template <class T> int fetch(T *host, const T *dev, size_t count, hipStream_t s)
+{
+    if (count)
+        TGK_HIP(hipMemcpyAsync(host, dev, count * sizeof(T), hipMemcpyDeviceToHost, s), "output copy");
+    TGK_HIP(hipStreamSynchronize(s), "one-op driver");
+    return TGK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tgk_evaluate(tgk_runtime *rt, int32_t model, int32_t n_parameters, const float *params, const float *orientations,
+                 int64_t n, float *out)
+{
+    if (!rt || n < 0 || (n > 0 && (!params || !orientations || !out)))
+        return fail(TGK_ERR_INVALID, "bad arguments");
+    tgk_algorithm a = {model, TGK_ALGO_TRANSPORT, n_parameters, 1.0f, 1.0f, 0.0f};
+    if (int r = check_algorithm(&a, nullptr))
+        return r;
+    if (n == 0)
+        return TGK_OK;
+    if (int r = select_device(rt))
+        return r;
+    Staging st;
+    const float *dp, *dor;
+    float *dout;
+    if (int r = st.in(params, (size_t)n * n_parameters, rt->stream, const_cast<float **>(&dp)))
+        return r;
+    if (int r = st.in(orientations, (size_t)n * 3, rt->stream, const_cast<float **>(&dor)))
+        return r;
+    if (int r = st.out((size_t)n * 3, &dout))
+        return r;
+    TGK_HIP(tgk::launch_evaluate(model, n_parameters, dp, dor, n, dout, rt->stream), "evaluate kernel");
+    return fetch(out, dout, (size_t)n * 3, rt->stream);
+}
+
+int tgk_update_orientation(tgk_runtime *rt, const tgk_algorithm *algorithm, uint32_t rng_seed, const float *params,
+                           const float *orientations, int64_t n, float *out)
+{
+    if (!rt || n < 0 || (n > 0 && (!params || !orientations || !out)))
+        return fail(TGK_ERR_INVALID, "bad arguments");
+    if (int r = check_algorithm(algorithm, nullptr))
+        return r;
+    if (n == 0)
+        return TGK_OK;
+    if (int r = select_device(rt))
+        return r;
+    Staging st;
+    const float *dp, *dor;
+    float *dout;
+    if (int r = st.in(params, (size_t)n * algorithm->n_parameters, rt->stream, const_cast<float **>(&dp)))
+        return r;
+    if (int r = st.in(orientations, (size_t)n * 3, rt->stream, const_cast<float **>(&dor)))
+        return r;
+    if (int r = st.out((size_t)n * 3, &dout))
+        return r;
+    TGK_HIP(tgk::launch_update(algo_params(algorithm), rng_seed, dp, dor, n, dout, rt->stream), "update kernel");
+    return fetch(out, dout, (size_t)n * 3, rt->stream);
+}
+
+int tgk_field_evaluate(tgk_runtime *rt, const tgk_field *field, const float *points, int64_t n, float *out)
+{
+    if (!rt || !field || n < 0 || (n > 0 && (!points || !out)))
+        return fail(TGK_ERR_INVALID, "bad arguments");
+    if (n == 0)
+        return TGK_OK;
+    if (int r = select_device(rt))
+        return r;
+    Staging st;
+    const float *dpts;
+    float *dout;
+    if (int r = st.in(points, (size_t)n * 3, rt->stream, const_cast<float **>(&dpts)))
+        return r;
+    if (int r = st.out((size_t)n * field->dims[3], &dout))
+        return r;
+    TGK_HIP(tgk::launch_field_evaluate(field_view(field), dpts, n, dout, rt->stream), "field kernel");
+    return fetch(out, dout, (size_t)n * field->dims[3], rt->stream);
+}
+
+int tgk_domain_contains(tgk_runtime *rt, const tgk_domain *domain, const float *points, int64_t n, uint8_t *out)
+{
+    if (!rt || !domain || n < 0 || (n > 0 && (!points || !out)))
+        return fail(TGK_ERR_INVALID, "bad arguments");
+    if (domain->kind != tgk::DOMAIN_MASK)
+        return fail(TGK_ERR_UNSUPPORTED, "point membership is defined for mask domains");
+    if (n == 0)
+        return TGK_OK;
+    if (int r = select_device(rt))
+        return r;
+    Staging st;
+    const float *dpts;
+    uint8_t *dout;
+    if (int r = st.in(points, (size_t)n * 3, rt->stream, const_cast<float **>(&dpts)))
+        return r;
+    if (int r = st.out((size_t)n, &dout))
+        return r;
+    TGK_HIP(tgk::launch_contains(mask_view(domain), dpts, n, dout, rt->stream), "contains kernel");
+    return fetch(out, dout, (size_t)n, rt->stream);
+}
+
+int tgk_bvh_segment_hit(tgk_runtime *rt, const tgk_domain *domain, const float *starts, const float *ends, int64_t n,
+                        uint8_t *found, uint32_t *hit, float *t)
+{
+    if (!rt || !domain || n < 0 || (n > 0 && (!starts || !ends || !found || !hit || !t)))
+        return fail(TGK_ERR_INVALID, "bad arguments");
+    if (domain->n_nodes == 0)
+        return fail(TGK_ERR_INVALID, "domain has no BVH");
+    if (n == 0)
+        return TGK_OK;
+    if (int r = select_device(rt))
+        return r;
+    Staging st;
+    const float *ds, *de;
+    uint8_t *dfound;
+    uint32_t *dhit;
+    float *dt;
+    if (int r = st.in(starts, (size_t)n * 3, rt->stream, const_cast<float **>(&ds)))
+        return r;
+    if (int r = st.in(ends, (size_t)n * 3, rt->stream, const_cast<float **>(&de)))
+        return r;
+    if (int r = st.out((size_t)n, &dfound))
+        return r;
+    if (int r = st.out((size_t)n, &dhit))
+        return r;
+    if (int r = st.out((size_t)n, &dt))
+        return r;
+    TGK_HIP(tgk::launch_bvh_hit(mesh_view(domain), ds, de, n, dfound, dhit, dt, rt->stream), "BVH kernel");
+    if (int r = fetch(found, dfound, (size_t)n, rt->stream))
+        return r;
+    if (int r = fetch(hit, dhit, (size_t)n, rt->stream))
+        return r;
+    return fetch(t, dt, (size_t)n, rt->stream);
+}
+
+int tgk_math(tgk_runtime *rt, int32_t fn, const float *x, const float *y, int64_t n, float *out)
+{
+    if (!rt || fn < 0 || fn > 7 || n < 0 || (n > 0 && (!x || !y || !out)))
+        return fail(TGK_ERR_INVALID, "bad arguments");
+    if (n == 0)
+        return TGK_OK;
+    if (int r = select_device(rt))
+        return r;
+    Staging st;
+    const float *dx, *dy;
+    float *dout;
+    if (int r = st.in(x, (size_t)n, rt->stream, const_cast<float **>(&dx)))
+        return r;
+    if (int r = st.in(y, (size_t)n, rt->stream, const_cast<float **>(&dy)))
+        return r;
+    if (int r = st.out((size_t)n, &dout))
+        return r;
+    TGK_HIP(tgk::launch_math(fn, dx, dy, n, dout, rt->stream), "math kernel");
+    return fetch(out, dout, (size_t)n, rt->stream);
+}
+
+int tgk_domain_seed(tgk_runtime *rt, const tgk_domain *domain, uint32_t rng_seed, int64_t n, float *out)
+{
+    if (!rt || !domain || n < 0 || (n > 0 && !out))
+        return fail(TGK_ERR_INVALID, "bad arguments");
+    if (domain->n_triangles == 0 || !domain->cumulative.ptr)
+        return fail(TGK_ERR_INVALID, "domain has no boundary surface");
+    if (n == 0)
+        return TGK_OK;
+    if (int r = select_device(rt))
+        return r;
+    Staging st;
+    float *dout;
+    if (int r = st.out((size_t)n * 8, &dout))
+        return r;
+    TGK_HIP(tgk::launch_domain_seed(domain->vertices.as<const float4>(), domain->triangles.as<const uint4>(),
+                                    domain->cumulative.as<const float>(), domain->n_triangles, rng_seed, n, dout,
+                                    rt->stream),
+            "boundary seeding kernel");
+    return fetch(out, dout, (size_t)n * 8, rt->stream);
+}
+
+int tgk_seeds_from_fod(tgk_runtime *rt, int32_t model, int32_t n_parameters, const float *fod, const float *voxels,
+                       uint32_t nnz, const float affine[16], uint32_t *states, int64_t n, int32_t method, float *out)
+{
+    (void)rt;
+    (void)model;
+    (void)n_parameters;
+    (void)fod;
+    (void)voxels;
+    (void)nnz;
+    (void)affine;
+    (void)states;
+    (void)n;
+    (void)method;
+    (void)out;
+    return fail(TGK_ERR_UNSUPPORTED, "seeding from FOD is not available in this build");
+}
+
+}  // extern "C"

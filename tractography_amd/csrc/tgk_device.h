@@ -1,0 +1,295 @@
+// tgk_device.h -- device-side building blocks shared by the tracking kernels:
+// launch parameter blocks, the MWC64X stream, world->voxel lookups of the mask
+// domain and the image field, and the boundary BVH traversal.
+//
+// Everything here follows the reference semantics operation by operation
+// (no contraction: translation units including this header for the plain
+// path are compiled with -ffp-contract=off).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tgk_math.h"
+
+namespace tgk {
+
+enum Model : int32_t { MODEL_SH = 0, MODEL_DTI = 1 };
+enum Algo : int32_t { ALGO_TRANSPORT = 0, ALGO_DIFFUSION = 1 };
+enum DomainKind : int32_t { DOMAIN_MASK = 0, DOMAIN_MESH = 1 };
+
+constexpr uint32_t NO_HIT = 0xffffffffu;
+
+// Field as laid out in HBM: (X, Y, Z) voxels of `stride` floats, the first K
+// of which are the model coefficients (stride = K rounded up to 4 so every
+// voxel starts 16-byte aligned).
+struct FieldView {
+    const float *data;
+    uint32_t dims[3];
+    uint32_t n_parameters;
+    uint32_t stride;
+    float inv[12];  // rows 0..2 of the float32 inverse affine
+};
+
+struct MaskView {
+    const uint8_t *mask;
+    uint32_t dims[3];
+    float inv[12];
+};
+
+struct MeshView {
+    const float4 *vertices;
+    const uint4 *triangles;
+    const float4 *bvh_min;
+    const float4 *bvh_max;
+    const int4 *bvh_nodes;
+    const uint32_t *bvh_order;
+    uint32_t n_nodes;
+    // Culling grid: near[cell] = 0 guarantees that no triangle's bounding box
+    // comes within `reach` (world units) of the cell, so a segment that starts
+    // in the cell and is at most `reach` long cannot hit the boundary and its
+    // BVH query is skipped. Cells are floor(inv * p + 0.5) as for masks.
+    const uint8_t *near;
+    uint32_t near_dims[3];
+    float near_inv[12];
+    float reach;
+};
+
+struct AlgoParams {
+    int32_t model;
+    int32_t algorithm;
+    int32_t n_parameters;
+    float step_size;
+    float gamma;
+    float noise_variance;
+};
+
+struct TrackLaunch {
+    int32_t domain_kind;
+    MaskView mask;
+    MeshView mesh;
+    FieldView field;
+    AlgoParams algo;
+    const float *seeds;          // (n, 8) device
+    int64_t first_seed;          // global index of seeds[0] (RNG stream ids)
+    int64_t n_seeds;
+    const uint32_t *rng_seeds;   // one per batch of `batch_size` global seeds
+    int64_t batch_size;
+    uint32_t n_points;           // point capacity per streamline, seed included
+    float save_at;
+    uint32_t *lengths;
+    uint32_t *steps;
+    uint32_t *hits;
+    float *points;               // dense output, layout chosen by the kernel
+};
+
+struct V3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+
+// OpenCL length() over xyz: sqrt((x*x + y*y) + z*z).
+__device__ __forceinline__ float length3(V3 v) { return tgk_sqrtf((v.x * v.x + v.y * v.y) + v.z * v.z); }
+
+// ------------------------------------------------------------- RNG ------
+// MWC64X with A = 4294883355 and the reference's seeding (core.cl:180-214).
+
+__device__ __forceinline__ uint32_t mwc64x(uint2 &s)
+{
+    const uint32_t A = 4294883355u;
+    const uint32_t res = s.x ^ s.y;
+    const uint32_t hi = __umulhi(s.x, A);
+    const uint32_t x = s.x * A + s.y;
+    s.y = hi + (x < s.y ? 1u : 0u);
+    s.x = x;
+    return res;
+}
+
+__device__ __forceinline__ uint32_t rng_mix32(uint32_t v)
+{
+    v ^= v >> 16;
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    v ^= v >> 16;
+    return v;
+}
+
+__device__ __forceinline__ uint2 rng_init(uint32_t seed, uint32_t stream_id)
+{
+    uint32_t x = rng_mix32(seed ^ (stream_id + 0x9e3779b9u));
+    x = x < 1u ? 1u : x;
+    const uint32_t c = rng_mix32(seed + stream_id * 0x85ebca6bu) % 4294883355u;
+    return make_uint2(x, c);
+}
+
+__device__ __forceinline__ float randu(uint2 &s) { return (float)(mwc64x(s) >> 8) * 0x1p-24f; }
+
+__device__ __forceinline__ uint32_t randi(uint2 &s, uint32_t max) { return (uint32_t)(randu(s) * (float)max); }
+
+// The reference seeds stream (batch k, index within batch) from the per-batch
+// host draw; a global seed index maps to it through the batch size.
+__device__ __forceinline__ uint2 seed_stream(const TrackLaunch &p, int64_t global_index)
+{
+    const int64_t batch = global_index / p.batch_size;
+    const uint32_t stream = (uint32_t)(global_index - batch * p.batch_size);
+    return rng_init(p.rng_seeds[batch], stream);
+}
+
+// ------------------------------------------------- world -> voxel -------
+
+// Row dots of the inverse affine with (p, 1), left to right (field.py:91-96).
+__device__ __forceinline__ float affine_row(const float *inv, int row, V3 p)
+{
+    return ((inv[4 * row] * p.x + inv[4 * row + 1] * p.y) + inv[4 * row + 2] * p.z) + inv[4 * row + 3];
+}
+
+// Mask domain membership (domain.py:358-382): cells are half-open,
+// index = floor(v + 0.5).
+__device__ __forceinline__ bool mask_contains(const MaskView &m, V3 p)
+{
+    const float rx = floorf(affine_row(m.inv, 0, p) + 0.5f);
+    const float ry = floorf(affine_row(m.inv, 1, p) + 0.5f);
+    const float rz = floorf(affine_row(m.inv, 2, p) + 0.5f);
+    if (!(rx >= 0.0f && rx < (float)m.dims[0] && ry >= 0.0f && ry < (float)m.dims[1] && rz >= 0.0f &&
+          rz < (float)m.dims[2]))
+        return false;
+    const uint64_t idx = ((uint64_t)(uint32_t)rx * m.dims[1] + (uint32_t)ry) * m.dims[2] + (uint32_t)rz;
+    return m.mask[idx] != 0;
+}
+
+// Field voxel (field.py:88-114): nearest voxel by rint (ties to even);
+// returns the voxel's linear index or -1 outside the image.
+__device__ __forceinline__ int64_t field_voxel(const FieldView &f, V3 p)
+{
+    const float rx = rintf(affine_row(f.inv, 0, p));
+    const float ry = rintf(affine_row(f.inv, 1, p));
+    const float rz = rintf(affine_row(f.inv, 2, p));
+    if (!(rx >= 0.0f && rx < (float)f.dims[0] && ry >= 0.0f && ry < (float)f.dims[1] && rz >= 0.0f &&
+          rz < (float)f.dims[2]))
+        return -1;
+    return ((int64_t)(uint32_t)rx * f.dims[1] + (uint32_t)ry) * f.dims[2] + (uint32_t)rz;
+}
+
+// ------------------------------------------------------------ BVH -------
+// Closest triangle hit on the segment start -> end with t in (1e-6, 1]
+// (bvh.cl:16-131): slab test per node, Moller-Trumbore per triangle, stack of
+// 64, right child pushed first so the left subtree is visited first, and a
+// triangle at an equal t replaces the current hit.
+
+__device__ __forceinline__ bool bvh_node_hit(const MeshView &m, uint32_t node, const float st[3], const float dir[3],
+                                             float max_t)
+{
+    const float4 lo4 = m.bvh_min[node];
+    const float4 hi4 = m.bvh_max[node];
+    const float lo[3] = {lo4.x, lo4.y, lo4.z};
+    const float hi[3] = {hi4.x, hi4.y, hi4.z};
+    float t_min = 0.0f, t_max = max_t;
+#pragma unroll
+    for (int axis = 0; axis < 3; axis++) {
+        if (fabsf(dir[axis]) < 1e-12f) {
+            if (st[axis] < lo[axis] || st[axis] > hi[axis])
+                return false;
+        } else {
+            const float inv = 1.0f / dir[axis];
+            float near_t = (lo[axis] - st[axis]) * inv;
+            float far_t = (hi[axis] - st[axis]) * inv;
+            if (near_t > far_t) {
+                const float tmp = near_t;
+                near_t = far_t;
+                far_t = tmp;
+            }
+            t_min = fmaxf(t_min, near_t);
+            t_max = fminf(t_max, far_t);
+            if (t_min > t_max)
+                return false;
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool bvh_triangle_hit(const MeshView &m, uint32_t tri, const float st[3],
+                                                 const float dir[3], float max_t, float &hit_t)
+{
+    const uint4 ix = m.triangles[tri];
+    const float4 a4 = m.vertices[ix.x];
+    const float4 b4 = m.vertices[ix.y];
+    const float4 c4 = m.vertices[ix.z];
+    const float e1[3] = {b4.x - a4.x, b4.y - a4.y, b4.z - a4.z};
+    const float e2[3] = {c4.x - a4.x, c4.y - a4.y, c4.z - a4.z};
+    const float p[3] = {dir[1] * e2[2] - dir[2] * e2[1], dir[2] * e2[0] - dir[0] * e2[2],
+                        dir[0] * e2[1] - dir[1] * e2[0]};
+    const float det = (e1[0] * p[0] + e1[1] * p[1]) + e1[2] * p[2];
+    if (fabsf(det) < 1e-10f)
+        return false;
+    const float inv = 1.0f / det;
+    const float off[3] = {st[0] - a4.x, st[1] - a4.y, st[2] - a4.z};
+    const float u = ((off[0] * p[0] + off[1] * p[1]) + off[2] * p[2]) * inv;
+    if (u < -1e-6f || u > 1.0f + 1e-6f)
+        return false;
+    const float q[3] = {off[1] * e1[2] - off[2] * e1[1], off[2] * e1[0] - off[0] * e1[2],
+                        off[0] * e1[1] - off[1] * e1[0]};
+    const float v = ((dir[0] * q[0] + dir[1] * q[1]) + dir[2] * q[2]) * inv;
+    if (v < -1e-6f || u + v > 1.0f + 1e-6f)
+        return false;
+    const float t = ((e2[0] * q[0] + e2[1] * q[1]) + e2[2] * q[2]) * inv;
+    if (t <= 1e-6f || t > max_t)
+        return false;
+    hit_t = t;
+    return true;
+}
+
+__device__ inline bool bvh_segment_hit(const MeshView &m, V3 start, V3 end, uint32_t &hit, float &hit_t)
+{
+    const float st[3] = {start.x, start.y, start.z};
+    const float dir[3] = {end.x - start.x, end.y - start.y, end.z - start.z};
+    int32_t stack[64];
+    uint32_t size = 1;
+    stack[0] = 0;
+    hit_t = 1.0f;
+    hit = 0;
+    bool found = false;
+    while (size > 0) {
+        const uint32_t node = (uint32_t)stack[--size];
+        if (!bvh_node_hit(m, node, st, dir, hit_t))
+            continue;
+        const int4 nd = m.bvh_nodes[node];
+        if (nd.w > 0) {
+            for (int i = 0; i < nd.w; i++) {
+                const uint32_t tri = m.bvh_order[nd.z + i];
+                float t;
+                if (bvh_triangle_hit(m, tri, st, dir, hit_t, t)) {
+                    hit_t = t;
+                    hit = tri;
+                    found = true;
+                }
+            }
+        } else {
+            if (size + 2 > 64)
+                return found;
+            stack[size++] = nd.y;
+            stack[size++] = nd.x;
+        }
+    }
+    return found;
+}
+
+// True when the culling grid proves that a segment from `start` of squared
+// length `len2` cannot reach any boundary triangle.
+__device__ __forceinline__ bool mesh_far(const MeshView &m, V3 start, float len2)
+{
+    if (!(len2 <= m.reach * m.reach))
+        return false;
+    const float rx = floorf(affine_row(m.near_inv, 0, start) + 0.5f);
+    const float ry = floorf(affine_row(m.near_inv, 1, start) + 0.5f);
+    const float rz = floorf(affine_row(m.near_inv, 2, start) + 0.5f);
+    if (!(rx >= 0.0f && rx < (float)m.near_dims[0] && ry >= 0.0f && ry < (float)m.near_dims[1] && rz >= 0.0f &&
+          rz < (float)m.near_dims[2]))
+        return false;
+    const uint64_t idx =
+        ((uint64_t)(uint32_t)rx * m.near_dims[1] + (uint32_t)ry) * m.near_dims[2] + (uint32_t)rz;
+    return m.near[idx] == 0;
+}
+
+}  // namespace tgk

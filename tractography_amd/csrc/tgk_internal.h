@@ -1,0 +1,58 @@
+// tgk_internal.h -- kernel launchers shared between the kernel translation
+// units and the C-ABI layer. Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tgk_device.h"
+
+namespace tgk {
+
+// Plain FP32 path (tgk_plain.hip): one lane per seed, point-major dense output.
+hipError_t launch_track_plain(const TrackLaunch &p, hipStream_t stream);
+hipError_t launch_compact_point_major(const float *dense, int64_t n_seeds, uint32_t n_points,
+                                      const uint32_t *lengths, const int64_t *offsets, float *out,
+                                      hipStream_t stream);
+
+// Fast path (tgk_fast.hip): persistent lanes with refill, seed-major output.
+struct FastLaunch {
+    TrackLaunch track;
+    uint32_t *queue;        // one u32 work counter, zeroed before launch
+    int32_t precision;      // TGK_PRECISION_*
+    const void *coeffs;     // transformed field (layout per precision)
+    uint32_t coeff_stride;  // elements per voxel in `coeffs`
+};
+hipError_t launch_track_fast(const FastLaunch &p, int grid_blocks, hipStream_t stream);
+int fast_grid_blocks(int device);
+hipError_t launch_compact_seed_major(const float *dense, int64_t n_seeds, uint32_t n_points,
+                                     const uint32_t *lengths, const int64_t *offsets, float *out,
+                                     hipStream_t stream);
+hipError_t launch_field_transform(const float *field, int64_t n_voxels, uint32_t stride, int32_t precision,
+                                  void *out, uint32_t out_stride, hipStream_t stream);
+
+// One-op drivers (tgk_plain.hip).
+hipError_t launch_evaluate(int model, int K, const float *params, const float *orientations, int64_t n, float *out,
+                           hipStream_t stream);
+hipError_t launch_update(const AlgoParams &a, uint32_t rng_seed, const float *params, const float *orientations,
+                         int64_t n, float *out, hipStream_t stream);
+hipError_t launch_field_evaluate(const FieldView &f, const float *points, int64_t n, float *out, hipStream_t stream);
+hipError_t launch_contains(const MaskView &m, const float *points, int64_t n, uint8_t *out, hipStream_t stream);
+hipError_t launch_bvh_hit(const MeshView &m, const float *starts, const float *ends, int64_t n, uint8_t *found,
+                          uint32_t *hit, float *t, hipStream_t stream);
+hipError_t launch_math(int fn, const float *x, const float *y, int64_t n, float *out, hipStream_t stream);
+hipError_t launch_domain_seed(const float4 *vertices, const uint4 *triangles, const float *cumulative,
+                              uint32_t n_triangles, uint32_t rng_seed, int64_t n, float *out, hipStream_t stream);
+
+// Seeding from FOD (tgk_seeds.hip).
+hipError_t launch_seeds_from_fod(int model, int K, const float *fod, const float *voxels, uint32_t nnz,
+                                 const float *affine, uint32_t *states, int64_t n, int32_t method, float *out,
+                                 void *workspace, size_t workspace_bytes, hipStream_t stream);
+size_t seeds_from_fod_workspace_bytes(int model, int K, int64_t n, int32_t method);
+
+// Exclusive scan of u32 lengths into i64 offsets (offsets[n] = total).
+hipError_t launch_offsets(const uint32_t *lengths, int64_t n, int64_t *offsets, void *workspace,
+                          size_t workspace_bytes, hipStream_t stream);
+size_t offsets_workspace_bytes(int64_t n);
+
+}  // namespace tgk

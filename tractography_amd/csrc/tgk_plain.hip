@@ -1,0 +1,413 @@
+// tgk_plain.hip -- the plain FP32 tracking path and the one-op drivers.
+//
+// One integrator per seed, one lane per seed, exactly the reference's loop
+// (tracking.py:112-148). Compiled with -ffp-contract=off: every result is bit
+// identical to the FP32 host restatement.
+//
+// Output layout: lanes of a wavefront start together and advance `time` by
+// the same step, so all live lanes save on the same iteration. Points are
+// therefore stored point-major, dense[(n * n_seeds + g) * 3 + c], which makes
+// each save of a wavefront one contiguous 768-byte store.
+#include <hip/hip_runtime.h>
+
+#include "tgk_internal.h"
+#include "tgk_plain_model.h"
+
+namespace tgk {
+
+namespace {
+
+__device__ __forceinline__ void store_point_major(float *dense, int64_t n_seeds, uint32_t n, int64_t g, V3 p)
+{
+    float *dst = dense + ((int64_t)n * n_seeds + g) * 3;
+    dst[0] = p.x;
+    dst[1] = p.y;
+    dst[2] = p.z;
+}
+
+template <int MODEL, int ALGO, int DOMAIN, int KT>
+__global__ __launch_bounds__(256) void track_plain_kernel(TrackLaunch p)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= p.n_seeds)
+        return;
+    const float4 s0 = reinterpret_cast<const float4 *>(p.seeds)[2 * g];
+    const float4 s1 = reinterpret_cast<const float4 *>(p.seeds)[2 * g + 1];
+    V3 point = v3(s0.x, s0.y, s0.z);
+    V3 o = v3(s1.x, s1.y, s1.z);
+    uint2 rng = make_uint2(0u, 0u);
+    if constexpr (ALGO == ALGO_DIFFUSION)
+        rng = seed_stream(p, p.first_seed + g);
+
+    store_point_major(p.points, p.n_seeds, 0, g, point);
+    const float step = p.algo.step_size;
+    const float save = p.save_at;
+    const uint32_t stride = p.field.stride;
+    uint32_t n = 1, steps = 0, hit = NO_HIT;
+    float time = 0.0f;
+    V3 previous = point;
+    bool first = true;
+    while (n < p.n_points) {
+        if constexpr (DOMAIN == DOMAIN_MASK) {
+            if (!mask_contains(p.mask, point))
+                break;
+        } else {
+            if (!first) {
+                const float dx = point.x - previous.x, dy = point.y - previous.y, dz = point.z - previous.z;
+                if (!mesh_far(p.mesh, previous, (dx * dx + dy * dy) + dz * dz)) {
+                    uint32_t tri;
+                    float t;
+                    if (bvh_segment_hit(p.mesh, previous, point, tri, t)) {
+                        hit = tri;
+                        break;
+                    }
+                }
+            }
+        }
+        const int64_t voxel = field_voxel(p.field, point);
+        if (voxel < 0)
+            break;
+        o = plain::update<MODEL, ALGO, KT>(p.algo, p.field.data + voxel * stride, o, rng);
+        steps++;
+        previous = point;
+        point.x += step * o.x;
+        point.y += step * o.y;
+        point.z += step * o.z;
+        first = false;
+        time += step;
+        if (time >= save) {
+            time -= save;
+            store_point_major(p.points, p.n_seeds, n, g,
+                              v3(point.x - time * o.x, point.y - time * o.y, point.z - time * o.z));
+            n++;
+        }
+    }
+    p.lengths[g] = n;
+    p.steps[g] = steps;
+    p.hits[g] = hit;
+}
+
+template <int MODEL, int ALGO, int DOMAIN>
+hipError_t launch_plain_kt(const TrackLaunch &p, hipStream_t stream)
+{
+    const dim3 block(256);
+    const dim3 grid((unsigned)((p.n_seeds + 255) / 256));
+    if (MODEL == MODEL_SH && p.algo.n_parameters == 45)
+        track_plain_kernel<MODEL, ALGO, DOMAIN, 45><<<grid, block, 0, stream>>>(p);
+    else
+        track_plain_kernel<MODEL, ALGO, DOMAIN, 0><<<grid, block, 0, stream>>>(p);
+    return hipGetLastError();
+}
+
+template <int MODEL, int ALGO>
+hipError_t launch_plain_domain(const TrackLaunch &p, hipStream_t stream)
+{
+    if (p.domain_kind == DOMAIN_MESH)
+        return launch_plain_kt<MODEL, ALGO, DOMAIN_MESH>(p, stream);
+    return launch_plain_kt<MODEL, ALGO, DOMAIN_MASK>(p, stream);
+}
+
+template <int MODEL>
+hipError_t launch_plain_algo(const TrackLaunch &p, hipStream_t stream)
+{
+    if (p.algo.algorithm == ALGO_DIFFUSION)
+        return launch_plain_domain<MODEL, ALGO_DIFFUSION>(p, stream);
+    return launch_plain_domain<MODEL, ALGO_TRANSPORT>(p, stream);
+}
+
+// Dense point-major -> ragged (offsets[g] + n). Threads walk the dense array
+// in order so reads are coalesced.
+__global__ __launch_bounds__(256) void compact_point_major_kernel(const float *__restrict__ dense, int64_t n_seeds,
+                                                                  uint32_t n_points,
+                                                                  const uint32_t *__restrict__ lengths,
+                                                                  const int64_t *__restrict__ offsets,
+                                                                  float *__restrict__ out)
+{
+    const int64_t total = n_seeds * (int64_t)n_points;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t n = t / n_seeds;
+        const int64_t g = t - n * n_seeds;
+        if (n < (int64_t)lengths[g]) {
+            const float *src = dense + t * 3;
+            float *dst = out + (offsets[g] + n) * 3;
+            dst[0] = src[0];
+            dst[1] = src[1];
+            dst[2] = src[2];
+        }
+    }
+}
+
+// ------------------------------------------------------ one-op drivers ---
+
+// Loads K (<= 45) coefficients from an unpadded row.
+__device__ __forceinline__ void load_row_scalar(const float *__restrict__ row, int K, float c[45])
+{
+#pragma unroll
+    for (int i = 0; i < 45; i++)
+        c[i] = i < K ? row[i] : 0.0f;
+}
+
+template <int MODEL>
+__device__ __forceinline__ plain::ModelValue evaluate_row(const float *__restrict__ row, int K, V3 o)
+{
+    if constexpr (MODEL == MODEL_DTI) {
+        alignas(16) const float m[8] = {row[0], row[1], row[2], row[3], row[4], row[5], 0.0f, 0.0f};
+        return plain::dti_model(m, o);
+    } else {
+        float c[45];
+        load_row_scalar(row, K, c);
+        float az, col;
+        plain::cart2sph(o, az, col);
+        float v, th, ph;
+        plain::sh45_contract(c, az, col, v, th, ph);
+        const float d = plain::dsoftplus(v);
+        plain::ModelValue r;
+        r.value = plain::softplus(v);
+        r.dtheta = th * d;
+        r.dphi = ph * d;
+        return r;
+    }
+}
+
+// local.py:317-334
+template <int MODEL>
+__global__ void evaluate_kernel(const float *__restrict__ params, int K, const float *__restrict__ orientations,
+                                int64_t n, float *__restrict__ out)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n)
+        return;
+    const V3 o = v3(orientations[3 * g], orientations[3 * g + 1], orientations[3 * g + 2]);
+    const plain::ModelValue mv = evaluate_row<MODEL>(params + (int64_t)K * g, K, o);
+    out[3 * g] = mv.value;
+    out[3 * g + 1] = mv.dtheta;
+    out[3 * g + 2] = mv.dphi;
+}
+
+// algorithm.py:739-758: stream id = row index.
+template <int MODEL, int ALGO>
+__global__ void update_kernel(AlgoParams a, uint32_t rng_seed, const float *__restrict__ params,
+                              const float *__restrict__ orientations, int64_t n, float *__restrict__ out)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n)
+        return;
+    const int K = a.n_parameters;
+    const float *row = params + (int64_t)K * g;
+    // Stage the row in the padded layout update() reads.
+    alignas(16) float padded[48];
+#pragma unroll
+    for (int i = 0; i < 48; i++)
+        padded[i] = i < K ? row[i] : 0.0f;
+    uint2 rng = rng_init(rng_seed, (uint32_t)g);
+    const V3 o = v3(orientations[3 * g], orientations[3 * g + 1], orientations[3 * g + 2]);
+    const V3 r = plain::update<MODEL, ALGO, 0>(a, padded, o, rng);
+    out[3 * g] = r.x;
+    out[3 * g + 1] = r.y;
+    out[3 * g + 2] = r.z;
+}
+
+// field.py:158-175
+__global__ void field_evaluate_kernel(FieldView f, const float *__restrict__ points, int64_t n,
+                                      float *__restrict__ out)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n)
+        return;
+    const int64_t voxel = field_voxel(f, v3(points[3 * g], points[3 * g + 1], points[3 * g + 2]));
+    for (uint32_t i = 0; i < f.n_parameters; i++)
+        out[(int64_t)f.n_parameters * g + i] = voxel >= 0 ? f.data[voxel * f.stride + i] : 0.0f;
+}
+
+// domain.py:457-468
+__global__ void contains_kernel(MaskView m, const float *__restrict__ points, int64_t n, uint8_t *__restrict__ out)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n)
+        return;
+    out[g] = mask_contains(m, v3(points[3 * g], points[3 * g + 1], points[3 * g + 2])) ? 1 : 0;
+}
+
+__global__ void bvh_hit_kernel(MeshView m, const float *__restrict__ starts, const float *__restrict__ ends,
+                               int64_t n, uint8_t *__restrict__ found, uint32_t *__restrict__ hit,
+                               float *__restrict__ t)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n)
+        return;
+    uint32_t tri;
+    float ht;
+    const bool f = bvh_segment_hit(m, v3(starts[3 * g], starts[3 * g + 1], starts[3 * g + 2]),
+                                   v3(ends[3 * g], ends[3 * g + 1], ends[3 * g + 2]), tri, ht);
+    found[g] = f ? 1 : 0;
+    hit[g] = tri;
+    t[g] = ht;
+}
+
+__global__ void math_kernel(int fn, const float *__restrict__ x, const float *__restrict__ y, int64_t n,
+                            float *__restrict__ out)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n)
+        return;
+    const float a = x[g];
+    float r;
+    switch (fn) {
+    case 0: r = tgk_sinf(a); break;
+    case 1: r = tgk_cosf(a); break;
+    case 2: r = tgk_atan2f(a, y[g]); break;
+    case 3: r = tgk_acosf(a); break;
+    case 4: r = tgk_expf(a); break;
+    case 5: r = tgk_logf(a); break;
+    case 6: r = tgk_sqrtf(a); break;
+    default: r = a / y[g]; break;
+    }
+    out[g] = r;
+}
+
+// domain.py:131-166 driven as domain.py:530-556; normalize(v) = v / |v|.
+__global__ void domain_seed_kernel(const float4 *__restrict__ vertices, const uint4 *__restrict__ triangles,
+                                   const float *__restrict__ cumulative, uint32_t n_triangles, uint32_t rng_seed,
+                                   int64_t n, float *__restrict__ out)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n)
+        return;
+    uint2 rng = rng_init(rng_seed, (uint32_t)g);
+    const float target = randu(rng);
+    uint32_t lower = 0, upper = n_triangles;
+    while (lower < upper) {
+        const uint32_t middle = lower + (upper - lower) / 2;
+        if (cumulative[middle] <= target)
+            lower = middle + 1;
+        else
+            upper = middle;
+    }
+    const uint32_t tri = lower < n_triangles - 1 ? lower : n_triangles - 1;
+    const uint4 ix = triangles[tri];
+    const float4 a = vertices[ix.x], b = vertices[ix.y], c = vertices[ix.z];
+    const float root_u = tgk_sqrtf(randu(rng));
+    const float v = randu(rng);
+    const float wa = 1.0f - root_u, wb = root_u * (1.0f - v), wc = root_u * v;
+    const float sx = (wa * a.x + wb * b.x) + wc * c.x;
+    const float sy = (wa * a.y + wb * b.y) + wc * c.y;
+    const float sz = (wa * a.z + wb * b.z) + wc * c.z;
+    const float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
+    const float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
+    const float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+    const float len = tgk_sqrtf((nx * nx + ny * ny) + nz * nz);
+    const float ox = -(nx / len), oy = -(ny / len), oz = -(nz / len);
+    float *row = out + 8 * g;
+    row[0] = sx + 1e-4f * ox;
+    row[1] = sy + 1e-4f * oy;
+    row[2] = sz + 1e-4f * oz;
+    row[3] = 1.0f;
+    row[4] = ox;
+    row[5] = oy;
+    row[6] = oz;
+    row[7] = 0.0f;
+}
+
+inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+}  // namespace
+
+hipError_t launch_track_plain(const TrackLaunch &p, hipStream_t stream)
+{
+    if (p.n_seeds == 0)
+        return hipSuccess;
+    if (p.algo.model == MODEL_DTI)
+        return launch_plain_algo<MODEL_DTI>(p, stream);
+    return launch_plain_algo<MODEL_SH>(p, stream);
+}
+
+hipError_t launch_compact_point_major(const float *dense, int64_t n_seeds, uint32_t n_points,
+                                      const uint32_t *lengths, const int64_t *offsets, float *out,
+                                      hipStream_t stream)
+{
+    if (n_seeds == 0)
+        return hipSuccess;
+    const int64_t total = n_seeds * (int64_t)n_points;
+    const int64_t blocks = (total + 255) / 256;
+    compact_point_major_kernel<<<(unsigned)(blocks < 262144 ? blocks : 262144), 256, 0, stream>>>(
+        dense, n_seeds, n_points, lengths, offsets, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_evaluate(int model, int K, const float *params, const float *orientations, int64_t n, float *out,
+                           hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    if (model == MODEL_DTI)
+        evaluate_kernel<MODEL_DTI><<<blocks_for(n, 128), 128, 0, stream>>>(params, K, orientations, n, out);
+    else
+        evaluate_kernel<MODEL_SH><<<blocks_for(n, 128), 128, 0, stream>>>(params, K, orientations, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_update(const AlgoParams &a, uint32_t rng_seed, const float *params, const float *orientations,
+                         int64_t n, float *out, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    const unsigned blocks = blocks_for(n, 128);
+    if (a.model == MODEL_DTI) {
+        if (a.algorithm == ALGO_DIFFUSION)
+            update_kernel<MODEL_DTI, ALGO_DIFFUSION><<<blocks, 128, 0, stream>>>(a, rng_seed, params, orientations, n, out);
+        else
+            update_kernel<MODEL_DTI, ALGO_TRANSPORT><<<blocks, 128, 0, stream>>>(a, rng_seed, params, orientations, n, out);
+    } else {
+        if (a.algorithm == ALGO_DIFFUSION)
+            update_kernel<MODEL_SH, ALGO_DIFFUSION><<<blocks, 128, 0, stream>>>(a, rng_seed, params, orientations, n, out);
+        else
+            update_kernel<MODEL_SH, ALGO_TRANSPORT><<<blocks, 128, 0, stream>>>(a, rng_seed, params, orientations, n, out);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_field_evaluate(const FieldView &f, const float *points, int64_t n, float *out, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    field_evaluate_kernel<<<blocks_for(n, 256), 256, 0, stream>>>(f, points, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_contains(const MaskView &m, const float *points, int64_t n, uint8_t *out, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    contains_kernel<<<blocks_for(n, 256), 256, 0, stream>>>(m, points, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bvh_hit(const MeshView &m, const float *starts, const float *ends, int64_t n, uint8_t *found,
+                          uint32_t *hit, float *t, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    bvh_hit_kernel<<<blocks_for(n, 128), 128, 0, stream>>>(m, starts, ends, n, found, hit, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_math(int fn, const float *x, const float *y, int64_t n, float *out, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    math_kernel<<<blocks_for(n, 256), 256, 0, stream>>>(fn, x, y, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_domain_seed(const float4 *vertices, const uint4 *triangles, const float *cumulative,
+                              uint32_t n_triangles, uint32_t rng_seed, int64_t n, float *out, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    domain_seed_kernel<<<blocks_for(n, 256), 256, 0, stream>>>(vertices, triangles, cumulative, n_triangles,
+                                                               rng_seed, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace tgk

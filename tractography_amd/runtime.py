@@ -1,0 +1,87 @@
+"""The device runtime: one GPU, one HIP stream (replaces opencl.py).
+
+``HipRuntime`` owns a ``tgk_runtime`` handle of libtgk.so. Fields and domains
+upload their arrays once per runtime and keep the device handles, so repeated
+tracking calls (and every batch of one call) reuse the volumes in HBM instead
+of re-uploading them as the reference does per batch.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import weakref
+
+import numpy as np
+
+from . import _native
+
+PATHS = {"plain": _native.PATH_PLAIN, "fast": _native.PATH_FAST}
+
+
+class HipRuntime:
+    """A GPU execution context for the tracking kernels.
+
+    Args:
+        device: HIP device index.
+        path: ``"plain"`` (the reference's integrator, bit-exact with the FP32
+            host restatement; default) or ``"fast"`` (persistent lanes with
+            refill, transcendental-free model evaluation, FP64 arc length).
+        precision: coefficient storage read by the fast path: ``"fp32"``,
+            ``"bf16x2"`` (split hi + lo bfloat16), ``"fp16"`` or ``"bf16"``.
+    """
+
+    def __init__(self, device: int = 0, *, path: str = "plain", precision: str = "fp32"):
+        if path not in PATHS:
+            raise ValueError(f"path must be one of {sorted(PATHS)}")
+        if precision not in _native.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_native.PRECISIONS)}")
+        lib = _native.lib()
+        handle = ctypes.c_void_p()
+        _native.check(lib.tgk_runtime_create(int(device), ctypes.byref(handle)))
+        self._handle = handle
+        self.device = int(device)
+        self.path = path
+        self.precision = precision
+        self._finalizer = weakref.finalize(self, lib.tgk_runtime_destroy, handle)
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._handle
+
+    @property
+    def path_id(self) -> int:
+        return PATHS[self.path]
+
+    def finish(self) -> None:
+        """Wait for all work submitted to this runtime."""
+        _native.check(_native.lib().tgk_runtime_synchronize(self._handle))
+
+    def info(self) -> dict:
+        name = ctypes.create_string_buffer(256)
+        cus = ctypes.c_int32()
+        total = ctypes.c_int64()
+        _native.check(
+            _native.lib().tgk_runtime_info(self._handle, name, 256, ctypes.byref(cus), ctypes.byref(total))
+        )
+        return {"name": name.value.decode(), "compute_units": cus.value, "total_bytes": total.value}
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        _native.check(_native.lib().tgk_runtime_stream(self._handle, ctypes.byref(s)))
+        return s.value or 0
+
+    def close(self) -> None:
+        self._finalizer()
+
+
+class DeviceHandle:
+    """A libtgk object (field or domain) owned by one runtime."""
+
+    def __init__(self, runtime: HipRuntime, handle: ctypes.c_void_p, destroy):
+        self.runtime = runtime
+        self.handle = handle
+        self._finalizer = weakref.finalize(self, destroy, handle)
+
+
+def as_float32_affine(matrix) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(matrix, dtype=np.float32).reshape(4, 4))

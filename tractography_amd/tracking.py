@@ -1,0 +1,142 @@
+"""Tracking driver (tracking.py of the reference).
+
+``tractogram`` keeps the reference's signature and semantics: one streamline
+per seed, points saved every ``save_at`` of arc length (back-interpolated to
+the exact save distance), the seed always included, capacity
+floor(maximum_streamline_length / save_at) points. The work runs in libtgk.so
+on the GPU; volumes stay resident in HBM across calls and the dense per-seed
+output of the reference is replaced by on-device compaction into one ragged
+point array.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native
+from .algorithm import BaseAlgorithm
+from .domain import BaseDomain
+from .field import BaseField
+from .image import Streamlines, Tractogram
+from .runtime import HipRuntime
+from .seeds import validate_seed_array
+
+
+@dataclass
+class TrackResult:
+    """Raw arrays of one tracking run (seed order).
+
+    lengths: saved points per streamline (seed included), u32.
+    steps: orientation updates per streamline, u32.
+    hits: boundary triangle slot crossed (mesh domains), 0xffffffff if none.
+    points: (sum(lengths), 3) float32 ragged points.
+    """
+
+    lengths: np.ndarray
+    steps: np.ndarray
+    hits: np.ndarray
+    points: np.ndarray
+
+    def streamlines(self) -> Streamlines:
+        return Streamlines(self.points, self.lengths.astype(np.int64))
+
+
+def point_capacity(maximum_streamline_length: float, save_at: float) -> int:
+    """floor(maximum_streamline_length / save_at) (tracking.py:89)."""
+    n_steps = int(np.floor(maximum_streamline_length / save_at))
+    if n_steps <= 0:
+        # The reference cannot allocate its zero-sized point buffer either
+        # (opencl.py:254-262).
+        raise ValueError("nbytes must be positive.")
+    return n_steps
+
+
+def track(
+    algorithm: BaseAlgorithm,
+    domain: BaseDomain,
+    field: BaseField,
+    seeds: np.ndarray,
+    rng_seeds,
+    batch_size: int,
+    save_at: float,
+    maximum_streamline_length: float,
+    runtime: HipRuntime,
+    path: str | None = None,
+) -> TrackResult:
+    """Track every seed in one device call.
+
+    Seed g draws its noise from rng_init(rng_seeds[g // batch_size],
+    g % batch_size), the stream the reference's per-batch calls give it.
+    """
+    n_seeds = len(seeds)
+    n_points = point_capacity(maximum_streamline_length, save_at)
+    rng = np.ascontiguousarray(np.asarray(rng_seeds, dtype=np.uint64) % 2**32, dtype=np.uint32)
+    if len(rng) < (n_seeds + batch_size - 1) // batch_size:
+        raise ValueError("one rng seed per batch is required")
+    field_device = field.upload(runtime)
+    domain_device = domain.upload(runtime)
+    lib = _native.lib()
+    path_id = runtime.path_id if path is None else {"plain": _native.PATH_PLAIN, "fast": _native.PATH_FAST}[path]
+    if path_id == _native.PATH_FAST:
+        _native.check(lib.tgk_field_set_fast_precision(field_device.handle, _native.PRECISIONS[runtime.precision]))
+    handle = ctypes.c_void_p()
+    _native.check(
+        lib.tgk_track(
+            runtime.handle, domain_device.handle, field_device.handle, algorithm.descriptor(), path_id,
+            _native.ptr(seeds), n_seeds, _native.ptr(rng), int(batch_size), n_points, float(np.float32(save_at)),
+            ctypes.byref(handle),
+        )
+    )
+    try:
+        n = ctypes.c_int64()
+        total = ctypes.c_int64()
+        _native.check(lib.tgk_tracks_size(handle, ctypes.byref(n), ctypes.byref(total)))
+        lengths = np.empty(n.value, dtype=np.uint32)
+        steps = np.empty(n.value, dtype=np.uint32)
+        hits = np.empty(n.value, dtype=np.uint32)
+        points = np.empty((total.value, 3), dtype=np.float32)
+        _native.check(
+            lib.tgk_tracks_copy(handle, _native.ptr(lengths), _native.ptr(steps), _native.ptr(hits), _native.ptr(points))
+        )
+    finally:
+        lib.tgk_tracks_destroy(handle)
+    return TrackResult(lengths, steps, hits, points)
+
+
+def tractogram(
+    algorithm: BaseAlgorithm,
+    domain: BaseDomain,
+    field: BaseField,
+    seeds: np.ndarray,
+    save_at: float = 1.0,
+    maximum_streamline_length: float = 300.0,
+    runtime: HipRuntime | None = None,
+) -> Tractogram:
+    """Track streamlines from position-and-orientation seeds (tracking.py:22-178).
+
+    Raises:
+        ValueError: incompatible local model and field, or invalid seeds.
+
+    An empty seed array returns an empty tractogram without creating a
+    runtime.
+    """
+    seeds = validate_seed_array(seeds)
+    if algorithm.local_model.n_parameters != field.n_parameters:
+        raise ValueError(
+            "Incompatible local model and field, they do not have the "
+            "same number of parameters: "
+            f"{algorithm.local_model.n_parameters} != {field.n_parameters}"
+        )
+    n_seeds = seeds.shape[0]
+    if n_seeds == 0:
+        return Tractogram([], affine_to_rasmm=np.eye(4))
+    if runtime is None:
+        runtime = HipRuntime()
+    result = track(
+        algorithm, domain, field, seeds, [algorithm.get_rng_seed()], n_seeds, save_at, maximum_streamline_length,
+        runtime,
+    )
+    return Tractogram(result.streamlines(), affine_to_rasmm=np.eye(4))
