@@ -312,19 +312,27 @@ class DeviceRun:
         total_steps = int(self.d_steps.to(torch.int64).sum().item())
         return wall, kernel, total_steps
 
-    def results(self):
-        torch = self.torch
-        lengths = self.d_lengths.cpu().numpy().view(np.uint32)
-        total = int(self.d_offsets[-1].item())
+    def results(self, n_first: int):
+        """lengths, steps and ragged points of the first n_first seeds."""
+        lengths = self.d_lengths[:n_first].cpu().numpy().view(np.uint32)
+        steps = self.d_steps[:n_first].cpu().numpy().view(np.uint32)
+        total = int(self.d_offsets[n_first].item())
         points = self.d_points[: total * 3].cpu().numpy().reshape(-1, 3)
-        return lengths, self.d_steps.cpu().numpy().view(np.uint32), points
+        return lengths, steps, points
 
 
-def cpu_baseline(cfg, seconds: float = 15.0):
-    """FP32 host restatement (OpenMP over seeds, all host cores) on a leading
-    sample of the workload's seeds, sized to ~`seconds` of wall time."""
+def host_leg(cfg, gpu_results: dict, seconds: float = 15.0, n_accuracy: int = 400):
+    """The host restatement's leg of the benchmark.
+
+    cpu_baseline: the FP32 restatement (OpenMP over seeds, all host cores) on
+    a leading sample of the workload's seeds sized to ~`seconds` of wall time.
+    accuracy: deviation of each GPU path from the FP64 restatement on the
+    first `n_accuracy` seeds (same noise streams), measured up to the first
+    possible voxel flip.
+    """
     sys.path.insert(0, str(ROOT / "tests"))
-    import oracle  # noqa: E402  (the host restatement; CPU baseline only)
+    import deviation  # noqa: E402
+    import oracle  # noqa: E402  (the host restatement)
 
     import tractography_amd as tg
 
@@ -344,9 +352,20 @@ def cpu_baseline(cfg, seconds: float = 15.0):
             break
         sample = min(len(cfg["seeds"]), int(sample * max(2.0, seconds / max(dt, 1e-3))))
     steps = int(r["steps"].astype(np.int64).sum())
-    return {"value": steps / dt, "unit": "streamline steps/s", "cores": threads, "kind": "port",
-            "sample": f"first {len(seeds)} seeds of the workload, {steps} steps in {dt:.1f} s "
-                      "(FP32 host restatement, OpenMP)"}
+    baseline = {"value": steps / dt, "unit": "streamline steps/s", "cores": threads, "kind": "port",
+                "sample": f"first {len(seeds)} seeds of the workload, {steps} steps in {dt:.1f} s "
+                          "(FP32 host restatement, OpenMP)"}
+    accuracy = {}
+    n_acc = min(n_accuracy, len(cfg["seeds"]))
+    ref64 = oracle.track(domain, field, algorithm, cfg["seeds"][:n_acc], rng_seeds, 50_000, cfg["save_at"],
+                         cfg["max_length"], precision=64, threads=threads)
+    for name, (lengths, steps_, points) in gpu_results.items():
+        rep = deviation.report(lengths[:n_acc], steps_[:n_acc], points, ref64)
+        accuracy[name] = {"max_point_deviation_voxel": rep["max_deviation"],
+                          "compared_fraction": rep["compared_fraction"],
+                          "length_equal_fraction": rep["length_equal_fraction"],
+                          "seeds": n_acc}
+    return baseline, accuracy
 
 
 def main():
@@ -383,6 +402,7 @@ def main():
     run = DeviceRun(tg, torch, runtime, cfg)
     paths = [p for p in args.paths.split(",") if p]
     results = {}
+    gpu_results = {}
     for name in paths:
         path = _native.PATH_FAST if name == "fast" else _native.PATH_PLAIN
         try:
@@ -391,6 +411,7 @@ def main():
             print(f"[bench] {name} path unavailable: {e}", file=sys.stderr, flush=True)
             continue
         results[name] = {"ms_per_step": wall, "kernel_ms": kernel, "steps": total}
+        gpu_results[name] = run.results(min(400, run.n))
         print(f"[bench] {name}: {wall:.1f} ms/step, kernel {kernel:.1f} ms, {total} orientation updates, "
               f"{total / wall * 1e3:.3e} steps/s", file=sys.stderr, flush=True)
     headline = "fast" if "fast" in results else "plain"
@@ -423,7 +444,7 @@ def main():
         line["speedup_fast_over_plain"] = (results["fast"]["steps"] / results["fast"]["ms_per_step"]) / (
             results["plain"]["steps"] / results["plain"]["ms_per_step"])
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(cfg)
+        line["cpu_baseline"], line["accuracy_vs_fp64"] = host_leg(cfg, gpu_results)
     print(json.dumps(line), flush=True)
 
 

@@ -1,0 +1,47 @@
+"""Trajectory deviation against the FP64 host restatement.
+
+Nearest-voxel lookups make trajectories that differ by a rounding error
+diverge by O(step) once one of them crosses a voxel face the other does not.
+Saved points are therefore compared only while the running maximum of the
+point deviation stays below the FP64 trajectory's running minimum distance
+to a voxel face (the prefix over which both provably used the same voxels);
+the report gives the maximum deviation over those prefixes and how much of
+the streamlines they cover.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def report(lengths, steps, points, ref64) -> dict:
+    """lengths/points: test path (ragged points in seed order); ref64: FP64
+    oracle result (dense points, margins)."""
+    lengths = np.asarray(lengths, dtype=np.int64)
+    offsets = np.concatenate([[0], np.cumsum(lengths)])
+    max_dev = 0.0
+    compared = 0
+    total = 0
+    full = 0
+    for i in range(len(lengths)):
+        n_ref = int(ref64["lengths"][i])
+        n = min(int(lengths[i]), n_ref)
+        total += n_ref
+        got = points[offsets[i] : offsets[i] + n].astype(np.float64)
+        ref = ref64["points"][i, :n]
+        dev = np.linalg.norm(got - ref, axis=1)
+        running = np.maximum.accumulate(dev)
+        ok = running < ref64["margins"][i, :n]
+        k = n if np.all(ok) else int(np.argmin(ok))
+        if k > 0:
+            max_dev = max(max_dev, float(running[k - 1]))
+        compared += k
+        if k == n and int(lengths[i]) == n_ref:
+            full += 1
+    return {
+        "max_deviation": max_dev,
+        "compared_fraction": compared / max(total, 1),
+        "identical_length_and_voxels_fraction": full / max(len(lengths), 1),
+        "length_equal_fraction": float(np.mean(lengths == ref64["lengths"])),
+        "steps_equal_fraction": float(np.mean(np.asarray(steps) == ref64["steps"])),
+    }

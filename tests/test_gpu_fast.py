@@ -1,0 +1,137 @@
+"""GPU accuracy of the fast path against the FP64 host restatement, with the
+plain path's deviation measured the same way for comparison.
+
+Tolerance (benchmark definition): saved points within 1e-3 voxel of the FP64
+trajectory over every prefix on which both used the same voxels."""
+
+import numpy as np
+import pytest
+
+import bench
+import deviation
+import fixtures
+import oracle
+import tractography_amd as tg
+
+pytestmark = pytest.mark.gpu
+
+TOLERANCE = 1e-3
+
+
+@pytest.fixture(scope="module")
+def scene():
+    cfg = bench.config0(seed=2, n_seeds=2000)
+    field = tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4)))
+    domain = tg.domain.Image(tg.Nifti1Image(cfg["mask"], np.eye(4)))
+    return cfg, field, domain
+
+
+@pytest.fixture(scope="module")
+def fast_runtime():
+    return tg.HipRuntime(path="fast")
+
+
+@pytest.mark.parametrize("name", ["transport", "diffusion"])
+def test_fast_sh45_tracks_fp64_oracle(fast_runtime, runtime, scene, name, oracle_threads):
+    cfg, field, domain = scene
+    model = tg.local.RealSymmetricSphericalHarmonics(45)
+    if name == "transport":
+        algorithm = tg.algorithm.Transport(model, step_size=0.1, gamma=1.0)
+    else:
+        algorithm = tg.algorithm.Diffusion(model, step_size=0.1, gamma=1.0, noise_variance=0.1, seed=21)
+    rng = [algorithm.get_rng_seed()]
+    ref = oracle.track(domain, field, algorithm, cfg["seeds"], rng, 2000, 0.25, 80.0, precision=64,
+                       threads=oracle_threads)
+    fast = tg.tracking.track(algorithm, domain, field, cfg["seeds"], rng, 2000, 0.25, 80.0, fast_runtime)
+    plain = tg.tracking.track(algorithm, domain, field, cfg["seeds"], rng, 2000, 0.25, 80.0, runtime)
+    rf = deviation.report(fast.lengths, fast.steps, fast.points, ref)
+    rp = deviation.report(plain.lengths, plain.steps, plain.points, ref)
+    print(f"\n[{name}] fast {rf}\n[{name}] plain {rp}")
+    assert rf["max_deviation"] <= TOLERANCE
+    assert rf["compared_fraction"] > 0.5
+    assert rp["max_deviation"] <= TOLERANCE
+    # Every streamline starts at its seed and saves at most n_points points.
+    assert np.all(fast.lengths >= 1) and np.all(fast.lengths <= 320)
+
+
+def test_fast_generic_k_and_dti(fast_runtime, oracle_threads):
+    fod, wm, _ = fixtures.cross(shape=(14, 14, 3), n_coefficients=15)
+    field = tg.field.Image(fod)
+    domain = tg.domain.Image(wm)
+    np.random.seed(4)
+    seeds = tg.seeds.from_mask(wm, 1500)
+    algorithm = tg.algorithm.Diffusion(tg.local.RealSymmetricSphericalHarmonics(15), step_size=0.05, gamma=2.0,
+                                       noise_variance=0.2, seed=8)
+    rng = [algorithm.get_rng_seed()]
+    got = tg.tracking.track(algorithm, domain, field, seeds, rng, 1500, 0.5, 40.0, fast_runtime)
+    ref = oracle.track(domain, field, algorithm, seeds, rng, 1500, 0.5, 40.0, precision=64, threads=oracle_threads)
+    r = deviation.report(got.lengths, got.steps, got.points, ref)
+    print(f"\n[sh15] fast {r}")
+    assert r["max_deviation"] <= TOLERANCE
+    tensor, wm, _ = fixtures.tensor_cross()
+    field = tg.field.Image(tensor)
+    domain = tg.domain.Image(wm)
+    seeds = tg.seeds.from_mask(wm, 1500)
+    algorithm = tg.algorithm.Transport(tg.local.DiffusionTensorImaging(), step_size=0.01, gamma=20.0)
+    got = tg.tracking.track(algorithm, domain, field, seeds, [0], 1500, 0.25, 30.0, fast_runtime)
+    ref = oracle.track(domain, field, algorithm, seeds, [0], 1500, 0.25, 30.0, precision=64, threads=oracle_threads)
+    r = deviation.report(got.lengths, got.steps, got.points, ref)
+    print(f"\n[dti] fast {r}")
+    assert r["max_deviation"] <= TOLERANCE
+
+
+def test_fast_known_answers(fast_runtime):
+    # test/test_tracking.py:28-100 on the fast path
+    domain = tg.domain.Image(tg.Nifti1Image(np.ones((20, 1, 1), np.uint8), np.eye(4)))
+    data = np.zeros((20, 1, 1, 15), np.float32)
+    data[..., 0] = 1
+    field = tg.field.Image(tg.Nifti1Image(data, np.eye(4)))
+    seeds = np.array([[0, 0, 0, 1, 1, 0, 0, 0]], np.float32)
+    t = tg.algorithm.Transport(tg.local.RealSymmetricSphericalHarmonics(15), step_size=0.25, gamma=0.0)
+    s = np.asarray(tg.tracking.tractogram(t, domain, field, seeds, save_at=0.5, maximum_streamline_length=3.0,
+                                          runtime=fast_runtime).streamlines[0])
+    assert len(s) == 6
+    np.testing.assert_allclose(s[:, 0], np.arange(0.0, 3.0, 0.5), atol=1e-6)
+
+
+@pytest.mark.parametrize("name", ["transport", "diffusion"])
+def test_fast_u0u3_exact_solution(fast_runtime, name):
+    coefficients = fixtures.u0u3_coefficients(0.3, 45)
+    fod = np.broadcast_to(coefficients, (1, 1, 1, 45)).copy()
+    affine = np.diag((1000.0, 1000.0, 1000.0, 1.0))
+    affine[:3, 3] = -0.5
+    image = tg.Nifti1Image(fod, affine)
+    orientation = np.asarray(tg.core.sph2cart(0.5, 0.12, 1.0))
+    seeds = np.array([[10.0, 10.0, 10.0, 1.0, *orientation, 0.0]], np.float32)
+    model = tg.local.RealSymmetricSphericalHarmonics(45)
+    if name == "transport":
+        algorithm = tg.algorithm.Transport(model, step_size=0.001, gamma=1.0)
+    else:
+        algorithm = tg.algorithm.Diffusion(model, step_size=0.001, gamma=1.0, noise_variance=0.0, seed=0)
+    streamline = np.asarray(tg.tractogram(image, seeds, None, algorithm, save_at=0.001, maximum_streamline_length=1.0,
+                                          batch_size=1, runtime=fast_runtime).streamlines[0])
+    u_ref = fixtures.u0u3_reference(0.001, 1.0, 0.12, 0.5, 0.3, len(streamline))
+    expected = np.zeros((len(streamline), 3))
+    expected[0] = [10.0, 10.0, 10.0]
+    for k in range(1, len(streamline)):
+        expected[k] = expected[k - 1] + 0.001 * u_ref[k]
+    np.testing.assert_array_almost_equal(streamline, expected, decimal=3)
+
+
+def test_fast_path_is_deterministic_and_chunk_invariant(fast_runtime, scene):
+    import os
+
+    cfg, field, domain = scene
+    algorithm = tg.algorithm.Diffusion(tg.local.RealSymmetricSphericalHarmonics(45), step_size=0.1,
+                                       noise_variance=0.1, seed=3)
+    rng = [algorithm.get_rng_seed() for _ in range(2)]
+    a = tg.tracking.track(algorithm, domain, field, cfg["seeds"], rng, 1000, 0.25, 30.0, fast_runtime)
+    b = tg.tracking.track(algorithm, domain, field, cfg["seeds"], rng, 1000, 0.25, 30.0, fast_runtime)
+    os.environ["TGK_CHUNK_BYTES"] = str(120 * 12 * 333)
+    try:
+        c = tg.tracking.track(algorithm, domain, field, cfg["seeds"], rng, 1000, 0.25, 30.0, fast_runtime)
+    finally:
+        del os.environ["TGK_CHUNK_BYTES"]
+    for x, y in [(a, b), (a, c)]:
+        np.testing.assert_array_equal(x.lengths, y.lengths)
+        np.testing.assert_array_equal(x.points, y.points)

@@ -5,6 +5,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -82,6 +83,7 @@ struct tgk_runtime {
     int32_t device = 0;
     hipStream_t stream = nullptr;
     DeviceBuffer workspace;
+    DeviceBuffer queue;  // work counter of the persistent fast kernel
 };
 
 struct tgk_field {
@@ -92,7 +94,7 @@ struct tgk_field {
     float inverse_affine[16];
     int32_t fast_precision = TGK_PRECISION_FP32;
     DeviceBuffer fast;              // transformed coefficients for the fast path
-    int32_t fast_cached = -1;       // precision held in `fast`
+    int32_t fast_cached = -1;       // model the transform in `fast` was made for
     uint32_t fast_stride = 0;
 };
 
@@ -517,8 +519,10 @@ int check_track_args(const tgk_domain *domain, const tgk_field *field, const tgk
         return fail(TGK_ERR_INVALID, "batch_size must be positive");
     if (n_points == 0)
         return fail(TGK_ERR_INVALID, "n_points must be positive");
-    if (!(save_at > 0.0f))
-        return fail(TGK_ERR_INVALID, "save_at must be positive");
+    if (!(save_at > 0.0f) || !std::isfinite(save_at))
+        return fail(TGK_ERR_INVALID, "save_at must be finite and positive");
+    if (!(algorithm->step_size > 0.0f) || !std::isfinite(algorithm->step_size))
+        return fail(TGK_ERR_INVALID, "step_size must be finite and positive");
     if (domain->kind == tgk::DOMAIN_MESH && domain->n_nodes == 0)
         return fail(TGK_ERR_INVALID, "mesh domain without BVH");
     return TGK_OK;
@@ -545,6 +549,9 @@ tgk::TrackLaunch make_launch(const tgk_domain *domain, const tgk_field *field, c
     p.batch_size = batch_size;
     p.n_points = n_points;
     p.save_at = save_at;
+    const double per_save = std::ceil((double)save_at / (double)algorithm->step_size);
+    const double cap = 2.0 * (double)n_points * per_save + 16.0;
+    p.max_steps = cap >= 4294967295.0 ? 4294967295u : (uint32_t)cap;
     p.lengths = d_lengths;
     p.steps = d_steps;
     p.hits = d_hits;
@@ -744,7 +751,31 @@ int run_track(tgk_runtime *rt, const tgk_domain *domain, tgk_field *field, const
         TGK_HIP(tgk::launch_track_plain(p, rt->stream), "plain tracking kernel");
         return TGK_OK;
     }
-    return fail(TGK_ERR_UNSUPPORTED, "the fast path is not available in this build");
+    if (field->fast_precision != TGK_PRECISION_FP32)
+        return fail(TGK_ERR_UNSUPPORTED, "fast-path coefficient storage %d is not available", field->fast_precision);
+    const int32_t model = algorithm->model;
+    if (field->fast_cached != model) {
+        const uint32_t out_stride = model == TGK_MODEL_SH ? 48u : 8u;
+        const int64_t n_voxels = (int64_t)field->dims[0] * field->dims[1] * field->dims[2];
+        TGK_HIP(field->fast.allocate((size_t)n_voxels * out_stride * sizeof(float)), "fast field allocation");
+        TGK_HIP(tgk::launch_field_transform(field->data.as<const float>(), n_voxels, field->stride, model,
+                                            field->fast.ptr, out_stride, rt->stream),
+                "field transform");
+        field->fast_cached = model;
+        field->fast_stride = out_stride;
+    }
+    TGK_HIP(rt->queue.allocate(256), "queue allocation");
+    tgk::FastLaunch f;
+    f.track = p;
+    f.queue = rt->queue.as<uint32_t>();
+    f.precision = field->fast_precision;
+    f.coeffs = field->fast.ptr;
+    f.coeff_stride = field->fast_stride;
+    int grid = 0;
+    if (const char *env = getenv("TGK_FAST_GRID"))
+        grid = atoi(env);
+    TGK_HIP(tgk::launch_track_fast(f, grid, rt->stream), "fast tracking kernel");
+    return TGK_OK;
 }
 
 // Scratch device copies for the one-op drivers.

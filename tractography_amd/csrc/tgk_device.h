@@ -77,6 +77,10 @@ struct TrackLaunch {
     int64_t batch_size;
     uint32_t n_points;           // point capacity per streamline, seed included
     float save_at;
+    // Orientation updates after which a streamline is stopped regardless:
+    // 2 n_points ceil(save_at / step) + 16, which only a `time` accumulator
+    // that stopped advancing (step below one ulp of save_at) can reach.
+    uint32_t max_steps;
     uint32_t *lengths;
     uint32_t *steps;
     uint32_t *hits;
@@ -279,7 +283,7 @@ __device__ inline bool bvh_segment_hit(const MeshView &m, V3 start, V3 end, uint
 // length `len2` cannot reach any boundary triangle.
 __device__ __forceinline__ bool mesh_far(const MeshView &m, V3 start, float len2)
 {
-    if (!(len2 <= m.reach * m.reach))
+    if (!(m.reach > 0.0f) || !(len2 <= m.reach * m.reach))
         return false;
     const float rx = floorf(affine_row(m.near_inv, 0, start) + 0.5f);
     const float ry = floorf(affine_row(m.near_inv, 1, start) + 0.5f);

@@ -1,12 +1,207 @@
-// tgk_fast.hip -- the fast tracking path: persistent lanes that refill from a
-// device seed queue, seed-major dense output, and its compaction.
+// tgk_fast.hip -- the fast tracking path.
+//
+// * Persistent lanes: a grid sized to fill the GPU once; every wavefront
+//   claims seeds from a device queue (one atomic per wavefront refill) and
+//   each lane starts a new streamline as soon as its previous one stops, so
+//   streamlines of very different lengths (5 to 3000 steps) do not leave
+//   lanes idle until the longest of a wavefront ends.
+// * Per-lane cell cache: mask value and the voxel's model coefficients stay
+//   in registers until the point crosses into another cell (about every ten
+//   steps at step 0.1 voxel).
+// * Transcendental-free SH model (tgk_fast_model.h), hardware exp2/log2/cos
+//   where the reference calls exp/log/cos, FP64 arc length.
+// * Seed-major dense output rows, compacted into ragged points afterwards.
+//
+// Compiled with -ffp-contract=fast.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <vector>
+
+#include "tgk_fast_model.h"
 #include "tgk_internal.h"
 
 namespace tgk {
 
 namespace {
+
+using fast::SH_STRIDE;
+using fast::SH_TERMS;
+
+__device__ __forceinline__ void store_row(float *dense, int64_t g, uint32_t n_points, uint32_t n, V3 p)
+{
+    float *dst = dense + (g * (int64_t)n_points + n) * 3;
+    dst[0] = p.x;
+    dst[1] = p.y;
+    dst[2] = p.z;
+}
+
+__device__ __forceinline__ int64_t mask_cell(const MaskView &m, V3 p)
+{
+    const float rx = floorf(affine_row(m.inv, 0, p) + 0.5f);
+    const float ry = floorf(affine_row(m.inv, 1, p) + 0.5f);
+    const float rz = floorf(affine_row(m.inv, 2, p) + 0.5f);
+    if (!(rx >= 0.0f && rx < (float)m.dims[0] && ry >= 0.0f && ry < (float)m.dims[1] && rz >= 0.0f &&
+          rz < (float)m.dims[2]))
+        return -1;
+    return ((int64_t)(uint32_t)rx * m.dims[1] + (uint32_t)ry) * m.dims[2] + (uint32_t)rz;
+}
+
+struct Lane {
+    int64_t g;          // global seed index being tracked (-1: idle)
+    V3 point, o, previous;
+    uint2 rng;
+    double time;
+    uint32_t n, steps;
+    int64_t field_cell; // cached cell ids (-2: none)
+    int64_t mask_cell;
+    bool mask_inside;
+    bool first;
+};
+
+template <int MODEL, int ALGO, int DOMAIN>
+__global__ __launch_bounds__(256) void track_fast_kernel(FastLaunch f)
+{
+    const TrackLaunch &p = f.track;
+    const int lane_id = __lane_id();
+    const float step = p.algo.step_size;
+    const double save = (double)p.save_at;
+    const float *coeffs = static_cast<const float *>(f.coeffs);
+    const uint32_t cstride = f.coeff_stride;
+
+    Lane L;
+    L.g = -1;
+    float q[SH_TERMS];
+    bool drained = false;
+
+    while (true) {
+        // Refill idle lanes from the queue, one atomic per wavefront.
+        const uint64_t need = __ballot(L.g < 0);
+        if (need != 0 && !drained) {
+            const int leader = __ffsll((unsigned long long)need) - 1;
+            const uint32_t count = (uint32_t)__popcll(need);
+            uint32_t base = 0;
+            if (lane_id == leader)
+                base = atomicAdd(f.queue, count);
+            base = __shfl(base, leader);
+            if (base + count >= (uint64_t)p.n_seeds)
+                drained = true;
+            if (L.g < 0) {
+                const uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane_id) - 1ull));
+                const uint64_t g = (uint64_t)base + rank;
+                if (g < (uint64_t)p.n_seeds) {
+                    L.g = (int64_t)g;
+                    const float4 s0 = reinterpret_cast<const float4 *>(p.seeds)[2 * g];
+                    const float4 s1 = reinterpret_cast<const float4 *>(p.seeds)[2 * g + 1];
+                    L.point = v3(s0.x, s0.y, s0.z);
+                    L.o = v3(s1.x, s1.y, s1.z);
+                    L.previous = L.point;
+                    if constexpr (ALGO == ALGO_DIFFUSION)
+                        L.rng = seed_stream(p, p.first_seed + L.g);
+                    L.time = 0.0;
+                    L.n = 1;
+                    L.steps = 0;
+                    L.field_cell = -2;
+                    L.mask_cell = -2;
+                    L.mask_inside = false;
+                    L.first = true;
+                    store_row(p.points, L.g, p.n_points, 0, L.point);
+                }
+            }
+        }
+        if (__ballot(L.g >= 0) == 0)
+            break;
+        if (L.g < 0)
+            continue;
+
+        // One iteration of the reference loop (tracking.py:121-147).
+        bool stop = !(L.n < p.n_points && L.steps < p.max_steps);
+        uint32_t hit = NO_HIT;
+        if (!stop) {
+            if constexpr (DOMAIN == DOMAIN_MASK) {
+                const int64_t cell = mask_cell(p.mask, L.point);
+                if (cell != L.mask_cell) {
+                    L.mask_cell = cell;
+                    L.mask_inside = cell >= 0 && p.mask.mask[cell] != 0;
+                }
+                stop = !L.mask_inside;
+            } else {
+                if (!L.first) {
+                    const float dx = L.point.x - L.previous.x, dy = L.point.y - L.previous.y,
+                                dz = L.point.z - L.previous.z;
+                    if (!mesh_far(p.mesh, L.previous, dx * dx + dy * dy + dz * dz)) {
+                        uint32_t tri;
+                        float t;
+                        if (bvh_segment_hit(p.mesh, L.previous, L.point, tri, t)) {
+                            hit = tri;
+                            stop = true;
+                        }
+                    }
+                }
+            }
+        }
+        if (!stop) {
+            const int64_t cell = field_voxel(p.field, L.point);
+            if (cell < 0) {
+                stop = true;
+            } else if (cell != L.field_cell) {
+                L.field_cell = cell;
+                const float4 *row = reinterpret_cast<const float4 *>(coeffs + cell * (int64_t)cstride);
+                if constexpr (MODEL == MODEL_SH) {
+#pragma unroll
+                    for (int j = 0; j < SH_STRIDE / 4; j++) {
+                        const float4 v = row[j];
+                        if (4 * j < SH_TERMS)
+                            q[4 * j] = v.x;
+                        if (4 * j + 1 < SH_TERMS)
+                            q[4 * j + 1] = v.y;
+                        if (4 * j + 2 < SH_TERMS)
+                            q[4 * j + 2] = v.z;
+                        if (4 * j + 3 < SH_TERMS)
+                            q[4 * j + 3] = v.w;
+                    }
+                } else {
+                    const float4 a = row[0], b = row[1];
+                    q[0] = a.x;
+                    q[1] = a.y;
+                    q[2] = a.z;
+                    q[3] = a.w;
+                    q[4] = b.x;
+                    q[5] = b.y;
+                }
+            }
+        }
+        if (stop) {
+            p.lengths[L.g] = L.n;
+            p.steps[L.g] = L.steps;
+            p.hits[L.g] = hit;
+            L.g = -1;
+            continue;
+        }
+
+        const fast::Direction dir = fast::direction_of(L.o);
+        fast::ModelOut mv;
+        if constexpr (MODEL == MODEL_SH)
+            mv = fast::sh_model(q, dir);
+        else
+            mv = fast::dti_model(q, L.o, dir);
+        L.o = fast::update<ALGO>(p.algo, mv, dir, L.o, L.rng);
+        L.steps++;
+        L.previous = L.point;
+        L.point = v3(L.point.x + step * L.o.x, L.point.y + step * L.o.y, L.point.z + step * L.o.z);
+        L.first = false;
+        L.time += (double)step;
+        if (L.time >= save) {
+            L.time -= save;
+            const float t = (float)L.time;
+            store_row(p.points, L.g, p.n_points, L.n,
+                      v3(L.point.x - t * L.o.x, L.point.y - t * L.o.y, L.point.z - t * L.o.z));
+            L.n++;
+        }
+    }
+}
+
+__global__ void zero_counter(uint32_t *counter) { *counter = 0u; }
 
 // Dense seed-major rows -> ragged points. One wavefront per streamline copies
 // its lengths[g] * 3 floats, coalesced on both sides.
@@ -27,7 +222,147 @@ __global__ __launch_bounds__(256) void compact_seed_major_kernel(const float *__
     }
 }
 
+// out[v, r] = sum_i T[r, i] c[v, i] in FP64 for the 45 x 45 SH -> polynomial
+// map; rows past 45 of the padded stride are zero.
+__constant__ double c_sh_to_poly[SH_TERMS * SH_TERMS];
+
+__global__ __launch_bounds__(256) void sh_to_poly_kernel(const float *__restrict__ field, int64_t n_voxels,
+                                                         uint32_t stride, uint32_t K, float *__restrict__ out,
+                                                         uint32_t out_stride)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n_voxels)
+        return;
+    double c[SH_TERMS];
+#pragma unroll
+    for (int i = 0; i < SH_TERMS; i++)
+        c[i] = i < (int)K ? (double)field[v * stride + i] : 0.0;
+    float *dst = out + v * out_stride;
+    for (int r = 0; r < SH_TERMS; r++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < SH_TERMS; i++)
+            acc += c_sh_to_poly[r * SH_TERMS + i] * c[i];
+        dst[r] = (float)acc;
+    }
+    for (uint32_t r = SH_TERMS; r < out_stride; r++)
+        dst[r] = 0.0f;
+}
+
+__global__ void copy_rows_kernel(const float *__restrict__ field, int64_t n_voxels, uint32_t stride,
+                                 float *__restrict__ out, uint32_t out_stride)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n_voxels)
+        return;
+    for (uint32_t r = 0; r < out_stride; r++)
+        out[v * out_stride + r] = r < stride ? field[v * stride + r] : 0.0f;
+}
+
+// Legendre polynomial P_l as ascending monomial coefficients.
+std::vector<double> legendre(int l)
+{
+    std::vector<double> prev(1, 1.0), cur(2, 0.0);
+    cur[1] = 1.0;
+    if (l == 0)
+        return prev;
+    for (int k = 1; k < l; k++) {
+        std::vector<double> next(k + 2, 0.0);
+        for (int j = 0; j <= k; j++)
+            next[j + 1] += (2.0 * k + 1.0) * cur[j] / (k + 1.0);
+        for (int j = 0; j < (int)prev.size(); j++)
+            next[j] -= k * prev[j] / (k + 1.0);
+        prev = cur;
+        cur = next;
+    }
+    return cur;
+}
+
+double factorial(int n)
+{
+    double f = 1.0;
+    for (int i = 2; i <= n; i++)
+        f *= i;
+    return f;
+}
+
+// T[r, i]: coefficient r of the polynomial layout (tgk_fast_model.h) per unit
+// SH coefficient i (MRtrix order). The real harmonic of order m > 0 is
+// sqrt(2) N_lm P_l^m(z) cos(m phi) (sin for -m) with
+// P_l^m = (-1)^m s^m d^m P_l / dz^m and N_lm = sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!).
+std::vector<double> sh_to_poly_matrix()
+{
+    std::vector<double> T(SH_TERMS * SH_TERMS, 0.0);
+    for (int l = 0; l <= 8; l += 2) {
+        const std::vector<double> P = legendre(l);
+        for (int m = 0; m <= l; m++) {
+            const double N = std::sqrt((2.0 * l + 1.0) / (4.0 * M_PI) * factorial(l - m) / factorial(l + m));
+            const double F = N * (m == 0 ? 1.0 : std::sqrt(2.0)) * ((m & 1) ? -1.0 : 1.0);
+            const int parity = m & 1;
+            for (int side = 0; side < (m == 0 ? 1 : 2); side++) {
+                const int column = l * (l + 1) / 2 + (side == 0 ? m : -m);
+                const int base = side == 0 ? fast::a_offset(m) : fast::b_offset(m);
+                for (int j = 0; j < fast::poly_terms(m); j++) {
+                    const int power = parity + 2 * j;
+                    if (power + m > l)
+                        continue;
+                    const double coefficient = P[power + m] * factorial(power + m) / factorial(power);
+                    T[(base + j) * SH_TERMS + column] += F * coefficient;
+                }
+            }
+        }
+    }
+    return T;
+}
+
+template <int MODEL, int ALGO, int DOMAIN>
+hipError_t launch_fast(const FastLaunch &f, int grid_blocks, hipStream_t stream)
+{
+    if (grid_blocks <= 0) {
+        int per_cu = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(track_fast_kernel<MODEL, ALGO, DOMAIN>), 256, 0);
+        if (e != hipSuccess)
+            return e;
+        int device = 0;
+        hipGetDevice(&device);
+        hipDeviceProp_t prop;
+        e = hipGetDeviceProperties(&prop, device);
+        if (e != hipSuccess)
+            return e;
+        grid_blocks = prop.multiProcessorCount * (per_cu > 0 ? per_cu : 1);
+    }
+    zero_counter<<<1, 1, 0, stream>>>(f.queue);
+    track_fast_kernel<MODEL, ALGO, DOMAIN><<<grid_blocks, 256, 0, stream>>>(f);
+    return hipGetLastError();
+}
+
+template <int MODEL, int ALGO>
+hipError_t launch_fast_domain(const FastLaunch &f, int grid, hipStream_t stream)
+{
+    if (f.track.domain_kind == DOMAIN_MESH)
+        return launch_fast<MODEL, ALGO, DOMAIN_MESH>(f, grid, stream);
+    return launch_fast<MODEL, ALGO, DOMAIN_MASK>(f, grid, stream);
+}
+
+template <int MODEL>
+hipError_t launch_fast_algo(const FastLaunch &f, int grid, hipStream_t stream)
+{
+    if (f.track.algo.algorithm == ALGO_DIFFUSION)
+        return launch_fast_domain<MODEL, ALGO_DIFFUSION>(f, grid, stream);
+    return launch_fast_domain<MODEL, ALGO_TRANSPORT>(f, grid, stream);
+}
+
 }  // namespace
+
+hipError_t launch_track_fast(const FastLaunch &f, int grid_blocks, hipStream_t stream)
+{
+    if (f.track.n_seeds == 0)
+        return hipSuccess;
+    if (f.track.algo.model == MODEL_DTI)
+        return launch_fast_algo<MODEL_DTI>(f, grid_blocks, stream);
+    return launch_fast_algo<MODEL_SH>(f, grid_blocks, stream);
+}
 
 hipError_t launch_compact_seed_major(const float *dense, int64_t n_seeds, uint32_t n_points,
                                      const uint32_t *lengths, const int64_t *offsets, float *out,
@@ -38,6 +373,36 @@ hipError_t launch_compact_seed_major(const float *dense, int64_t n_seeds, uint32
     const int64_t blocks = (n_seeds + 3) / 4;
     compact_seed_major_kernel<<<(unsigned)(blocks < 65536 ? blocks : 65536), 256, 0, stream>>>(
         dense, n_seeds, n_points, lengths, offsets, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_field_transform(const float *field, int64_t n_voxels, uint32_t stride, int32_t model, void *out,
+                                  uint32_t out_stride, hipStream_t stream)
+{
+    if (n_voxels == 0)
+        return hipSuccess;
+    const unsigned blocks = (unsigned)((n_voxels + 255) / 256);
+    if (model == MODEL_SH) {
+        // The map lives in constant memory of each device used.
+        static bool uploaded[64] = {};
+        int device = 0;
+        hipError_t e = hipGetDevice(&device);
+        if (e != hipSuccess)
+            return e;
+        if (device < 0 || device >= 64)
+            return hipErrorInvalidDevice;
+        if (!uploaded[device]) {
+            const std::vector<double> T = sh_to_poly_matrix();
+            e = hipMemcpyToSymbol(HIP_SYMBOL(c_sh_to_poly), T.data(), T.size() * sizeof(double));
+            if (e != hipSuccess)
+                return e;
+            uploaded[device] = true;
+        }
+        sh_to_poly_kernel<<<blocks, 256, 0, stream>>>(field, n_voxels, stride, stride < SH_TERMS ? stride : SH_TERMS,
+                                                      static_cast<float *>(out), out_stride);
+    } else {
+        copy_rows_kernel<<<blocks, 256, 0, stream>>>(field, n_voxels, stride, static_cast<float *>(out), out_stride);
+    }
     return hipGetLastError();
 }
 

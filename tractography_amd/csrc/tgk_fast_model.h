@@ -1,0 +1,244 @@
+// tgk_fast_model.h -- model evaluation and orientation update of the fast
+// path (device side). Include only from tgk_fast.hip (contraction on).
+//
+// Real symmetric SH to l <= 8 in polynomial form. For a unit direction with
+// z = cos(theta), s = sin(theta) and u = s e^{i phi} = x + i y:
+//
+//   f     = A_0(z) + sum_{m>=1} A_m(z) Re u^m + B_m(z) Im u^m
+//   df/dt = -s A_0'(z) + sum_{m>=1} m z (A_m Re v_m + B_m Im v_m)
+//                                  - s (A_m' Re u^m + B_m' Im u^m)
+//   (df/dphi)/sin(theta) = sum_{m>=1} m (B_m Re v_m - A_m Im v_m)
+//
+// with v_m = u^{m-1} e^{i phi} and A_m, B_m polynomials in z whose monomial
+// coefficients are a fixed linear map of the voxel's 45 SH coefficients
+// (A_m collects the order +m harmonics, B_m the order -m ones; each includes
+// the normalisation, the sqrt(2) of the real basis and the Condon-Shortley
+// sign). The map is applied once per field upload (launch_field_transform),
+// so a step costs ~90 polynomial FMAs and two complex power sequences
+// instead of the trigonometric basis.
+#pragma once
+
+#include "tgk_device.h"
+
+namespace tgk {
+namespace fast {
+
+// Layout of the 45 transformed coefficients per voxel: for m = 0..8, the
+// coefficients of A_m (then B_m for m > 0) by ascending power of z, powers
+// m mod 2, m mod 2 + 2, ..., 8 - m.
+constexpr int SH_TERMS = 45;
+constexpr int SH_STRIDE = 48;
+
+__host__ __device__ constexpr int poly_terms(int m) { return (8 - m - (m & 1)) / 2 + 1; }
+__host__ __device__ constexpr int a_offset(int m)
+{
+    int off = 0;
+    for (int k = 0; k < m; k++)
+        off += poly_terms(k) * (k == 0 ? 1 : 2);
+    return off;
+}
+__host__ __device__ constexpr int b_offset(int m) { return a_offset(m) + poly_terms(m); }
+
+// q(w) and q'(w) for q = sum_j c[j] w^j (Horner).
+template <int N>
+__device__ __forceinline__ void horner(const float *c, float w, float &v, float &d)
+{
+    v = c[N - 1];
+    d = 0.0f;
+#pragma unroll
+    for (int j = N - 2; j >= 0; j--) {
+        d = d * w + v;
+        v = v * w + c[j];
+    }
+}
+
+// A(z) and A'(z) for A(z) = z^p q(z^2), p = m mod 2.
+template <int M>
+__device__ __forceinline__ void poly_m(const float *c, float z, float w, float &a, float &ad)
+{
+    float v, d;
+    horner<poly_terms(M)>(c, w, v, d);
+    if constexpr ((M & 1) == 0) {
+        a = v;
+        ad = 2.0f * z * d;
+    } else {
+        a = z * v;
+        ad = v + 2.0f * w * d;
+    }
+}
+
+struct Direction {
+    float z, s;   // cos and sin of the colatitude
+    float c1, s1; // cos and sin of the azimuth
+};
+
+// Angles of a (not necessarily unit) orientation, as the reference's
+// cart2sph: the zero vector maps to colatitude 0, azimuth 0.
+__device__ __forceinline__ Direction direction_of(V3 o)
+{
+    Direction d;
+    const float rho2 = o.x * o.x + o.y * o.y;
+    const float r2 = rho2 + o.z * o.z;
+    if (r2 > 0.0f) {
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        d.z = fminf(fmaxf(o.z * inv_r, -1.0f), 1.0f);
+        d.s = rho2 * __builtin_amdgcn_rsqf(r2 * rho2 > 0.0f ? r2 * rho2 : 1.0f);
+    } else {
+        d.z = 1.0f;
+        d.s = 0.0f;
+    }
+    if (rho2 > 0.0f) {
+        const float inv_rho = __builtin_amdgcn_rsqf(rho2);
+        d.c1 = o.x * inv_rho;
+        d.s1 = o.y * inv_rho;
+    } else {
+        d.c1 = 1.0f;
+        d.s1 = 0.0f;
+    }
+    return d;
+}
+
+struct Raw {
+    float value, dtheta, dphi;
+};
+
+__device__ __forceinline__ Raw sh8_evaluate(const float q[SH_TERMS], const Direction &dir)
+{
+    const float z = dir.z, s = dir.s, w = z * z;
+    Raw r;
+    float a, ad, b, bd;
+    poly_m<0>(q + a_offset(0), z, w, a, ad);
+    r.value = a;
+    r.dtheta = -s * ad;
+    r.dphi = 0.0f;
+    const float u1r = s * dir.c1, u1i = s * dir.s1;
+    float ur = 1.0f, ui = 0.0f;         // u^(m-1) -> u^m
+    float vr = dir.c1, vi = dir.s1;     // v_m = u^(m-1) e^{i phi}
+#define TGK_FAST_M(M)                                                                \
+    {                                                                                \
+        if (M > 1) {                                                                 \
+            const float nvr = vr * u1r - vi * u1i;                                  \
+            vi = vr * u1i + vi * u1r;                                                \
+            vr = nvr;                                                                \
+        }                                                                            \
+        const float nur = ur * u1r - ui * u1i;                                      \
+        ui = ur * u1i + ui * u1r;                                                    \
+        ur = nur;                                                                    \
+        poly_m<M>(q + a_offset(M), z, w, a, ad);                                     \
+        poly_m<M>(q + b_offset(M), z, w, b, bd);                                     \
+        r.value += a * ur + b * ui;                                                  \
+        r.dtheta += (float)M * z * (a * vr + b * vi) - s * (ad * ur + bd * ui);     \
+        r.dphi += (float)M * (b * vr - a * vi);                                     \
+    }
+    TGK_FAST_M(1)
+    TGK_FAST_M(2)
+    TGK_FAST_M(3)
+    TGK_FAST_M(4)
+    TGK_FAST_M(5)
+    TGK_FAST_M(6)
+    TGK_FAST_M(7)
+    TGK_FAST_M(8)
+#undef TGK_FAST_M
+    return r;
+}
+
+// Softplus with k = 100 and its derivative (core.cl:111-120) from one
+// hardware exp2: e = exp(100 x), value = log(1 + e) / 100, d = e / (1 + e).
+__device__ __forceinline__ void softplus(float x, float &value, float &d)
+{
+    if (x * 100.0f >= 30.0f) {
+        value = x;
+        d = 1.0f;
+        return;
+    }
+    const float e = __builtin_amdgcn_exp2f(x * 144.26950408889634f);
+    const float one_e = 1.0f + e;
+    value = __builtin_amdgcn_logf(one_e) * 0.006931471805599453f;
+    d = e * __builtin_amdgcn_rcpf(one_e);
+}
+
+struct ModelOut {
+    float value, dtheta, dphi;
+};
+
+__device__ __forceinline__ ModelOut sh_model(const float q[SH_TERMS], const Direction &dir)
+{
+    const Raw raw = sh8_evaluate(q, dir);
+    ModelOut m;
+    float d;
+    softplus(raw.value, m.value, d);
+    m.dtheta = raw.dtheta * d;
+    m.dphi = raw.dphi * d;
+    return m;
+}
+
+// local.py:212-230: t = D o, f = o.t, df/dtheta = 2 t.e_theta,
+// (df/dphi)/sin(theta) = 2 t.e_phi (o as given, unit frame vectors).
+__device__ __forceinline__ ModelOut dti_model(const float m[8], V3 o, const Direction &dir)
+{
+    const float t0 = m[0] * o.x + m[3] * o.y + m[4] * o.z;
+    const float t1 = m[3] * o.x + m[1] * o.y + m[5] * o.z;
+    const float t2 = m[4] * o.x + m[5] * o.y + m[2] * o.z;
+    ModelOut r;
+    r.value = o.x * t0 + o.y * t1 + o.z * t2;
+    r.dtheta = 2.0f * (dir.z * (t0 * dir.c1 + t1 * dir.s1) - t2 * dir.s);
+    r.dphi = 2.0f * (t1 * dir.c1 - t0 * dir.s1);
+    return r;
+}
+
+// Box-Muller normal from the MWC64X stream (core.cl:240-245) with the
+// hardware log2 and cos(2 pi x) (v_cos_f32 takes revolutions).
+__device__ __forceinline__ float randn(uint2 &rng)
+{
+    float u = randu(rng);
+    while (u <= 0.0f)
+        u = randu(rng);
+    const float radius = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u));
+    return radius * __builtin_amdgcn_cosf(randu(rng));
+}
+
+// exps2(p, x, 1) (core.cl:101-109) with cos(n) and sin(n)/n as Taylor
+// polynomials in w = |x|^2 for |x| < 0.5 (truncation below 2e-8).
+__device__ __forceinline__ V3 exps2(V3 p, V3 x)
+{
+    const float w = x.x * x.x + x.y * x.y + x.z * x.z;
+    if (!(w > 0.0f))
+        return p;
+    float c, sn;
+    if (w < 0.25f) {
+        c = 1.0f + w * (-0.5f + w * (1.0f / 24.0f + w * (-1.0f / 720.0f + w * (1.0f / 40320.0f))));
+        sn = 1.0f + w * (-1.0f / 6.0f + w * (1.0f / 120.0f + w * (-1.0f / 5040.0f + w * (1.0f / 362880.0f))));
+    } else {
+        const float n = __builtin_sqrtf(w);
+        float s;
+        __sincosf(n, &s, &c);
+        sn = s / n;
+    }
+    return v3(c * p.x + sn * x.x, c * p.y + sn * x.y, c * p.z + sn * x.z);
+}
+
+template <int ALGO>
+__device__ __forceinline__ V3 update(const AlgoParams &a, const ModelOut &mv, const Direction &dir, V3 o, uint2 &rng)
+{
+    const float etx = dir.z * dir.c1, ety = dir.z * dir.s1, etz = -dir.s;
+    const float epx = -dir.s1, epy = dir.c1;
+    const float inv = __builtin_amdgcn_rcpf(fmaxf(mv.value, 1e-6f));
+    const float scale = a.gamma * a.step_size * inv;
+    float tx = scale * (mv.dtheta * etx + mv.dphi * epx);
+    float ty = scale * (mv.dtheta * ety + mv.dphi * epy);
+    float tz = scale * (mv.dtheta * etz);
+    if constexpr (ALGO == ALGO_DIFFUSION) {
+        if (a.noise_variance > 0.0f) {
+            const float r1 = randn(rng);
+            const float r2 = randn(rng);
+            const float sn = __builtin_sqrtf(a.noise_variance * a.gamma * a.step_size);
+            tx += sn * (r1 * etx + r2 * epx);
+            ty += sn * (r1 * ety + r2 * epy);
+            tz += sn * (r1 * etz);
+        }
+    }
+    return exps2(o, v3(tx, ty, tz));
+}
+
+}  // namespace fast
+}  // namespace tgk

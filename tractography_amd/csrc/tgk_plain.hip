@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void track_plain_kernel(TrackLaunch p)
     float time = 0.0f;
     V3 previous = point;
     bool first = true;
-    while (n < p.n_points) {
+    while (n < p.n_points && steps < p.max_steps) {
         if constexpr (DOMAIN == DOMAIN_MASK) {
             if (!mask_contains(p.mask, point))
                 break;
