@@ -226,7 +226,7 @@ namespace tgk {
 size_t offsets_workspace_bytes(int64_t n)
 {
     size_t bytes = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int64_t *)nullptr, (int64_t *)nullptr, (int)(n + 1));
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int64_t *)nullptr, (int64_t *)nullptr, (int)(n + 1));
     const size_t widened = (((size_t)(n + 1) * sizeof(int64_t)) + 255) & ~(size_t)255;
     return widened + bytes;
 }
@@ -978,18 +978,41 @@ int tgk_domain_seed(tgk_runtime *rt, const tgk_domain *domain, uint32_t rng_seed
 int tgk_seeds_from_fod(tgk_runtime *rt, int32_t model, int32_t n_parameters, const float *fod, const float *voxels,
                        uint32_t nnz, const float affine[16], uint32_t *states, int64_t n, int32_t method, float *out)
 {
-    (void)rt;
-    (void)model;
-    (void)n_parameters;
-    (void)fod;
-    (void)voxels;
-    (void)nnz;
-    (void)affine;
-    (void)states;
-    (void)n;
-    (void)method;
-    (void)out;
-    return fail(TGK_ERR_UNSUPPORTED, "seeding from FOD is not available in this build");
+    if (!rt || n < 0 || (n > 0 && (!fod || !voxels || !affine || !states || !out)))
+        return fail(TGK_ERR_INVALID, "bad arguments");
+    tgk_algorithm a = {model, TGK_ALGO_TRANSPORT, n_parameters, 1.0f, 1.0f, 0.0f};
+    if (int r = check_algorithm(&a, nullptr))
+        return r;
+    if (method != TGK_SEEDS_VALU)
+        return fail(TGK_ERR_UNSUPPORTED, "seeding method not available in this build");
+    if (n > 0 && nnz == 0)
+        return fail(TGK_ERR_INVALID, "the FOD has no nonzero voxel");
+    if (n == 0)
+        return TGK_OK;
+    if (int r = select_device(rt))
+        return r;
+    Staging st;
+    float *dfod, *dvox, *daff, *dout;
+    uint32_t *dstates;
+    if (int r = st.in(fod, (size_t)nnz * n_parameters, rt->stream, &dfod))
+        return r;
+    if (int r = st.in(voxels, (size_t)nnz * 4, rt->stream, &dvox))
+        return r;
+    if (int r = st.in(affine, 16, rt->stream, &daff))
+        return r;
+    if (int r = st.in(states, (size_t)n * 2, rt->stream, &dstates))
+        return r;
+    if (int r = st.out((size_t)n * 8, &dout))
+        return r;
+    const size_t ws = tgk::seeds_from_fod_workspace_bytes(model, n_parameters, n, method);
+    if (int r = ensure_workspace(rt, ws))
+        return r;
+    TGK_HIP(tgk::launch_seeds_from_fod(model, n_parameters, dfod, dvox, nnz, daff, dstates, n, method, dout,
+                                       rt->workspace.ptr, ws, rt->stream),
+            "FOD seeding kernel");
+    TGK_HIP(hipMemcpyAsync(states, dstates, (size_t)n * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, rt->stream),
+            "state copy");
+    return fetch(out, dout, (size_t)n * 8, rt->stream);
 }
 
 }  // extern "C"

@@ -59,13 +59,99 @@ struct Lane {
     bool first;
 };
 
+// One iteration of the reference loop (tracking.py:121-147) for a lane with
+// a streamline in flight; on termination the lane's results are written and
+// the lane becomes idle (L.g = -1).
+template <int MODEL, int ALGO, int DOMAIN>
+__device__ __forceinline__ void advance(const TrackLaunch &p, const float *__restrict__ coeffs, uint32_t cstride,
+                                        Lane &L, float q[SH_TERMS])
+{
+    bool stop = !(L.n < p.n_points && L.steps < p.max_steps);
+    uint32_t hit = NO_HIT;
+    if (!stop) {
+        if constexpr (DOMAIN == DOMAIN_MASK) {
+            const int64_t cell = mask_cell(p.mask, L.point);
+            if (cell != L.mask_cell) {
+                L.mask_cell = cell;
+                L.mask_inside = cell >= 0 && p.mask.mask[cell] != 0;
+            }
+            stop = !L.mask_inside;
+        } else if (!L.first) {
+            const float dx = L.point.x - L.previous.x, dy = L.point.y - L.previous.y, dz = L.point.z - L.previous.z;
+            if (!mesh_far(p.mesh, L.previous, dx * dx + dy * dy + dz * dz)) {
+                uint32_t tri;
+                float t;
+                if (bvh_segment_hit(p.mesh, L.previous, L.point, tri, t)) {
+                    hit = tri;
+                    stop = true;
+                }
+            }
+        }
+    }
+    int64_t cell = -1;
+    if (!stop) {
+        cell = field_voxel(p.field, L.point);
+        stop = cell < 0;
+    }
+    if (stop) {
+        p.lengths[L.g] = L.n;
+        p.steps[L.g] = L.steps;
+        p.hits[L.g] = hit;
+        L.g = -1;
+        return;
+    }
+    if (cell != L.field_cell) {
+        L.field_cell = cell;
+        const float4 *row = reinterpret_cast<const float4 *>(coeffs + cell * (int64_t)cstride);
+        if constexpr (MODEL == MODEL_SH) {
+#pragma unroll
+            for (int j = 0; j < SH_STRIDE / 4; j++) {
+                const float4 v = row[j];
+                if (4 * j < SH_TERMS)
+                    q[4 * j] = v.x;
+                if (4 * j + 1 < SH_TERMS)
+                    q[4 * j + 1] = v.y;
+                if (4 * j + 2 < SH_TERMS)
+                    q[4 * j + 2] = v.z;
+                if (4 * j + 3 < SH_TERMS)
+                    q[4 * j + 3] = v.w;
+            }
+        } else {
+            const float4 a = row[0], b = row[1];
+            q[0] = a.x;
+            q[1] = a.y;
+            q[2] = a.z;
+            q[3] = a.w;
+            q[4] = b.x;
+            q[5] = b.y;
+        }
+    }
+    const fast::Direction dir = fast::direction_of(L.o);
+    fast::ModelOut mv;
+    if constexpr (MODEL == MODEL_SH)
+        mv = fast::sh_model(q, dir);
+    else
+        mv = fast::dti_model(q, L.o, dir);
+    L.o = fast::update<ALGO>(p.algo, mv, dir, L.o, L.rng);
+    L.steps++;
+    L.previous = L.point;
+    const float step = p.algo.step_size;
+    L.point = v3(L.point.x + step * L.o.x, L.point.y + step * L.o.y, L.point.z + step * L.o.z);
+    L.first = false;
+    L.time += (double)step;
+    if (L.time >= (double)p.save_at) {
+        L.time -= (double)p.save_at;
+        const float t = (float)L.time;
+        store_row(p.points, L.g, p.n_points, L.n, v3(L.point.x - t * L.o.x, L.point.y - t * L.o.y, L.point.z - t * L.o.z));
+        L.n++;
+    }
+}
+
 template <int MODEL, int ALGO, int DOMAIN>
 __global__ __launch_bounds__(256) void track_fast_kernel(FastLaunch f)
 {
     const TrackLaunch &p = f.track;
     const int lane_id = __lane_id();
-    const float step = p.algo.step_size;
-    const double save = (double)p.save_at;
     const float *coeffs = static_cast<const float *>(f.coeffs);
     const uint32_t cstride = f.coeff_stride;
 
@@ -111,93 +197,8 @@ __global__ __launch_bounds__(256) void track_fast_kernel(FastLaunch f)
         }
         if (__ballot(L.g >= 0) == 0)
             break;
-        if (L.g < 0)
-            continue;
-
-        // One iteration of the reference loop (tracking.py:121-147).
-        bool stop = !(L.n < p.n_points && L.steps < p.max_steps);
-        uint32_t hit = NO_HIT;
-        if (!stop) {
-            if constexpr (DOMAIN == DOMAIN_MASK) {
-                const int64_t cell = mask_cell(p.mask, L.point);
-                if (cell != L.mask_cell) {
-                    L.mask_cell = cell;
-                    L.mask_inside = cell >= 0 && p.mask.mask[cell] != 0;
-                }
-                stop = !L.mask_inside;
-            } else {
-                if (!L.first) {
-                    const float dx = L.point.x - L.previous.x, dy = L.point.y - L.previous.y,
-                                dz = L.point.z - L.previous.z;
-                    if (!mesh_far(p.mesh, L.previous, dx * dx + dy * dy + dz * dz)) {
-                        uint32_t tri;
-                        float t;
-                        if (bvh_segment_hit(p.mesh, L.previous, L.point, tri, t)) {
-                            hit = tri;
-                            stop = true;
-                        }
-                    }
-                }
-            }
-        }
-        if (!stop) {
-            const int64_t cell = field_voxel(p.field, L.point);
-            if (cell < 0) {
-                stop = true;
-            } else if (cell != L.field_cell) {
-                L.field_cell = cell;
-                const float4 *row = reinterpret_cast<const float4 *>(coeffs + cell * (int64_t)cstride);
-                if constexpr (MODEL == MODEL_SH) {
-#pragma unroll
-                    for (int j = 0; j < SH_STRIDE / 4; j++) {
-                        const float4 v = row[j];
-                        if (4 * j < SH_TERMS)
-                            q[4 * j] = v.x;
-                        if (4 * j + 1 < SH_TERMS)
-                            q[4 * j + 1] = v.y;
-                        if (4 * j + 2 < SH_TERMS)
-                            q[4 * j + 2] = v.z;
-                        if (4 * j + 3 < SH_TERMS)
-                            q[4 * j + 3] = v.w;
-                    }
-                } else {
-                    const float4 a = row[0], b = row[1];
-                    q[0] = a.x;
-                    q[1] = a.y;
-                    q[2] = a.z;
-                    q[3] = a.w;
-                    q[4] = b.x;
-                    q[5] = b.y;
-                }
-            }
-        }
-        if (stop) {
-            p.lengths[L.g] = L.n;
-            p.steps[L.g] = L.steps;
-            p.hits[L.g] = hit;
-            L.g = -1;
-            continue;
-        }
-
-        const fast::Direction dir = fast::direction_of(L.o);
-        fast::ModelOut mv;
-        if constexpr (MODEL == MODEL_SH)
-            mv = fast::sh_model(q, dir);
-        else
-            mv = fast::dti_model(q, L.o, dir);
-        L.o = fast::update<ALGO>(p.algo, mv, dir, L.o, L.rng);
-        L.steps++;
-        L.previous = L.point;
-        L.point = v3(L.point.x + step * L.o.x, L.point.y + step * L.o.y, L.point.z + step * L.o.z);
-        L.first = false;
-        L.time += (double)step;
-        if (L.time >= save) {
-            L.time -= save;
-            const float t = (float)L.time;
-            store_row(p.points, L.g, p.n_points, L.n,
-                      v3(L.point.x - t * L.o.x, L.point.y - t * L.o.y, L.point.z - t * L.o.z));
-            L.n++;
-        }
+        if (L.g >= 0)
+            advance<MODEL, ALGO, DOMAIN>(p, coeffs, cstride, L, q);
     }
 }
 

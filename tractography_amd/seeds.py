@@ -99,7 +99,7 @@ def from_fod(
         model = local_models.from_shape(fod.shape)
         if states is None:
             states = np.random.randint(4294967295, size=(n_seeds, 2)).astype(np.uint32)
-        states = np.ascontiguousarray(states, dtype=np.uint32)
+        states = np.array(states, dtype=np.uint32, copy=True, order="C")  # the device advances its copy
         if states.shape != (n_seeds, 2):
             raise ValueError("states must have shape (n_seeds, 2)")
         out = np.zeros((n_seeds, 8), dtype=np.float32)
