@@ -160,7 +160,10 @@ int tgk_domain_seed(tgk_runtime *rt, const tgk_domain *domain, uint32_t rng_seed
 
 /* Replaces the OpenCL branch of seeds.from_fod (seeds.py:174-216): sparse
  * FOD (nnz, K), voxel indices (nnz, 4) with w = 1, forward affine 4x4,
- * states (n, 2) u32 (updated in place), out (n, 8). method TGK_SEEDS_*. */
+ * states (n, 2) u32 (updated in place), out (n, 8). method TGK_SEEDS_*:
+ * VALU takes SH K <= 45 or DTI; MFMA takes SH K <= 96 (order 12 = 91
+ * included) or DTI. The voxel draw is clamped to nnz - 1, which changes no
+ * result for nnz <= 2^24. */
 int tgk_seeds_from_fod(tgk_runtime *rt, int32_t model, int32_t n_parameters, const float *fod, const float *voxels,
                        uint32_t nnz, const float affine[16], uint32_t *states, int64_t n, int32_t method, float *out);
 

@@ -980,11 +980,18 @@ int tgk_seeds_from_fod(tgk_runtime *rt, int32_t model, int32_t n_parameters, con
 {
     if (!rt || n < 0 || (n > 0 && (!fod || !voxels || !affine || !states || !out)))
         return fail(TGK_ERR_INVALID, "bad arguments");
-    tgk_algorithm a = {model, TGK_ALGO_TRANSPORT, n_parameters, 1.0f, 1.0f, 0.0f};
-    if (int r = check_algorithm(&a, nullptr))
-        return r;
-    if (method != TGK_SEEDS_VALU)
-        return fail(TGK_ERR_UNSUPPORTED, "seeding method not available in this build");
+    if (method != TGK_SEEDS_VALU && method != TGK_SEEDS_MFMA)
+        return fail(TGK_ERR_INVALID, "unknown seeding method %d", method);
+    if (method == TGK_SEEDS_MFMA && model == TGK_MODEL_SH && n_parameters > 45) {
+        // The MFMA method takes SH to order 12 (91 coefficients) and beyond,
+        // up to 96 coefficients.
+        if (n_parameters > 96)
+            return fail(TGK_ERR_UNSUPPORTED, "SH with %d coefficients: 1..96 supported", n_parameters);
+    } else {
+        tgk_algorithm a = {model, TGK_ALGO_TRANSPORT, n_parameters, 1.0f, 1.0f, 0.0f};
+        if (int r = check_algorithm(&a, nullptr))
+            return r;
+    }
     if (n > 0 && nnz == 0)
         return fail(TGK_ERR_INVALID, "the FOD has no nonzero voxel");
     if (n == 0)

@@ -20,6 +20,7 @@
 
 #include "tgk_fast_model.h"
 #include "tgk_internal.h"
+#include "tgk_sh_basis.h"
 
 namespace tgk {
 
@@ -260,33 +261,6 @@ __global__ void copy_rows_kernel(const float *__restrict__ field, int64_t n_voxe
         out[v * out_stride + r] = r < stride ? field[v * stride + r] : 0.0f;
 }
 
-// Legendre polynomial P_l as ascending monomial coefficients.
-std::vector<double> legendre(int l)
-{
-    std::vector<double> prev(1, 1.0), cur(2, 0.0);
-    cur[1] = 1.0;
-    if (l == 0)
-        return prev;
-    for (int k = 1; k < l; k++) {
-        std::vector<double> next(k + 2, 0.0);
-        for (int j = 0; j <= k; j++)
-            next[j + 1] += (2.0 * k + 1.0) * cur[j] / (k + 1.0);
-        for (int j = 0; j < (int)prev.size(); j++)
-            next[j] -= k * prev[j] / (k + 1.0);
-        prev = cur;
-        cur = next;
-    }
-    return cur;
-}
-
-double factorial(int n)
-{
-    double f = 1.0;
-    for (int i = 2; i <= n; i++)
-        f *= i;
-    return f;
-}
-
 // T[r, i]: coefficient r of the polynomial layout (tgk_fast_model.h) per unit
 // SH coefficient i (MRtrix order). The real harmonic of order m > 0 is
 // sqrt(2) N_lm P_l^m(z) cos(m phi) (sin for -m) with
@@ -295,9 +269,9 @@ std::vector<double> sh_to_poly_matrix()
 {
     std::vector<double> T(SH_TERMS * SH_TERMS, 0.0);
     for (int l = 0; l <= 8; l += 2) {
-        const std::vector<double> P = legendre(l);
+        const std::vector<double> P = shb::legendre(l);
         for (int m = 0; m <= l; m++) {
-            const double N = std::sqrt((2.0 * l + 1.0) / (4.0 * M_PI) * factorial(l - m) / factorial(l + m));
+            const double N = std::sqrt((2.0 * l + 1.0) / (4.0 * M_PI) * shb::factorial(l - m) / shb::factorial(l + m));
             const double F = N * (m == 0 ? 1.0 : std::sqrt(2.0)) * ((m & 1) ? -1.0 : 1.0);
             const int parity = m & 1;
             for (int side = 0; side < (m == 0 ? 1 : 2); side++) {
@@ -307,7 +281,7 @@ std::vector<double> sh_to_poly_matrix()
                     const int power = parity + 2 * j;
                     if (power + m > l)
                         continue;
-                    const double coefficient = P[power + m] * factorial(power + m) / factorial(power);
+                    const double coefficient = P[power + m] * shb::factorial(power + m) / shb::factorial(power);
                     T[(base + j) * SH_TERMS + column] += F * coefficient;
                 }
             }

@@ -46,7 +46,8 @@ hipError_t launch_math(int fn, const float *x, const float *y, int64_t n, float 
 hipError_t launch_domain_seed(const float4 *vertices, const uint4 *triangles, const float *cumulative,
                               uint32_t n_triangles, uint32_t rng_seed, int64_t n, float *out, hipStream_t stream);
 
-// Seeding from FOD (tgk_seeds.hip).
+// Seeding from FOD (tgk_seeds.hip). VALU: K <= 45, bit-exact; MFMA: K <= 96.
+enum SeedsMethod : int32_t { SEEDS_VALU = 0, SEEDS_MFMA = 1 };
 hipError_t launch_seeds_from_fod(int model, int K, const float *fod, const float *voxels, uint32_t nnz,
                                  const float *affine, uint32_t *states, int64_t n, int32_t method, float *out,
                                  void *workspace, size_t workspace_bytes, hipStream_t stream);
