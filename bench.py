@@ -6,14 +6,16 @@ SH order-8 FOD with 1M seeds, FAST path (persistent lanes, transcendental-free
 model evaluation, FP64 arc length) with the PLAIN FP32 path (the reference's
 integrator, bit-exact with the host restatement) timed beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--seeds N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sh45] [--seeds N]
 
 Prints one JSON line. Inputs (field, mask, seeds) are resident in HBM before
 timing; each timed step is the tracking kernel plus the on-device compaction
 of all saved points into one ragged array.
 
-The module also holds the synthetic workload generators (configs 0-4 of the
-benchmark definition) used by the tests.
+Other workloads (--workload): sphere32 (32^3 SH45 Transport, 10k seeds), dti
+(145x174x145 tensors, Transport, 1M seeds), shell (the SH45 field bounded by
+two displaced ellipsoid meshes of ~200k triangles each, Diffusion, 2M seeds).
+The module also holds the synthetic workload generators used by the tests.
 """
 
 from __future__ import annotations
@@ -188,7 +190,7 @@ def seeds_in_mask(rng, mask: np.ndarray, n_seeds: int) -> np.ndarray:
     return seeds
 
 
-def config0(seed: int = 0, n_seeds: int = 10_000):
+def sphere32_transport(seed: int = 0, n_seeds: int = 10_000):
     """32^3 SH order-8 FOD from 1-3 random fibres per voxel, sphere mask of
     radius 14, seeds in the mask; Transport, step 0.1, save 0.25, max 300."""
     rng = np.random.default_rng(seed)
@@ -203,7 +205,7 @@ def config0(seed: int = 0, n_seeds: int = 10_000):
             "step_size": 0.1, "gamma": 1.0, "noise_variance": 0.0, "save_at": 0.25, "max_length": 300.0}
 
 
-def config1(seed: int = 0, n_seeds: int = 1_000_000, shape=SHAPE_145):
+def brain145_diffusion(seed: int = 0, n_seeds: int = 1_000_000, shape=SHAPE_145):
     """145x174x145 SH order-8 FOD whose 1-3 fibres per voxel follow smooth
     random direction fields (sigma 4), ellipsoid mask 60/75/60; Diffusion
     gamma 1, noise variance 0.1, step 0.1, save 0.25, max 300."""
@@ -219,6 +221,155 @@ def config1(seed: int = 0, n_seeds: int = 1_000_000, shape=SHAPE_145):
             "step_size": 0.1, "gamma": 1.0, "noise_variance": 0.1, "save_at": 0.25, "max_length": 300.0}
 
 
+def _semi_axes(shape, semi) -> np.ndarray:
+    return np.array(semi, dtype=np.float64) * np.array(shape) / np.array(SHAPE_145)
+
+
+def brain145_dti_transport(seed: int = 0, n_seeds: int = 1_000_000, shape=SHAPE_145):
+    """145x174x145 diffusion tensors D = R diag(1.7, 0.3, 0.3)e-3 R^T whose
+    principal axis is the first smooth direction field of
+    brain145_diffusion(seed) (sigma 4), 0.8e-3 I outside the ellipsoid
+    60/75/60; Transport gamma 1, step 0.1, save 0.25, max 300."""
+    rng = np.random.default_rng(seed)
+    e = smooth_direction_fields(rng, shape, n_fields=1)[:, 0]
+    mask = ellipsoid_mask(shape, _semi_axes(shape, (60, 75, 60)))
+    inside = mask.reshape(-1)
+    low, delta = 0.3e-3, 1.7e-3 - 0.3e-3
+    tensors = np.zeros((len(e), 6), dtype=np.float64)
+    for c in range(3):
+        tensors[:, c] = low + delta * e[:, c] * e[:, c]
+    tensors[:, 3] = delta * e[:, 0] * e[:, 1]
+    tensors[:, 4] = delta * e[:, 0] * e[:, 2]
+    tensors[:, 5] = delta * e[:, 1] * e[:, 2]
+    tensors[~inside] = [0.8e-3, 0.8e-3, 0.8e-3, 0.0, 0.0, 0.0]
+    seeds = seeds_in_mask(rng, mask, n_seeds)
+    return {"fod": tensors.astype(np.float32).reshape(*shape, 6), "mask": mask.astype(np.uint8), "seeds": seeds,
+            "model": "dti", "algorithm": "transport", "step_size": 0.1, "gamma": 1.0, "noise_variance": 0.0,
+            "save_at": 0.25, "max_length": 300.0}
+
+
+def _fibonacci(n: int) -> np.ndarray:
+    i = np.arange(n, dtype=np.float64)
+    z = 1.0 - 2.0 * i / (n - 1)
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    t = i * (2.0 * math.pi / ((1.0 + math.sqrt(5.0)) / 2.0))
+    return np.stack([r * np.cos(t), r * np.sin(t), z], axis=1)
+
+
+def radial_noise(rng, amplitude: float, lmax: int = 4):
+    """A smooth function on the sphere: random real SH coefficients of
+    degree <= lmax, scaled to max |n| = amplitude. Returns n(u) for unit
+    vectors u (m, 3)."""
+    K = (lmax + 1) * (lmax + 2) // 2
+    coefficients = rng.standard_normal(K)
+    peak = np.abs(real_sh_basis(_fibonacci(20000), lmax) @ coefficients).max()
+    scale = amplitude / peak
+
+    def noise(u):
+        return scale * (real_sh_basis(u, lmax) @ coefficients)
+
+    return noise
+
+
+def ellipsoid_radius(u: np.ndarray, semi_axes) -> np.ndarray:
+    """Distance from the centre to the ellipsoid along unit vectors u."""
+    return 1.0 / np.sqrt(np.sum((u / np.asarray(semi_axes, dtype=np.float64)) ** 2, axis=1))
+
+
+def ellipsoid_surface(centre, semi_axes, n_rings: int, n_segments: int, noise=None, inward: bool = False):
+    """Latitude-longitude triangulation of an ellipsoid whose vertices are
+    moved radially by noise(u): poles plus n_rings - 1 rings of n_segments
+    vertices, 2 n_segments (n_rings - 1) triangles, normals outward (or
+    inward). Returns vertices (V, 3) float64 and faces (F, 3) int64."""
+    theta = math.pi * np.arange(1, n_rings) / n_rings
+    phi = 2.0 * math.pi * np.arange(n_segments) / n_segments
+    st, ct = np.sin(theta)[:, None], np.cos(theta)[:, None]
+    ring = np.stack([st * np.cos(phi)[None, :], st * np.sin(phi)[None, :],
+                     np.broadcast_to(ct, (len(theta), n_segments))], axis=-1).reshape(-1, 3)
+    u = np.vstack([[0.0, 0.0, 1.0], ring, [0.0, 0.0, -1.0]])
+    r = ellipsoid_radius(u, semi_axes)
+    if noise is not None:
+        r = r + noise(u)
+    vertices = np.asarray(centre, dtype=np.float64)[None, :] + r[:, None] * u
+    north, south = 0, len(u) - 1
+    j = np.arange(n_segments)
+    jn = (j + 1) % n_segments
+    faces = [np.stack([np.full(n_segments, north), 1 + j, 1 + jn], axis=1)]
+    for i in range(n_rings - 2):
+        a = 1 + i * n_segments + j
+        b = 1 + (i + 1) * n_segments + j
+        c = 1 + (i + 1) * n_segments + jn
+        d = 1 + i * n_segments + jn
+        faces.append(np.stack([a, b, c], axis=1))
+        faces.append(np.stack([a, c, d], axis=1))
+    last = 1 + (n_rings - 2) * n_segments
+    faces.append(np.stack([np.full(n_segments, south), last + jn, last + j], axis=1))
+    faces = np.vstack(faces).astype(np.int64)
+    if inward:
+        faces = faces[:, ::-1].copy()
+    return vertices, faces
+
+
+def shell_domain(rng, shape, outer, inner, n_rings: int = 315, n_segments: int = 318, amplitude: float = 2.0):
+    """Two nested displaced ellipsoid surfaces (semi-axes in voxels) about
+    the volume centre, the inner one facing inward, as one triangle list,
+    with the radial functions that define them."""
+    centre = (np.array(shape) - 1) / 2.0
+    outer, inner = np.asarray(outer, dtype=np.float64), np.asarray(inner, dtype=np.float64)
+    noise_out = radial_noise(rng, amplitude)
+    noise_in = radial_noise(rng, amplitude)
+    vo, fo = ellipsoid_surface(centre, outer, n_rings, n_segments, noise_out)
+    vi, fi = ellipsoid_surface(centre, inner, n_rings, n_segments, noise_in, inward=True)
+    vertices = np.vstack([vo, vi]).astype(np.float32)
+    faces = np.vstack([fo, fi + len(vo)]).astype(np.uint32)
+
+    def radii(u):
+        return (ellipsoid_radius(u, inner) + noise_in(u), ellipsoid_radius(u, outer) + noise_out(u))
+
+    return vertices, faces, centre, radii
+
+
+def seeds_in_shell(rng, centre, radii, extent, n_seeds: int, margin: float = 0.5) -> np.ndarray:
+    """Uniform seeds between the two surfaces (at least `margin` from each),
+    by rejection in the box centre +- extent; normalised Gaussian directions."""
+    kept = []
+    count = 0
+    while count < n_seeds:
+        q = centre + rng.uniform(-1.0, 1.0, (max(4096, 3 * (n_seeds - count)), 3)) * extent
+        d = q - centre
+        r = np.linalg.norm(d, axis=1)
+        u = d / np.maximum(r, 1e-12)[:, None]
+        r_in, r_out = radii(u)
+        q = q[(r > r_in + margin) & (r < r_out - margin)]
+        kept.append(q)
+        count += len(q)
+    locations = np.vstack(kept)[:n_seeds]
+    directions = rng.standard_normal((n_seeds, 3))
+    directions /= np.linalg.norm(directions, axis=1, keepdims=True)
+    seeds = np.zeros((n_seeds, 8), dtype=np.float32)
+    seeds[:, :3] = locations
+    seeds[:, 3] = 1
+    seeds[:, 4:7] = directions
+    return seeds
+
+
+def brain145_shell_diffusion(seed: int = 0, n_seeds: int = 2_000_000, shape=SHAPE_145, n_rings: int = 315,
+                             n_segments: int = 318, amplitude: float = 2.0):
+    """The brain145_diffusion field with the domain bounded by two nested
+    closed meshes (ellipsoids 60/75/60 and 30/40/30, ~200k triangles each,
+    vertices displaced radially by degree-4 noise of the given amplitude);
+    seeds uniform between the surfaces; Diffusion gamma 1, noise variance
+    0.1, step 0.1, save 0.25, max 300. Tracking stops on a boundary hit."""
+    cfg = brain145_diffusion(seed, n_seeds=0, shape=shape)
+    rng = np.random.default_rng(seed + 1)
+    vertices, faces, centre, radii = shell_domain(rng, shape, _semi_axes(shape, (60, 75, 60)),
+                                                  _semi_axes(shape, (30, 40, 30)), n_rings, n_segments, amplitude)
+    extent = _semi_axes(shape, (60, 75, 60)) + amplitude + 1.0
+    cfg["seeds"] = seeds_in_shell(rng, centre, radii, extent, n_seeds)
+    cfg["mesh"] = (vertices, faces)
+    return cfg
+
+
 # --------------------------------------------------------------------------
 # Measurement
 # --------------------------------------------------------------------------
@@ -226,19 +377,30 @@ def config1(seed: int = 0, n_seeds: int = 1_000_000, shape=SHAPE_145):
 MI355X_HBM_PEAK_GBS = 8000.0
 
 
-def bytes_per_step(K: int, step: float, save_at: float) -> float:
-    """Algorithmic HBM bytes per step: K f32 coefficients, one mask byte
-    counted as the reference's 4-byte int, and 12 of the 16 bytes of a saved
-    point times saves per step."""
-    return 4.0 * K + 4.0 + 16.0 * step / save_at
+def bytes_per_step(K: int, step: float, save_at: float, mask: bool = True) -> float:
+    """Algorithmic HBM bytes per step: K f32 coefficients, one mask word (the
+    reference's 4-byte int; mask domains only) and a 16-byte saved point
+    times saves per step. BVH traversal of mesh domains is not counted."""
+    return 4.0 * K + (4.0 if mask else 0.0) + 16.0 * step / save_at
 
 
 def make_algorithm(tg, cfg, seed: int = 0):
-    model = tg.local.RealSymmetricSphericalHarmonics(cfg["fod"].shape[-1])
+    if cfg.get("model") == "dti":
+        model = tg.local.DiffusionTensorImaging()
+    else:
+        model = tg.local.RealSymmetricSphericalHarmonics(cfg["fod"].shape[-1])
     if cfg["algorithm"] == "transport":
         return tg.algorithm.Transport(model, step_size=cfg["step_size"], gamma=cfg["gamma"])
     return tg.algorithm.Diffusion(model, step_size=cfg["step_size"], gamma=cfg["gamma"],
                                   noise_variance=cfg["noise_variance"], seed=seed)
+
+
+def make_domain(tg, cfg):
+    """The workload's tracking domain: its boundary meshes if it has any,
+    else its mask image."""
+    if cfg.get("mesh") is not None:
+        return tg.domain.Mesh(*cfg["mesh"])
+    return tg.domain.Image(tg.Nifti1Image(cfg["mask"], np.eye(4)))
 
 
 class DeviceRun:
@@ -249,9 +411,8 @@ class DeviceRun:
 
         self.tg, self.torch, self.runtime, self.cfg = tg, torch, runtime, cfg
         self.native = _native
-        fod_image = tg.Nifti1Image(cfg["fod"], np.eye(4))
-        self.field = tg.field.Image(fod_image)
-        self.domain = tg.domain.Image(tg.Nifti1Image(cfg["mask"], np.eye(4)))
+        self.field = tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4)))
+        self.domain = make_domain(tg, cfg)
         self.algorithm = make_algorithm(tg, cfg)
         self.n_points = tg.tracking.point_capacity(cfg["max_length"], cfg["save_at"])
         seeds = np.ascontiguousarray(cfg["seeds"], dtype=np.float32)
@@ -337,7 +498,7 @@ def host_leg(cfg, gpu_results: dict, seconds: float = 15.0, n_accuracy: int = 40
     import tractography_amd as tg
 
     field = tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4)))
-    domain = tg.domain.Image(tg.Nifti1Image(cfg["mask"], np.eye(4)))
+    domain = make_domain(tg, cfg)
     algorithm = make_algorithm(tg, cfg)
     rng_seeds = [algorithm.get_rng_seed() for _ in range((len(cfg["seeds"]) + 49_999) // 50_000)]
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
@@ -368,12 +529,25 @@ def host_leg(cfg, gpu_results: dict, seconds: float = 15.0, n_accuracy: int = 40
     return baseline, accuracy
 
 
+WORKLOADS = {
+    "sphere32": (sphere32_transport, 10_000,
+                 "32^3 SH45 (lmax 8) Transport gamma 1, {n} seeds, step 0.1, save 0.25, max 300"),
+    "sh45": (brain145_diffusion, 1_000_000,
+             "145x174x145 SH45 (lmax 8) Diffusion gamma 1 sigma^2 0.1, {n} seeds, step 0.1, save 0.25, max 300"),
+    "dti": (brain145_dti_transport, 1_000_000,
+            "145x174x145 DTI Transport gamma 1, {n} seeds, step 0.1, save 0.25, max 300"),
+    "shell": (brain145_shell_diffusion, 2_000_000,
+              "145x174x145 SH45 Diffusion gamma 1 sigma^2 0.1 between two displaced ellipsoid meshes "
+              "(2 x ~200k triangles, BVH termination), {n} seeds, step 0.1, save 0.25, max 300"),
+}
+
+
 def main():
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=3)
     parser.add_argument("--warmup", type=int, default=1)
-    parser.add_argument("--config", type=int, default=1, choices=[0, 1])
+    parser.add_argument("--workload", default="sh45", choices=sorted(WORKLOADS))
     parser.add_argument("--seeds", type=int, default=None)
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--paths", default="fast,plain")
@@ -389,13 +563,9 @@ def main():
     from tractography_amd import _native
 
     t0 = time.perf_counter()
-    if args.config == 0:
-        cfg = config0(n_seeds=args.seeds or 10_000)
-        workload = "config0: 32^3 SH45 Transport, 10k seeds, step 0.1, save 0.25, max 300"
-    else:
-        cfg = config1(n_seeds=args.seeds or 1_000_000)
-        workload = ("config1: 145x174x145 SH45 (lmax 8) Diffusion gamma 1 sigma^2 0.1, "
-                    f"{len(cfg['seeds'])} seeds, step 0.1, save 0.25, max 300")
+    generator, default_seeds, description = WORKLOADS[args.workload]
+    cfg = generator(n_seeds=args.seeds or default_seeds)
+    workload = f"{args.workload}: {description.format(n=len(cfg['seeds']))}"
     print(f"[bench] workload generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
     runtime = tg.HipRuntime()
@@ -417,7 +587,7 @@ def main():
     headline = "fast" if "fast" in results else "plain"
     h = results[headline]
     K = cfg["fod"].shape[-1]
-    bps = bytes_per_step(K, cfg["step_size"], cfg["save_at"])
+    bps = bytes_per_step(K, cfg["step_size"], cfg["save_at"], mask=cfg.get("mesh") is None)
     achieved = bps * h["steps"] / (h["kernel_ms"] * 1e-3) / 1e9
     line = {
         "metric": "streamline steps/s on 1 MI355X, mixed vs FP32 path, at <=1e-3 voxel max deviation",

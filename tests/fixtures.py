@@ -160,3 +160,41 @@ def expected_quadratic_update(orientation, baseline, epsilon, step_size, gamma):
     if tangent_norm == 0:
         return orientation.copy()
     return np.cos(tangent_norm) * orientation + np.sin(tangent_norm) / tangent_norm * tangent
+
+
+def shell_scene(shape=(32, 32, 32), outer=(13.0, 14.0, 12.0), inner=(5.0, 6.0, 5.0), n_rings=30, n_segments=36,
+                amplitude=1.0, seed=0, n_seeds=1000):
+    """A mesh domain bounded by two displaced ellipsoids about the volume
+    centre (the shell workload at small size): (Mesh domain, seeds, radii)."""
+    import bench
+
+    rng = np.random.default_rng(seed)
+    vertices, faces, centre, radii = bench.shell_domain(rng, shape, outer, inner, n_rings, n_segments, amplitude)
+    seeds = bench.seeds_in_shell(rng, centre, radii, np.asarray(outer) + amplitude + 1.0, n_seeds)
+    return tg.domain.Mesh(vertices, faces), seeds, centre, radii
+
+
+def first_ray_hits(vertices, faces, origins, directions):
+    """Brute-force closest triangle along each ray (FP64 Moller-Trumbore):
+    (triangle index or -1, distance, smallest barycentric margin)."""
+    tri = np.asarray(vertices, np.float64)[np.asarray(faces, np.int64)]
+    a, e1, e2 = tri[:, 0], tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]
+    best = np.full(len(origins), -1, np.int64)
+    dist = np.full(len(origins), np.inf)
+    margin = np.zeros(len(origins))
+    for i, (o, d) in enumerate(zip(np.asarray(origins, np.float64), np.asarray(directions, np.float64))):
+        p = np.cross(d[None, :], e2)
+        det = np.einsum("ij,ij->i", e1, p)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            inv = 1.0 / det
+            s = o[None, :] - a
+            u = np.einsum("ij,ij->i", s, p) * inv
+            q = np.cross(s, e1)
+            v = (q @ d) * inv
+            t = np.einsum("ij,ij->i", e2, q) * inv
+        ok = (np.abs(det) > 1e-12) & (u >= 0) & (v >= 0) & (u + v <= 1) & (t > 1e-9)
+        if ok.any():
+            j = np.flatnonzero(ok)[np.argmin(t[ok])]
+            best[i], dist[i] = j, t[j]
+            margin[i] = min(u[j], v[j], 1 - u[j] - v[j])
+    return best, dist, margin

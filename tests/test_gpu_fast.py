@@ -20,7 +20,7 @@ TOLERANCE = 1e-3
 
 @pytest.fixture(scope="module")
 def scene():
-    cfg = bench.config0(seed=2, n_seeds=2000)
+    cfg = bench.sphere32_transport(seed=2, n_seeds=2000)
     field = tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4)))
     domain = tg.domain.Image(tg.Nifti1Image(cfg["mask"], np.eye(4)))
     return cfg, field, domain
@@ -138,3 +138,19 @@ def test_fast_path_is_deterministic_and_chunk_invariant(fast_runtime, scene):
     for x, y in [(a, b), (a, c)]:
         np.testing.assert_array_equal(x.lengths, y.lengths)
         np.testing.assert_array_equal(x.points, y.points)
+
+
+def test_fast_mesh_domain_tracks_fp64_oracle(fast_runtime, scene, oracle_threads):
+    cfg, field, _ = scene
+    mesh, seeds, _, _ = fixtures.shell_scene(n_seeds=2000, seed=12)
+    algorithm = tg.algorithm.Diffusion(tg.local.RealSymmetricSphericalHarmonics(45), step_size=0.1, gamma=1.0,
+                                       noise_variance=0.1, seed=22)
+    rng = [algorithm.get_rng_seed()]
+    got = tg.tracking.track(algorithm, mesh, field, seeds, rng, 2000, 0.25, 80.0, fast_runtime)
+    ref = oracle.track(mesh, field, algorithm, seeds, rng, 2000, 0.25, 80.0, precision=64, threads=oracle_threads)
+    r = deviation.report(got.lengths, got.steps, got.points, ref)
+    same = got.lengths == ref["lengths"]
+    hits_equal = np.mean(got.hits[same] == ref["hits"][same])
+    print(f"\n[mesh] fast {r}, hit ids equal on {hits_equal:.4f} of equal-length streamlines")
+    assert r["max_deviation"] <= TOLERANCE
+    assert hits_equal > 0.99

@@ -164,7 +164,7 @@ def _compare_tracks(got, ref):
 
 @pytest.fixture(scope="module")
 def cfg0():
-    return bench.config0(seed=0, n_seeds=3000)
+    return bench.sphere32_transport(seed=0, n_seeds=3000)
 
 
 @pytest.mark.parametrize("name", ["transport", "diffusion"])
@@ -291,3 +291,23 @@ def test_public_tractogram_batches_and_boundary_seeds(runtime):
     assert len(result.streamlines) == len(expected)
     for got, exp in zip(result.streamlines, expected):
         np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.parametrize("name", ["transport", "diffusion"])
+def test_plain_mesh_domain_bit_exact(runtime, cfg0, name, oracle_threads):
+    # Termination on a two-surface boundary mesh (BVH per step segment):
+    # lengths, steps, hit triangles and points equal the FP32 restatement.
+    mesh, seeds, _, _ = fixtures.shell_scene(n_seeds=3000, seed=11)
+    field = tg.field.Image(tg.Nifti1Image(cfg0["fod"], np.eye(4)))
+    model = tg.local.RealSymmetricSphericalHarmonics(45)
+    if name == "transport":
+        algorithm = tg.algorithm.Transport(model, step_size=0.1, gamma=1.0)
+    else:
+        algorithm = tg.algorithm.Diffusion(model, step_size=0.1, gamma=1.0, noise_variance=0.1, seed=6)
+    rng = [algorithm.get_rng_seed() for _ in range(3)]
+    got = tg.tracking.track(algorithm, mesh, field, seeds, rng, 1000, 0.25, 60.0, runtime)
+    ref = oracle.track(mesh, field, algorithm, seeds, rng, 1000, 0.25, 60.0, threads=oracle_threads)
+    _compare_tracks(got, ref)
+    hit = got.hits != 0xFFFFFFFF
+    print(f"[mesh {name}] {hit.mean():.3f} of streamlines end on the boundary, {int(got.steps.sum())} steps")
+    assert hit.mean() > 0.3

@@ -30,7 +30,7 @@ def _check_bit_exact(runtime, fod_image, model_id, K, n, seed):
 
 def test_sh45_seeds_bit_exact():
     runtime = tg.HipRuntime(0)
-    cfg = bench.config0(seed=3, n_seeds=16)
+    cfg = bench.sphere32_transport(seed=3, n_seeds=16)
     fod = cfg["fod"] * cfg["mask"][..., None]
     image = tg.Nifti1Image(fod, np.diag([1.5, 1.5, 2.0, 1.0]))
     _check_bit_exact(runtime, image, 0, 45, 30000, 11)
@@ -51,7 +51,7 @@ def test_sh15_seeds_bit_exact():
     # The public from_fod accepts 45 or 6 coefficients only (local.py:265-282);
     # the kernel takes any K <= 45.
     runtime = tg.HipRuntime(0)
-    cfg = bench.config0(seed=4, n_seeds=16)
+    cfg = bench.sphere32_transport(seed=4, n_seeds=16)
     fod = np.ascontiguousarray(cfg["fod"][..., :15] * cfg["mask"][..., None])
     inline, voxels, affine = tg.seeds.sparse_fod(tg.Nifti1Image(fod, np.eye(4)))
     states = _states(10000, 12)
@@ -188,7 +188,7 @@ def _mfma_seeds(runtime, model_id, K, inline, voxels, affine, states):
 def test_mfma_seeds_against_the_host(case):
     runtime = tg.HipRuntime(0)
     if case == "sh45":
-        cfg = bench.config0(seed=5, n_seeds=16)
+        cfg = bench.sphere32_transport(seed=5, n_seeds=16)
         fod = cfg["fod"] * cfg["mask"][..., None]
         image, model_id, K = tg.Nifti1Image(fod, np.diag([1.5, 1.5, 2.0, 1.0])), 0, 45
     else:

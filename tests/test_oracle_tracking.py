@@ -183,3 +183,29 @@ def test_fp32_and_fp64_restatements_agree_before_voxel_flips():
             assert running[k - 1] < 1e-3
     assert compared > 500
     assert domain_image is not None
+
+
+def test_mesh_domain_straight_rays_stop_at_the_first_triangle():
+    # Isotropic field: orientations never change, so each streamline is the
+    # ray from its seed; tracking stops after the step whose segment crosses
+    # the first triangle along the ray (bvh.cl:83-131 per step segment).
+    mesh, seeds, centre, radii = fixtures.shell_scene(n_seeds=300, seed=3)
+    fod = fixtures.uniform_isotropic(shape=(32, 32, 32))
+    field = tg.field.Image(tg.Nifti1Image(fod, np.eye(4)))
+    algorithm = tg.algorithm.Transport(tg.local.RealSymmetricSphericalHarmonics(45), step_size=0.1, gamma=1.0)
+    r = _run(mesh, field, algorithm, seeds, 0.5, 200.0)
+    data = mesh._ensure_boundary_data()
+    faces = data["indices"][:, :3]
+    expected, dist, margin = fixtures.first_ray_hits(data["vertices"][:, :3], faces, seeds[:, :3], seeds[:, 4:7])
+    assert np.all(expected >= 0)
+    ratio = dist / 0.1
+    clear = (margin > 1e-4) & (np.abs(ratio - np.round(ratio)) > 2e-3)
+    assert clear.mean() > 0.9
+    np.testing.assert_array_equal(r["hits"][clear], expected[clear])
+    np.testing.assert_array_equal(r["steps"][clear], np.ceil(ratio[clear]).astype(np.uint32))
+    # Saved points stay in the shell, up to the last (crossing) step.
+    for i in range(len(seeds)):
+        p = _streamline(r, i).astype(np.float64) - centre
+        rr = np.linalg.norm(p, axis=1)
+        r_in, r_out = radii(p / rr[:, None])
+        assert np.all(rr > r_in - 0.11) and np.all(rr < r_out + 0.11)
