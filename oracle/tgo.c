@@ -112,6 +112,75 @@ static inline void tgo_sincos64(double x, double *s, double *c)
     *c = cos(x);
 }
 
+/* Real symmetric SH of any even order lmax <= TGO_LMAX_GENERIC with the
+ * colatitude derivative and the azimuth derivative over sin(colatitude), in
+ * the convention of core.py:36-69 (MRtrix order, index l(l+1)/2 + m; m > 0
+ * sqrt(2) N P_l^m cos(m phi), m < 0 sqrt(2) N P_l^|m| sin(|m| phi), P with
+ * the Condon-Shortley phase). No reference kernel goes past l = 8: this
+ * extends the FP64 oracle to order 12 (91 coefficients) for the fast path.
+ * P_l^m by the three-term recurrence in l; P_l^m / sin(theta) (m >= 1) by the
+ * same recurrence from (-1)^m (2m-1)!! sin^(m-1); d P_l^m / d theta =
+ * (P_l^(m+1) - (l+m)(l-m+1) P_l^(m-1)) / 2, and P_l^1 for m = 0. */
+#define TGO_LMAX_GENERIC 16
+static void tgo_sh_generic64(int lmax, double phi, double theta, double *ylm, double *dp, double *dt)
+{
+    enum { L = TGO_LMAX_GENERIC + 2 };
+    double P[L][L], Q[L][L];
+    const double st = sin(theta), ct = cos(theta);
+    for (int l = 0; l < L; l++)
+        for (int m = 0; m < L; m++)
+            P[l][m] = Q[l][m] = 0.0;
+    const int top = lmax + 1; /* P_l^(m+1) needs m + 1 <= l */
+    double pmm = 1.0, qmm = 1.0;
+    for (int m = 0; m <= top; m++) {
+        if (m > 0) {
+            pmm *= -(2.0 * m - 1.0) * st;
+            qmm = (m == 1) ? -1.0 : qmm * -(2.0 * m - 1.0) * st;
+        }
+        P[m][m] = pmm;
+        Q[m][m] = m > 0 ? qmm : 0.0;
+        if (m + 1 <= top) {
+            P[m + 1][m] = (2.0 * m + 1.0) * ct * pmm;
+            Q[m + 1][m] = (2.0 * m + 1.0) * ct * Q[m][m];
+        }
+        for (int l = m + 2; l <= top; l++) {
+            P[l][m] = ((2.0 * l - 1.0) * ct * P[l - 1][m] - (l + m - 1.0) * P[l - 2][m]) / (double)(l - m);
+            Q[l][m] = ((2.0 * l - 1.0) * ct * Q[l - 1][m] - (l + m - 1.0) * Q[l - 2][m]) / (double)(l - m);
+        }
+    }
+    for (int l = 0; l <= lmax; l += 2) {
+        const int centre = l * (l + 1) / 2;
+        for (int m = 0; m <= l; m++) {
+            double ratio = 1.0; /* (l - m)! / (l + m)! */
+            for (int k = l - m + 1; k <= l + m; k++)
+                ratio /= (double)k;
+            const double N = sqrt((2.0 * l + 1.0) / (4.0 * 3.141592653589793) * ratio);
+            const double dP = m == 0 ? P[l][1] : 0.5 * (P[l][m + 1] - (double)(l + m) * (l - m + 1.0) * P[l][m - 1]);
+            if (m == 0) {
+                ylm[centre] = N * P[l][0];
+                dt[centre] = N * dP;
+                dp[centre] = 0.0;
+            } else {
+                const double F = sqrt(2.0) * N, cm = cos(m * phi), sm = sin(m * phi);
+                ylm[centre + m] = F * P[l][m] * cm;
+                dt[centre + m] = F * dP * cm;
+                dp[centre + m] = -F * m * Q[l][m] * sm;
+                ylm[centre - m] = F * P[l][m] * sm;
+                dt[centre - m] = F * dP * sm;
+                dp[centre - m] = F * m * Q[l][m] * cm;
+            }
+        }
+    }
+}
+
+void tgo64_sh_basis(int lmax, const double *phi, const double *theta, int64_t n, double *ylm, double *dp, double *dt)
+{
+    const int K = (lmax + 1) * (lmax + 2) / 2;
+    for (int64_t i = 0; i < n; i++)
+        tgo_sh_generic64(lmax, phi[i], theta[i], ylm + (size_t)K * i, dp + (size_t)K * i, dt + (size_t)K * i);
+}
+
+#define TGO_GENERIC_SH 1
 #define R double
 #define FN(x) tgo64_##x
 #define K(c) c

@@ -66,6 +66,10 @@ void tgo_rng_draws(uint32_t state[2], int kind, uint32_t aux, int64_t n, void *o
 /* SH basis l<=8 (spharm.cl:216-413): outputs (n, 45) each. */
 void tgo32_sh45(const float *phi, const float *theta, int64_t n, float *ylm, float *dp, float *dt);
 void tgo64_sh45(const double *phi, const double *theta, int64_t n, double *ylm, double *dp, double *dt);
+/* FP64 real SH of any even order lmax <= 16 (core.py:36-69 convention), K =
+ * (lmax + 1)(lmax + 2) / 2 values per point; extends the FP64 models past
+ * the reference's l = 8 (tgo64_evaluate/update/track take K up to 153). */
+void tgo64_sh_basis(int lmax, const double *phi, const double *theta, int64_t n, double *ylm, double *dp, double *dt);
 
 /* cart2sph (core.cl:142-153): out (n, 2) azimuth, colatitude. */
 void tgo32_cart2sph(const float *xyz, int64_t n, float *out);

@@ -246,13 +246,27 @@ typedef struct {
 } FN(model_value);
 #define MV FN(model_value)
 
-/* local.py:91-120 (SH, K <= 45) */
+/* local.py:91-120 (SH, K <= 45; the FP64 instance also takes even orders
+ * past 8, up to the generic basis's limit) */
 static inline MV FN(sh_model)(const R *model, int n_parameters, VEC o)
 {
+#ifdef TGO_GENERIC_SH
+    R ylm[(TGO_LMAX_GENERIC + 1) * (TGO_LMAX_GENERIC + 2) / 2], dp[sizeof ylm / sizeof ylm[0]],
+        dt[sizeof ylm / sizeof ylm[0]];
+#else
     R ylm[45], dp[45], dt[45];
+#endif
     R az, col;
     FN(cart2sph_one)(o, &az, &col);
-    FN(sh45_one)(az, col, ylm, dp, dt);
+#ifdef TGO_GENERIC_SH
+    if (n_parameters > 45) {
+        int lmax = 0;
+        while ((lmax + 1) * (lmax + 2) / 2 < n_parameters)
+            lmax += 2;
+        tgo_sh_generic64(lmax, az, col, ylm, dp, dt);
+    } else
+#endif
+        FN(sh45_one)(az, col, ylm, dp, dt);
     MV r;
     r.value = K(0.0);
     r.dtheta = K(0.0);

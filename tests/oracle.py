@@ -190,6 +190,16 @@ def sh45(phi, theta, precision=32):
     return ylm, dp, dt
 
 
+def sh_basis64(lmax: int, phi, theta):
+    """FP64 real SH of even order lmax with d/dtheta and (d/dphi)/sin(theta)."""
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    K = (lmax + 1) * (lmax + 2) // 2
+    ylm, dp, dt = (np.zeros((len(phi), K)) for _ in range(3))
+    lib().tgo64_sh_basis(c_int(lmax), _p(phi), _p(theta), c_int64(len(phi)), _p(ylm), _p(dp), _p(dt))
+    return ylm, dp, dt
+
+
 def cart2sph32(xyz):
     xyz = np.ascontiguousarray(xyz, dtype=np.float32)
     out = np.zeros((len(xyz), 2), np.float32)

@@ -154,3 +154,44 @@ def test_fast_mesh_domain_tracks_fp64_oracle(fast_runtime, scene, oracle_threads
     print(f"\n[mesh] fast {r}, hit ids equal on {hits_equal:.4f} of equal-length streamlines")
     assert r["max_deviation"] <= TOLERANCE
     assert hits_equal > 0.99
+
+
+def _order12_scene(n_seeds=1500, seed=30):
+    rng = np.random.default_rng(seed)
+    shape = (24, 24, 24)
+    directions = bench.smooth_direction_fields(rng, shape, sigma=3.0)
+    _, weights = bench._random_fibres(rng, int(np.prod(shape)), base_directions=directions)
+    fod = bench.fod_from_fibres(directions, weights, lmax=12).reshape(*shape, 91)
+    idx = np.indices(shape).reshape(3, -1).T
+    mask = (np.linalg.norm(idx - 11.5, axis=1) <= 10).reshape(shape)
+    fod[~mask] = 0
+    seeds = bench.seeds_in_mask(rng, mask, n_seeds)
+    field = tg.field.Image(tg.Nifti1Image(fod, np.eye(4)))
+    domain = tg.domain.Image(tg.Nifti1Image(mask.astype(np.uint8), np.eye(4)))
+    return field, domain, seeds
+
+
+@pytest.mark.parametrize("name", ["transport", "diffusion"])
+def test_fast_sh_order12_tracks_fp64_oracle(fast_runtime, name, oracle_threads):
+    # SH order 12 (91 coefficients) exists only on the fast path; the FP64
+    # oracle evaluates the closed-form basis (pinned against core.ishtmtx).
+    field, domain, seeds = _order12_scene()
+    model = tg.local.RealSymmetricSphericalHarmonics(91)
+    if name == "transport":
+        algorithm = tg.algorithm.Transport(model, step_size=0.1, gamma=1.0)
+    else:
+        algorithm = tg.algorithm.Diffusion(model, step_size=0.1, gamma=1.0, noise_variance=0.1, seed=31)
+    rng = [algorithm.get_rng_seed()]
+    got = tg.tracking.track(algorithm, domain, field, seeds, rng, 1500, 0.25, 60.0, fast_runtime)
+    ref = oracle.track(domain, field, algorithm, seeds, rng, 1500, 0.25, 60.0, precision=64, threads=oracle_threads)
+    r = deviation.report(got.lengths, got.steps, got.points, ref)
+    print(f"\n[sh91 {name}] fast {r}")
+    assert r["max_deviation"] <= TOLERANCE
+    assert r["compared_fraction"] > 0.5
+
+
+def test_plain_path_refuses_order12(runtime):
+    field, domain, seeds = _order12_scene(n_seeds=8)
+    algorithm = tg.algorithm.Transport(tg.local.RealSymmetricSphericalHarmonics(91), step_size=0.1, gamma=1.0)
+    with pytest.raises(NotImplementedError, match="91"):
+        tg.tracking.track(algorithm, domain, field, seeds, [0], 8, 0.25, 10.0, runtime)

@@ -163,3 +163,47 @@ def test_diffusion_without_noise_equals_transport_and_noise_is_reproducible():
     np.testing.assert_allclose(np.linalg.norm(first, axis=1), 1.0, atol=1e-5)
     assert np.any(np.linalg.norm(first - o, axis=1) > 0.0)
     np.testing.assert_allclose(oracle.update(noisy, 1234, params, o, 64), first, atol=1e-5)
+
+
+def test_generic_fp64_basis_matches_the_l8_basis():
+    rng = np.random.default_rng(40)
+    phi = rng.uniform(0, 2 * np.pi, 3000)
+    theta = np.concatenate([rng.uniform(0, np.pi, 2997), [0.0, np.pi, 1e-9]])
+    y8, dp8, dt8 = oracle.sh45(phi, theta, precision=64)
+    y, dp, dt = oracle.sh_basis64(8, phi, theta)
+    # The l <= 8 expressions carry the reference's 11-digit constants.
+    np.testing.assert_allclose(y, y8, atol=1e-9)
+    np.testing.assert_allclose(dp, dp8, atol=1e-8)
+    np.testing.assert_allclose(dt, dt8, atol=1e-8)
+
+
+@pytest.mark.parametrize("lmax", [10, 12])
+def test_generic_fp64_basis_matches_ishtmtx(lmax):
+    # Order 12 has no reference kernel; it is pinned by the closed form
+    # (core.py:36-69, scipy's sph_harm_y) with its angular derivatives.
+    rng = np.random.default_rng(lmax)
+    phi = rng.uniform(0, 2 * np.pi, 2000)
+    theta = rng.uniform(1e-3, np.pi - 1e-3, 2000)
+    K = (lmax + 1) * (lmax + 2) // 2
+    y, dp, dt = oracle.sh_basis64(lmax, phi, theta)
+    matrix, derivatives = tg.core.ishtmtx(phi, theta, K)
+    np.testing.assert_allclose(y, matrix, atol=1e-10)
+    np.testing.assert_allclose(dt, derivatives[0], atol=1e-9)
+    np.testing.assert_allclose(dp * np.sin(theta)[:, None], derivatives[1], atol=1e-9)
+
+
+def test_fp64_model_of_order_12_matches_the_closed_form():
+    rng = np.random.default_rng(41)
+    params = rng.normal(size=(500, 91)) * 0.1
+    o = rng.standard_normal((500, 3))
+    o /= np.linalg.norm(o, axis=1, keepdims=True)
+    got = oracle.evaluate(0, 91, params, o, 64)
+    az, col, _ = tg.core.cart2sph(*o.T)
+    matrix, derivatives = tg.core.ishtmtx(az, col, 91)
+    v = np.einsum("nk,nk->n", matrix, params)
+    d = 1 / (1 + np.exp(-100 * v))
+    value = np.where(100 * v < 30, np.log1p(np.exp(np.minimum(100 * v, 30))) / 100, v)
+    np.testing.assert_allclose(got[:, 0], value, atol=1e-12, rtol=1e-10)
+    np.testing.assert_allclose(got[:, 1], d * np.einsum("nk,nk->n", derivatives[0], params), atol=1e-10)
+    np.testing.assert_allclose(got[:, 2] * np.sin(col), d * np.einsum("nk,nk->n", derivatives[1], params),
+                               atol=1e-10)

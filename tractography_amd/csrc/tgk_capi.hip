@@ -168,7 +168,10 @@ tgk::MeshView mesh_view(const tgk_domain *d)
     return v;
 }
 
-int check_algorithm(const tgk_algorithm *a, const tgk_field *f)
+// max_sh: the largest SH coefficient count the caller's kernels take (45 for
+// the reference's l <= 8 basis; 91 where the order-12 polynomial form is
+// available).
+int check_algorithm(const tgk_algorithm *a, const tgk_field *f, int max_sh = 45)
 {
     if (!a)
         return fail(TGK_ERR_INVALID, "algorithm is NULL");
@@ -178,8 +181,9 @@ int check_algorithm(const tgk_algorithm *a, const tgk_field *f)
         return fail(TGK_ERR_INVALID, "unknown algorithm %d", a->algorithm);
     if (a->model == TGK_MODEL_DTI && a->n_parameters != 6)
         return fail(TGK_ERR_INVALID, "DTI needs 6 parameters, got %d", a->n_parameters);
-    if (a->model == TGK_MODEL_SH && (a->n_parameters < 1 || a->n_parameters > 45))
-        return fail(TGK_ERR_UNSUPPORTED, "SH with %d coefficients: 1..45 supported", a->n_parameters);
+    if (a->model == TGK_MODEL_SH && (a->n_parameters < 1 || a->n_parameters > max_sh))
+        return fail(TGK_ERR_UNSUPPORTED, "SH with %d coefficients: 1..%d supported%s", a->n_parameters, max_sh,
+                    max_sh == 45 ? " (up to 91 on the fast path)" : "");
     if (f && (uint32_t)a->n_parameters != f->dims[3])
         return fail(TGK_ERR_INVALID, "incompatible local model and field: %d != %u", a->n_parameters, f->dims[3]);
     return TGK_OK;
@@ -509,10 +513,10 @@ int check_track_args(const tgk_domain *domain, const tgk_field *field, const tgk
 {
     if (!domain || !field)
         return fail(TGK_ERR_INVALID, "domain and field are required");
-    if (int r = check_algorithm(algorithm, field))
-        return r;
     if (path != TGK_PATH_PLAIN && path != TGK_PATH_FAST)
         return fail(TGK_ERR_INVALID, "unknown path %d", path);
+    if (int r = check_algorithm(algorithm, field, path == TGK_PATH_FAST ? 91 : 45))
+        return r;
     if (n_seeds < 0)
         return fail(TGK_ERR_INVALID, "negative seed count");
     if (batch_size <= 0)
@@ -754,12 +758,14 @@ int run_track(tgk_runtime *rt, const tgk_domain *domain, tgk_field *field, const
     if (field->fast_precision != TGK_PRECISION_FP32)
         return fail(TGK_ERR_UNSUPPORTED, "fast-path coefficient storage %d is not available", field->fast_precision);
     const int32_t model = algorithm->model;
+    const int32_t sh_order = algorithm->n_parameters > 45 ? 12 : 8;
     if (field->fast_cached != model) {
-        const uint32_t out_stride = model == TGK_MODEL_SH ? 48u : 8u;
+        const uint32_t out_stride = model == TGK_MODEL_SH ? (sh_order == 12 ? 92u : 48u) : 8u;
         const int64_t n_voxels = (int64_t)field->dims[0] * field->dims[1] * field->dims[2];
         TGK_HIP(field->fast.allocate((size_t)n_voxels * out_stride * sizeof(float)), "fast field allocation");
         TGK_HIP(tgk::launch_field_transform(field->data.as<const float>(), n_voxels, field->stride, model,
-                                            field->fast.ptr, out_stride, rt->stream),
+                                            (int32_t)field->dims[3], sh_order, field->fast.ptr, out_stride,
+                                            rt->stream),
                 "field transform");
         field->fast_cached = model;
         field->fast_stride = out_stride;
@@ -771,6 +777,7 @@ int run_track(tgk_runtime *rt, const tgk_domain *domain, tgk_field *field, const
     f.precision = field->fast_precision;
     f.coeffs = field->fast.ptr;
     f.coeff_stride = field->fast_stride;
+    f.sh_order = sh_order;
     int grid = 0;
     if (const char *env = getenv("TGK_FAST_GRID"))
         grid = atoi(env);

@@ -26,8 +26,7 @@ namespace tgk {
 
 namespace {
 
-using fast::SH_STRIDE;
-using fast::SH_TERMS;
+using fast::Order;
 
 __device__ __forceinline__ void store_row(float *dense, int64_t g, uint32_t n_points, uint32_t n, V3 p)
 {
@@ -63,10 +62,11 @@ struct Lane {
 // One iteration of the reference loop (tracking.py:121-147) for a lane with
 // a streamline in flight; on termination the lane's results are written and
 // the lane becomes idle (L.g = -1).
-template <int MODEL, int ALGO, int DOMAIN>
+template <int MODEL, int ALGO, int DOMAIN, int LMAX>
 __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__restrict__ coeffs, uint32_t cstride,
-                                        Lane &L, float q[SH_TERMS])
+                                        Lane &L, float *q)
 {
+    constexpr int TERMS = Order<LMAX>::TERMS;
     bool stop = !(L.n < p.n_points && L.steps < p.max_steps);
     uint32_t hit = NO_HIT;
     if (!stop) {
@@ -106,15 +106,15 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
         const float4 *row = reinterpret_cast<const float4 *>(coeffs + cell * (int64_t)cstride);
         if constexpr (MODEL == MODEL_SH) {
 #pragma unroll
-            for (int j = 0; j < SH_STRIDE / 4; j++) {
+            for (int j = 0; j < Order<LMAX>::STRIDE / 4; j++) {
                 const float4 v = row[j];
-                if (4 * j < SH_TERMS)
+                if (4 * j < TERMS)
                     q[4 * j] = v.x;
-                if (4 * j + 1 < SH_TERMS)
+                if (4 * j + 1 < TERMS)
                     q[4 * j + 1] = v.y;
-                if (4 * j + 2 < SH_TERMS)
+                if (4 * j + 2 < TERMS)
                     q[4 * j + 2] = v.z;
-                if (4 * j + 3 < SH_TERMS)
+                if (4 * j + 3 < TERMS)
                     q[4 * j + 3] = v.w;
             }
         } else {
@@ -130,7 +130,7 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
     const fast::Direction dir = fast::direction_of(L.o);
     fast::ModelOut mv;
     if constexpr (MODEL == MODEL_SH)
-        mv = fast::sh_model(q, dir);
+        mv = fast::sh_model<LMAX>(q, dir);
     else
         mv = fast::dti_model(q, L.o, dir);
     L.o = fast::update<ALGO>(p.algo, mv, dir, L.o, L.rng);
@@ -148,7 +148,7 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
     }
 }
 
-template <int MODEL, int ALGO, int DOMAIN>
+template <int MODEL, int ALGO, int DOMAIN, int LMAX>
 __global__ __launch_bounds__(256) void track_fast_kernel(FastLaunch f)
 {
     const TrackLaunch &p = f.track;
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void track_fast_kernel(FastLaunch f)
 
     Lane L;
     L.g = -1;
-    float q[SH_TERMS];
+    float q[MODEL == MODEL_SH ? Order<LMAX>::TERMS : 6];
     bool drained = false;
 
     while (true) {
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void track_fast_kernel(FastLaunch f)
         if (__ballot(L.g >= 0) == 0)
             break;
         if (L.g >= 0)
-            advance<MODEL, ALGO, DOMAIN>(p, coeffs, cstride, L, q);
+            advance<MODEL, ALGO, DOMAIN, LMAX>(p, coeffs, cstride, L, q);
     }
 }
 
@@ -224,30 +224,32 @@ __global__ __launch_bounds__(256) void compact_seed_major_kernel(const float *__
     }
 }
 
-// out[v, r] = sum_i T[r, i] c[v, i] in FP64 for the 45 x 45 SH -> polynomial
-// map; rows past 45 of the padded stride are zero.
-__constant__ double c_sh_to_poly[SH_TERMS * SH_TERMS];
-
+// out[v, r] = sum_i T[r, i] c[v, i] in FP64 for the SH -> polynomial map of
+// order L (T row-major TERMS x TERMS in global memory, wave-uniform reads);
+// coefficients past the field's K are zero, rows past TERMS of the padded
+// stride are zero.
+template <int L>
 __global__ __launch_bounds__(256) void sh_to_poly_kernel(const float *__restrict__ field, int64_t n_voxels,
-                                                         uint32_t stride, uint32_t K, float *__restrict__ out,
-                                                         uint32_t out_stride)
+                                                         uint32_t stride, uint32_t K, const double *__restrict__ T,
+                                                         float *__restrict__ out, uint32_t out_stride)
 {
+    constexpr int TERMS = Order<L>::TERMS;
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n_voxels)
         return;
-    double c[SH_TERMS];
+    double c[TERMS];
 #pragma unroll
-    for (int i = 0; i < SH_TERMS; i++)
+    for (int i = 0; i < TERMS; i++)
         c[i] = i < (int)K ? (double)field[v * stride + i] : 0.0;
     float *dst = out + v * out_stride;
-    for (int r = 0; r < SH_TERMS; r++) {
+    for (int r = 0; r < TERMS; r++) {
         double acc = 0.0;
 #pragma unroll
-        for (int i = 0; i < SH_TERMS; i++)
-            acc += c_sh_to_poly[r * SH_TERMS + i] * c[i];
+        for (int i = 0; i < TERMS; i++)
+            acc += T[r * TERMS + i] * c[i];
         dst[r] = (float)acc;
     }
-    for (uint32_t r = SH_TERMS; r < out_stride; r++)
+    for (uint32_t r = TERMS; r < out_stride; r++)
         dst[r] = 0.0f;
 }
 
@@ -261,14 +263,17 @@ __global__ void copy_rows_kernel(const float *__restrict__ field, int64_t n_voxe
         out[v * out_stride + r] = r < stride ? field[v * stride + r] : 0.0f;
 }
 
-// T[r, i]: coefficient r of the polynomial layout (tgk_fast_model.h) per unit
-// SH coefficient i (MRtrix order). The real harmonic of order m > 0 is
-// sqrt(2) N_lm P_l^m(z) cos(m phi) (sin for -m) with
-// P_l^m = (-1)^m s^m d^m P_l / dz^m and N_lm = sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!).
-std::vector<double> sh_to_poly_matrix()
+// T[r, i]: coefficient r of the polynomial layout of order L
+// (tgk_fast_model.h) per unit SH coefficient i (MRtrix order). The real
+// harmonic of order m > 0 is sqrt(2) N_lm P_l^m(z) cos(m phi) (sin for -m)
+// with P_l^m = (-1)^m s^m d^m P_l / dz^m and
+// N_lm = sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!).
+template <int L> std::vector<double> sh_to_poly_matrix()
 {
-    std::vector<double> T(SH_TERMS * SH_TERMS, 0.0);
-    for (int l = 0; l <= 8; l += 2) {
+    using O = Order<L>;
+    constexpr int TERMS = O::TERMS;
+    std::vector<double> T((size_t)TERMS * TERMS, 0.0);
+    for (int l = 0; l <= L; l += 2) {
         const std::vector<double> P = shb::legendre(l);
         for (int m = 0; m <= l; m++) {
             const double N = std::sqrt((2.0 * l + 1.0) / (4.0 * M_PI) * shb::factorial(l - m) / shb::factorial(l + m));
@@ -276,13 +281,13 @@ std::vector<double> sh_to_poly_matrix()
             const int parity = m & 1;
             for (int side = 0; side < (m == 0 ? 1 : 2); side++) {
                 const int column = l * (l + 1) / 2 + (side == 0 ? m : -m);
-                const int base = side == 0 ? fast::a_offset(m) : fast::b_offset(m);
-                for (int j = 0; j < fast::poly_terms(m); j++) {
+                const int base = side == 0 ? O::a_offset(m) : O::b_offset(m);
+                for (int j = 0; j < O::poly_terms(m); j++) {
                     const int power = parity + 2 * j;
                     if (power + m > l)
                         continue;
                     const double coefficient = P[power + m] * shb::factorial(power + m) / shb::factorial(power);
-                    T[(base + j) * SH_TERMS + column] += F * coefficient;
+                    T[(size_t)(base + j) * TERMS + column] += F * coefficient;
                 }
             }
         }
@@ -290,13 +295,41 @@ std::vector<double> sh_to_poly_matrix()
     return T;
 }
 
-template <int MODEL, int ALGO, int DOMAIN>
+// The map of order L on the current device, uploaded on first use and kept
+// for the life of the process.
+template <int L> hipError_t device_matrix(const double **out)
+{
+    static const double *matrices[64] = {};
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess)
+        return e;
+    if (device < 0 || device >= 64)
+        return hipErrorInvalidDevice;
+    if (!matrices[device]) {
+        const std::vector<double> T = sh_to_poly_matrix<L>();
+        void *ptr = nullptr;
+        e = hipMalloc(&ptr, T.size() * sizeof(double));
+        if (e != hipSuccess)
+            return e;
+        e = hipMemcpy(ptr, T.data(), T.size() * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(ptr);
+            return e;
+        }
+        matrices[device] = static_cast<const double *>(ptr);
+    }
+    *out = matrices[device];
+    return hipSuccess;
+}
+
+template <int MODEL, int ALGO, int DOMAIN, int LMAX>
 hipError_t launch_fast(const FastLaunch &f, int grid_blocks, hipStream_t stream)
 {
     if (grid_blocks <= 0) {
         int per_cu = 0;
         hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void *>(track_fast_kernel<MODEL, ALGO, DOMAIN>), 256, 0);
+            &per_cu, reinterpret_cast<const void *>(track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX>), 256, 0);
         if (e != hipSuccess)
             return e;
         int device = 0;
@@ -308,16 +341,26 @@ hipError_t launch_fast(const FastLaunch &f, int grid_blocks, hipStream_t stream)
         grid_blocks = prop.multiProcessorCount * (per_cu > 0 ? per_cu : 1);
     }
     zero_counter<<<1, 1, 0, stream>>>(f.queue);
-    track_fast_kernel<MODEL, ALGO, DOMAIN><<<grid_blocks, 256, 0, stream>>>(f);
+    track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX><<<grid_blocks, 256, 0, stream>>>(f);
     return hipGetLastError();
+}
+
+template <int MODEL, int ALGO, int DOMAIN>
+hipError_t launch_fast_order(const FastLaunch &f, int grid, hipStream_t stream)
+{
+    if constexpr (MODEL == MODEL_SH) {
+        if (f.sh_order == 12)
+            return launch_fast<MODEL, ALGO, DOMAIN, 12>(f, grid, stream);
+    }
+    return launch_fast<MODEL, ALGO, DOMAIN, 8>(f, grid, stream);
 }
 
 template <int MODEL, int ALGO>
 hipError_t launch_fast_domain(const FastLaunch &f, int grid, hipStream_t stream)
 {
     if (f.track.domain_kind == DOMAIN_MESH)
-        return launch_fast<MODEL, ALGO, DOMAIN_MESH>(f, grid, stream);
-    return launch_fast<MODEL, ALGO, DOMAIN_MASK>(f, grid, stream);
+        return launch_fast_order<MODEL, ALGO, DOMAIN_MESH>(f, grid, stream);
+    return launch_fast_order<MODEL, ALGO, DOMAIN_MASK>(f, grid, stream);
 }
 
 template <int MODEL>
@@ -351,30 +394,23 @@ hipError_t launch_compact_seed_major(const float *dense, int64_t n_seeds, uint32
     return hipGetLastError();
 }
 
-hipError_t launch_field_transform(const float *field, int64_t n_voxels, uint32_t stride, int32_t model, void *out,
-                                  uint32_t out_stride, hipStream_t stream)
+hipError_t launch_field_transform(const float *field, int64_t n_voxels, uint32_t stride, int32_t model, int32_t K,
+                                  int32_t sh_order, void *out, uint32_t out_stride, hipStream_t stream)
 {
     if (n_voxels == 0)
         return hipSuccess;
     const unsigned blocks = (unsigned)((n_voxels + 255) / 256);
     if (model == MODEL_SH) {
-        // The map lives in constant memory of each device used.
-        static bool uploaded[64] = {};
-        int device = 0;
-        hipError_t e = hipGetDevice(&device);
+        const double *T = nullptr;
+        hipError_t e = sh_order == 12 ? device_matrix<12>(&T) : device_matrix<8>(&T);
         if (e != hipSuccess)
             return e;
-        if (device < 0 || device >= 64)
-            return hipErrorInvalidDevice;
-        if (!uploaded[device]) {
-            const std::vector<double> T = sh_to_poly_matrix();
-            e = hipMemcpyToSymbol(HIP_SYMBOL(c_sh_to_poly), T.data(), T.size() * sizeof(double));
-            if (e != hipSuccess)
-                return e;
-            uploaded[device] = true;
-        }
-        sh_to_poly_kernel<<<blocks, 256, 0, stream>>>(field, n_voxels, stride, stride < SH_TERMS ? stride : SH_TERMS,
-                                                      static_cast<float *>(out), out_stride);
+        if (sh_order == 12)
+            sh_to_poly_kernel<12><<<blocks, 256, 0, stream>>>(field, n_voxels, stride, (uint32_t)K, T,
+                                                              static_cast<float *>(out), out_stride);
+        else
+            sh_to_poly_kernel<8><<<blocks, 256, 0, stream>>>(field, n_voxels, stride, (uint32_t)K, T,
+                                                             static_cast<float *>(out), out_stride);
     } else {
         copy_rows_kernel<<<blocks, 256, 0, stream>>>(field, n_voxels, stride, static_cast<float *>(out), out_stride);
     }

@@ -1,8 +1,9 @@
 // tgk_fast_model.h -- model evaluation and orientation update of the fast
 // path (device side). Include only from tgk_fast.hip (contraction on).
 //
-// Real symmetric SH to l <= 8 in polynomial form. For a unit direction with
-// z = cos(theta), s = sin(theta) and u = s e^{i phi} = x + i y:
+// Real symmetric SH to even order L (8, or 12 past the reference's basis) in
+// polynomial form. For a unit direction with z = cos(theta), s = sin(theta)
+// and u = s e^{i phi} = x + i y:
 //
 //   f     = A_0(z) + sum_{m>=1} A_m(z) Re u^m + B_m(z) Im u^m
 //   df/dt = -s A_0'(z) + sum_{m>=1} m z (A_m Re v_m + B_m Im v_m)
@@ -10,11 +11,11 @@
 //   (df/dphi)/sin(theta) = sum_{m>=1} m (B_m Re v_m - A_m Im v_m)
 //
 // with v_m = u^{m-1} e^{i phi} and A_m, B_m polynomials in z whose monomial
-// coefficients are a fixed linear map of the voxel's 45 SH coefficients
-// (A_m collects the order +m harmonics, B_m the order -m ones; each includes
-// the normalisation, the sqrt(2) of the real basis and the Condon-Shortley
+// coefficients are a fixed linear map of the voxel's SH coefficients (A_m
+// collects the order +m harmonics, B_m the order -m ones; each includes the
+// normalisation, the sqrt(2) of the real basis and the Condon-Shortley
 // sign). The map is applied once per field upload (launch_field_transform),
-// so a step costs ~90 polynomial FMAs and two complex power sequences
+// so a step costs ~2K polynomial FMAs and two complex power sequences
 // instead of the trigonometric basis.
 #pragma once
 
@@ -23,21 +24,22 @@
 namespace tgk {
 namespace fast {
 
-// Layout of the 45 transformed coefficients per voxel: for m = 0..8, the
-// coefficients of A_m (then B_m for m > 0) by ascending power of z, powers
-// m mod 2, m mod 2 + 2, ..., 8 - m.
-constexpr int SH_TERMS = 45;
-constexpr int SH_STRIDE = 48;
-
-__host__ __device__ constexpr int poly_terms(int m) { return (8 - m - (m & 1)) / 2 + 1; }
-__host__ __device__ constexpr int a_offset(int m)
-{
-    int off = 0;
-    for (int k = 0; k < m; k++)
-        off += poly_terms(k) * (k == 0 ? 1 : 2);
-    return off;
-}
-__host__ __device__ constexpr int b_offset(int m) { return a_offset(m) + poly_terms(m); }
+// Layout of the (L+1)(L+2)/2 transformed coefficients per voxel: for
+// m = 0..L, the coefficients of A_m (then B_m for m > 0) by ascending power
+// of z, powers m mod 2, m mod 2 + 2, ..., L - m.
+template <int L> struct Order {
+    static constexpr int TERMS = (L + 1) * (L + 2) / 2;
+    static constexpr int STRIDE = (TERMS + 3) / 4 * 4;
+    __host__ __device__ static constexpr int poly_terms(int m) { return (L - m - (m & 1)) / 2 + 1; }
+    __host__ __device__ static constexpr int a_offset(int m)
+    {
+        int off = 0;
+        for (int k = 0; k < m; k++)
+            off += poly_terms(k) * (k == 0 ? 1 : 2);
+        return off;
+    }
+    __host__ __device__ static constexpr int b_offset(int m) { return a_offset(m) + poly_terms(m); }
+};
 
 // q(w) and q'(w) for q = sum_j c[j] w^j (Horner).
 template <int N>
@@ -52,12 +54,12 @@ __device__ __forceinline__ void horner(const float *c, float w, float &v, float 
     }
 }
 
-// A(z) and A'(z) for A(z) = z^p q(z^2), p = m mod 2.
-template <int M>
+// A(z) and A'(z) for A(z) = z^p q(z^2), p = m mod 2, q with N terms.
+template <int M, int N>
 __device__ __forceinline__ void poly_m(const float *c, float z, float w, float &a, float &ad)
 {
     float v, d;
-    horner<poly_terms(M)>(c, w, v, d);
+    horner<N>(c, w, v, d);
     if constexpr ((M & 1) == 0) {
         a = v;
         ad = 2.0f * z * d;
@@ -102,43 +104,48 @@ struct Raw {
     float value, dtheta, dphi;
 };
 
-__device__ __forceinline__ Raw sh8_evaluate(const float q[SH_TERMS], const Direction &dir)
+// Running state of the order loop: u^m and v_m.
+struct Powers {
+    float ur, ui, vr, vi;
+};
+
+template <int L, int M>
+__device__ __forceinline__ void sh_order_terms(const float *q, float z, float w, float s, float u1r, float u1i,
+                                               Powers &pw, Raw &r)
 {
+    if constexpr (M <= L) {
+        using O = Order<L>;
+        if constexpr (M > 1) {
+            const float nvr = pw.vr * u1r - pw.vi * u1i;
+            pw.vi = pw.vr * u1i + pw.vi * u1r;
+            pw.vr = nvr;
+        }
+        const float nur = pw.ur * u1r - pw.ui * u1i;
+        pw.ui = pw.ur * u1i + pw.ui * u1r;
+        pw.ur = nur;
+        float a, ad, b, bd;
+        poly_m<M, O::poly_terms(M)>(q + O::a_offset(M), z, w, a, ad);
+        poly_m<M, O::poly_terms(M)>(q + O::b_offset(M), z, w, b, bd);
+        r.value += a * pw.ur + b * pw.ui;
+        r.dtheta += (float)M * z * (a * pw.vr + b * pw.vi) - s * (ad * pw.ur + bd * pw.ui);
+        r.dphi += (float)M * (b * pw.vr - a * pw.vi);
+        sh_order_terms<L, M + 1>(q, z, w, s, u1r, u1i, pw, r);
+    }
+}
+
+template <int L>
+__device__ __forceinline__ Raw sh_evaluate(const float *q, const Direction &dir)
+{
+    using O = Order<L>;
     const float z = dir.z, s = dir.s, w = z * z;
     Raw r;
-    float a, ad, b, bd;
-    poly_m<0>(q + a_offset(0), z, w, a, ad);
+    float a, ad;
+    poly_m<0, O::poly_terms(0)>(q + O::a_offset(0), z, w, a, ad);
     r.value = a;
     r.dtheta = -s * ad;
     r.dphi = 0.0f;
-    const float u1r = s * dir.c1, u1i = s * dir.s1;
-    float ur = 1.0f, ui = 0.0f;         // u^(m-1) -> u^m
-    float vr = dir.c1, vi = dir.s1;     // v_m = u^(m-1) e^{i phi}
-#define TGK_FAST_M(M)                                                                \
-    {                                                                                \
-        if (M > 1) {                                                                 \
-            const float nvr = vr * u1r - vi * u1i;                                  \
-            vi = vr * u1i + vi * u1r;                                                \
-            vr = nvr;                                                                \
-        }                                                                            \
-        const float nur = ur * u1r - ui * u1i;                                      \
-        ui = ur * u1i + ui * u1r;                                                    \
-        ur = nur;                                                                    \
-        poly_m<M>(q + a_offset(M), z, w, a, ad);                                     \
-        poly_m<M>(q + b_offset(M), z, w, b, bd);                                     \
-        r.value += a * ur + b * ui;                                                  \
-        r.dtheta += (float)M * z * (a * vr + b * vi) - s * (ad * ur + bd * ui);     \
-        r.dphi += (float)M * (b * vr - a * vi);                                     \
-    }
-    TGK_FAST_M(1)
-    TGK_FAST_M(2)
-    TGK_FAST_M(3)
-    TGK_FAST_M(4)
-    TGK_FAST_M(5)
-    TGK_FAST_M(6)
-    TGK_FAST_M(7)
-    TGK_FAST_M(8)
-#undef TGK_FAST_M
+    Powers pw{1.0f, 0.0f, dir.c1, dir.s1};
+    sh_order_terms<L, 1>(q, z, w, s, s * dir.c1, s * dir.s1, pw, r);
     return r;
 }
 
@@ -161,9 +168,10 @@ struct ModelOut {
     float value, dtheta, dphi;
 };
 
-__device__ __forceinline__ ModelOut sh_model(const float q[SH_TERMS], const Direction &dir)
+template <int L>
+__device__ __forceinline__ ModelOut sh_model(const float *q, const Direction &dir)
 {
-    const Raw raw = sh8_evaluate(q, dir);
+    const Raw raw = sh_evaluate<L>(q, dir);
     ModelOut m;
     float d;
     softplus(raw.value, m.value, d);

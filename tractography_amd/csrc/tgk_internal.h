@@ -22,16 +22,18 @@ struct FastLaunch {
     int32_t precision;      // TGK_PRECISION_*
     const void *coeffs;     // transformed field (layout per precision)
     uint32_t coeff_stride;  // elements per voxel in `coeffs`
+    int32_t sh_order;       // 8 (K <= 45) or 12 (K <= 91): polynomial layout of `coeffs`
 };
 // grid_blocks <= 0 sizes the persistent grid to the device's occupancy.
 hipError_t launch_track_fast(const FastLaunch &p, int grid_blocks, hipStream_t stream);
 hipError_t launch_compact_seed_major(const float *dense, int64_t n_seeds, uint32_t n_points,
                                      const uint32_t *lengths, const int64_t *offsets, float *out,
                                      hipStream_t stream);
-// Per-voxel coefficients the fast path reads: the SH -> polynomial map for SH
-// fields (out_stride 48), a padded copy for DTI (out_stride 8).
-hipError_t launch_field_transform(const float *field, int64_t n_voxels, uint32_t stride, int32_t model, void *out,
-                                  uint32_t out_stride, hipStream_t stream);
+// Per-voxel coefficients the fast path reads: the SH -> polynomial map of
+// order sh_order (8: out_stride >= 45, 12: out_stride >= 91) for SH fields of
+// K coefficients, a padded copy for DTI (out_stride 8).
+hipError_t launch_field_transform(const float *field, int64_t n_voxels, uint32_t stride, int32_t model, int32_t K,
+                                  int32_t sh_order, void *out, uint32_t out_stride, hipStream_t stream);
 
 // One-op drivers (tgk_plain.hip).
 hipError_t launch_evaluate(int model, int K, const float *params, const float *orientations, int64_t n, float *out,
