@@ -167,6 +167,35 @@ int tgk_domain_seed(tgk_runtime *rt, const tgk_domain *domain, uint32_t rng_seed
 int tgk_seeds_from_fod(tgk_runtime *rt, int32_t model, int32_t n_parameters, const float *fod, const float *voxels,
                        uint32_t nnz, const float affine[16], uint32_t *states, int64_t n, int32_t method, float *out);
 
+/* Device-pointer form of tgk_seeds_from_fod, enqueued on the runtime stream
+ * without synchronising: d_fod (nnz, K), d_voxels (nnz, 4), d_affine 4x4,
+ * d_states (n, 2) advanced in place, d_out (n, 8). The seeding table for
+ * (model, K, method) is built on first use or by tgk_seeds_prepare. */
+int tgk_seeds_from_fod_device(tgk_runtime *rt, int32_t model, int32_t n_parameters, const float *d_fod,
+                              const float *d_voxels, uint32_t nnz, const float *d_affine, uint32_t *d_states,
+                              int64_t n, int32_t method, float *d_out);
+int tgk_seeds_prepare(tgk_runtime *rt, int32_t model, int32_t n_parameters, int32_t method);
+
+/* ------------------------------------------------------------ HIP graphs --- */
+
+/* No counterpart in the reference (its OpenCL queue replays nothing): for
+ * callers that run the same device pipeline many times, e.g. seeding,
+ * tracking and compaction over chunks of a large seed set. Between
+ * tgk_graph_begin and tgk_graph_end the runtime's stream is recorded, not
+ * run; only the *_device entry points may be called, and none of them may
+ * allocate: tgk_track_prepare (field transform, work queue, scan workspace
+ * for up to max_seeds seeds, kernel occupancy) and tgk_seeds_prepare must
+ * come first, or the call fails with TGK_ERR_INVALID. A graph holds the
+ * device pointers it was recorded with; tgk_graph_launch replays it on the
+ * runtime's stream. */
+typedef struct tgk_graph tgk_graph;
+int tgk_track_prepare(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field,
+                      const tgk_algorithm *algorithm, int32_t path, int64_t max_seeds);
+int tgk_graph_begin(tgk_runtime *rt);
+int tgk_graph_end(tgk_runtime *rt, tgk_graph **out);
+int tgk_graph_launch(tgk_graph *graph);
+int tgk_graph_destroy(tgk_graph *graph);
+
 /* ------------------------------------------------------ one-op drivers --- */
 
 /* Replaces local.evaluate (local.py:285-362): params (n, K), orientations

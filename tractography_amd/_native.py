@@ -81,6 +81,13 @@ _SIGNATURES = {
     "tgk_compact_device": (c_int, [_P, c_int32, _P, c_int64, c_uint32, _P, _P, _P]),
     "tgk_domain_seed": (c_int, [_P, _P, c_uint32, c_int64, _P]),
     "tgk_seeds_from_fod": (c_int, [_P, c_int32, c_int32, _P, _P, c_uint32, _P, _P, c_int64, c_int32, _P]),
+    "tgk_seeds_prepare": (c_int, [_P, c_int32, c_int32, c_int32]),
+    "tgk_seeds_from_fod_device": (c_int, [_P, c_int32, c_int32, _P, _P, c_uint32, _P, _P, c_int64, c_int32, _P]),
+    "tgk_track_prepare": (c_int, [_P, _P, _P, POINTER(tgk_algorithm), c_int32, c_int64]),
+    "tgk_graph_begin": (c_int, [_P]),
+    "tgk_graph_end": (c_int, [_P, POINTER(c_void_p)]),
+    "tgk_graph_launch": (c_int, [_P]),
+    "tgk_graph_destroy": (c_int, [_P]),
     "tgk_evaluate": (c_int, [_P, c_int32, c_int32, _P, _P, c_int64, _P]),
     "tgk_update_orientation": (c_int, [_P, POINTER(tgk_algorithm), c_uint32, _P, _P, c_int64, _P]),
     "tgk_field_evaluate": (c_int, [_P, _P, _P, c_int64, _P]),

@@ -46,6 +46,7 @@ SOURCES = [
     ("tgk_plain.hip", "off"),
     ("tgk_capi.hip", "off"),
     ("tgk_seeds.hip", "off"),
+    ("tgk_scan.hip", "off"),
     ("tgk_fast.hip", "fast"),
 ]
 

@@ -2,7 +2,6 @@
 // include/tgk.h): runtime and buffer ownership, argument validation, chunked
 // host-array tracking and the one-op drivers.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -83,7 +82,16 @@ struct tgk_runtime {
     int32_t device = 0;
     hipStream_t stream = nullptr;
     DeviceBuffer workspace;
-    DeviceBuffer queue;  // work counter of the persistent fast kernel
+    DeviceBuffer queue;        // work counter of the persistent fast kernel
+    DeviceBuffer seeds_table;  // prepared FOD-seeding table for seeds_key
+    int32_t seeds_key[3] = {-1, -1, -1};  // model, K, method
+    bool capturing = false;    // the stream is being recorded into a graph
+};
+
+struct tgk_graph {
+    tgk_runtime *rt = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 struct tgk_field {
@@ -215,52 +223,25 @@ int select_device(tgk_runtime *rt)
     return TGK_OK;
 }
 
-int ensure_workspace(tgk_runtime *rt, size_t bytes)
+// Grows a runtime-owned buffer to `bytes`; while the stream is being
+// recorded into a graph no allocation may happen, so growth is an error
+// (the caller should have prepared the buffer before recording).
+int reserve(tgk_runtime *rt, DeviceBuffer &buf, size_t bytes, const char *what)
 {
-    TGK_HIP(rt->workspace.allocate(bytes), "workspace allocation");
+    if (bytes == 0 || (buf.ptr && bytes <= buf.bytes))
+        return TGK_OK;
+    if (rt->capturing)
+        return fail(TGK_ERR_INVALID, "%s needs %zu bytes while a graph is being recorded; prepare it first", what,
+                    bytes);
+    TGK_HIP(buf.allocate(bytes), what);
     return TGK_OK;
 }
 
+int ensure_workspace(tgk_runtime *rt, size_t bytes) { return reserve(rt, rt->workspace, bytes, "workspace"); }
+
 }  // namespace
 
-namespace tgk {
 
-// Workspace: widened lengths (n + 1 int64, 256-byte aligned) followed by
-// the scan's temporary storage.
-size_t offsets_workspace_bytes(int64_t n)
-{
-    size_t bytes = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const int64_t *)nullptr, (int64_t *)nullptr, (int)(n + 1));
-    const size_t widened = (((size_t)(n + 1) * sizeof(int64_t)) + 255) & ~(size_t)255;
-    return widened + bytes;
-}
-
-namespace {
-__global__ void widen_lengths(const uint32_t *__restrict__ lengths, int64_t n, int64_t *__restrict__ out)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i <= n)
-        out[i] = i < n ? (int64_t)lengths[i] : 0;
-}
-}  // namespace
-
-hipError_t launch_offsets(const uint32_t *lengths, int64_t n, int64_t *offsets, void *workspace,
-                          size_t workspace_bytes, hipStream_t stream)
-{
-    const size_t widened_bytes = (((size_t)(n + 1) * sizeof(int64_t)) + 255) & ~(size_t)255;
-    if (workspace_bytes < widened_bytes)
-        return hipErrorInvalidValue;
-    int64_t *widened = static_cast<int64_t *>(workspace);
-    widen_lengths<<<(unsigned)((n + 1 + 255) / 256), 256, 0, stream>>>(lengths, n, widened);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess)
-        return e;
-    size_t bytes = workspace_bytes - widened_bytes;
-    return hipcub::DeviceScan::ExclusiveSum(static_cast<char *>(workspace) + widened_bytes, bytes,
-                                            (const int64_t *)widened, offsets, (int)(n + 1), stream);
-}
-
-}  // namespace tgk
 
 extern "C" {
 
@@ -563,6 +544,48 @@ tgk::TrackLaunch make_launch(const tgk_domain *domain, const tgk_field *field, c
     return p;
 }
 
+// The field as the fast path reads it: transformed once per model (the
+// SH -> polynomial map of order 8 or 12, or padded DTI rows) and cached on
+// the field.
+int ensure_fast_field(tgk_runtime *rt, tgk_field *field, const tgk_algorithm *algorithm, int32_t *sh_order)
+{
+    if (field->fast_precision != TGK_PRECISION_FP32)
+        return fail(TGK_ERR_UNSUPPORTED, "fast-path coefficient storage %d is not available", field->fast_precision);
+    const int32_t model = algorithm->model;
+    *sh_order = algorithm->n_parameters > 45 ? 12 : 8;
+    if (field->fast_cached != model) {
+        if (rt->capturing)
+            return fail(TGK_ERR_INVALID, "the field's fast-path transform must be prepared before recording a graph");
+        const uint32_t out_stride = model == TGK_MODEL_SH ? (*sh_order == 12 ? 92u : 48u) : 8u;
+        const int64_t n_voxels = (int64_t)field->dims[0] * field->dims[1] * field->dims[2];
+        TGK_HIP(field->fast.allocate((size_t)n_voxels * out_stride * sizeof(float)), "fast field allocation");
+        TGK_HIP(tgk::launch_field_transform(field->data.as<const float>(), n_voxels, field->stride, model,
+                                            (int32_t)field->dims[3], *sh_order, field->fast.ptr, out_stride,
+                                            rt->stream),
+                "field transform");
+        field->fast_cached = model;
+        field->fast_stride = out_stride;
+    }
+    return TGK_OK;
+}
+
+int fast_launch(tgk_runtime *rt, tgk_field *field, const tgk_algorithm *algorithm, const tgk::TrackLaunch &p,
+                tgk::FastLaunch *f)
+{
+    int32_t sh_order = 8;
+    if (int r = ensure_fast_field(rt, field, algorithm, &sh_order))
+        return r;
+    if (int r = reserve(rt, rt->queue, 256, "queue"))
+        return r;
+    f->track = p;
+    f->queue = rt->queue.as<uint32_t>();
+    f->precision = field->fast_precision;
+    f->coeffs = field->fast.ptr;
+    f->coeff_stride = field->fast_stride;
+    f->sh_order = sh_order;
+    return TGK_OK;
+}
+
 int run_track(tgk_runtime *rt, const tgk_domain *domain, tgk_field *field, const tgk_algorithm *algorithm,
               int32_t path, const float *d_seeds, int64_t first_seed, int64_t n_seeds, const uint32_t *d_rng_seeds,
               int64_t batch_size, uint32_t n_points, float save_at, uint32_t *d_lengths, uint32_t *d_steps,
@@ -755,29 +778,9 @@ int run_track(tgk_runtime *rt, const tgk_domain *domain, tgk_field *field, const
         TGK_HIP(tgk::launch_track_plain(p, rt->stream), "plain tracking kernel");
         return TGK_OK;
     }
-    if (field->fast_precision != TGK_PRECISION_FP32)
-        return fail(TGK_ERR_UNSUPPORTED, "fast-path coefficient storage %d is not available", field->fast_precision);
-    const int32_t model = algorithm->model;
-    const int32_t sh_order = algorithm->n_parameters > 45 ? 12 : 8;
-    if (field->fast_cached != model) {
-        const uint32_t out_stride = model == TGK_MODEL_SH ? (sh_order == 12 ? 92u : 48u) : 8u;
-        const int64_t n_voxels = (int64_t)field->dims[0] * field->dims[1] * field->dims[2];
-        TGK_HIP(field->fast.allocate((size_t)n_voxels * out_stride * sizeof(float)), "fast field allocation");
-        TGK_HIP(tgk::launch_field_transform(field->data.as<const float>(), n_voxels, field->stride, model,
-                                            (int32_t)field->dims[3], sh_order, field->fast.ptr, out_stride,
-                                            rt->stream),
-                "field transform");
-        field->fast_cached = model;
-        field->fast_stride = out_stride;
-    }
-    TGK_HIP(rt->queue.allocate(256), "queue allocation");
     tgk::FastLaunch f;
-    f.track = p;
-    f.queue = rt->queue.as<uint32_t>();
-    f.precision = field->fast_precision;
-    f.coeffs = field->fast.ptr;
-    f.coeff_stride = field->fast_stride;
-    f.sh_order = sh_order;
+    if (int r = fast_launch(rt, field, algorithm, p, &f))
+        return r;
     int grid = 0;
     if (const char *env = getenv("TGK_FAST_GRID"))
         grid = atoi(env);
@@ -982,11 +985,12 @@ int tgk_domain_seed(tgk_runtime *rt, const tgk_domain *domain, uint32_t rng_seed
     return fetch(out, dout, (size_t)n * 8, rt->stream);
 }
 
-int tgk_seeds_from_fod(tgk_runtime *rt, int32_t model, int32_t n_parameters, const float *fod, const float *voxels,
-                       uint32_t nnz, const float affine[16], uint32_t *states, int64_t n, int32_t method, float *out)
+}  // extern "C"
+
+namespace {
+
+int check_seeds_args(int32_t model, int32_t n_parameters, int32_t method)
 {
-    if (!rt || n < 0 || (n > 0 && (!fod || !voxels || !affine || !states || !out)))
-        return fail(TGK_ERR_INVALID, "bad arguments");
     if (method != TGK_SEEDS_VALU && method != TGK_SEEDS_MFMA)
         return fail(TGK_ERR_INVALID, "unknown seeding method %d", method);
     if (method == TGK_SEEDS_MFMA && model == TGK_MODEL_SH && n_parameters > 45) {
@@ -994,16 +998,49 @@ int tgk_seeds_from_fod(tgk_runtime *rt, int32_t model, int32_t n_parameters, con
         // up to 96 coefficients.
         if (n_parameters > 96)
             return fail(TGK_ERR_UNSUPPORTED, "SH with %d coefficients: 1..96 supported", n_parameters);
-    } else {
-        tgk_algorithm a = {model, TGK_ALGO_TRANSPORT, n_parameters, 1.0f, 1.0f, 0.0f};
-        if (int r = check_algorithm(&a, nullptr))
-            return r;
+        return TGK_OK;
     }
+    tgk_algorithm a = {model, TGK_ALGO_TRANSPORT, n_parameters, 1.0f, 1.0f, 0.0f};
+    return check_algorithm(&a, nullptr);
+}
+
+// The runtime's seeding table for (model, K, method), built on first use.
+int prepare_seeds(tgk_runtime *rt, int32_t model, int32_t n_parameters, int32_t method)
+{
+    if (rt->seeds_key[0] == model && rt->seeds_key[1] == n_parameters && rt->seeds_key[2] == method)
+        return TGK_OK;
+    if (rt->capturing)
+        return fail(TGK_ERR_INVALID, "the seeding table must be prepared before recording a graph");
+    const size_t bytes = tgk::seeds_from_fod_workspace_bytes(model, n_parameters, 0, method);
+    if (int r = reserve(rt, rt->seeds_table, bytes, "seeding table"))
+        return r;
+    rt->seeds_key[0] = -1;
+    TGK_HIP(tgk::prepare_seeds_from_fod(model, n_parameters, method, rt->seeds_table.ptr, bytes, rt->stream),
+            "seeding table");
+    rt->seeds_key[0] = model;
+    rt->seeds_key[1] = n_parameters;
+    rt->seeds_key[2] = method;
+    return TGK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tgk_seeds_from_fod(tgk_runtime *rt, int32_t model, int32_t n_parameters, const float *fod, const float *voxels,
+                       uint32_t nnz, const float affine[16], uint32_t *states, int64_t n, int32_t method, float *out)
+{
+    if (!rt || n < 0 || (n > 0 && (!fod || !voxels || !affine || !states || !out)))
+        return fail(TGK_ERR_INVALID, "bad arguments");
+    if (int r = check_seeds_args(model, n_parameters, method))
+        return r;
     if (n > 0 && nnz == 0)
         return fail(TGK_ERR_INVALID, "the FOD has no nonzero voxel");
     if (n == 0)
         return TGK_OK;
     if (int r = select_device(rt))
+        return r;
+    if (int r = prepare_seeds(rt, model, n_parameters, method))
         return r;
     Staging st;
     float *dfod, *dvox, *daff, *dout;
@@ -1018,15 +1055,134 @@ int tgk_seeds_from_fod(tgk_runtime *rt, int32_t model, int32_t n_parameters, con
         return r;
     if (int r = st.out((size_t)n * 8, &dout))
         return r;
-    const size_t ws = tgk::seeds_from_fod_workspace_bytes(model, n_parameters, n, method);
-    if (int r = ensure_workspace(rt, ws))
-        return r;
-    TGK_HIP(tgk::launch_seeds_from_fod(model, n_parameters, dfod, dvox, nnz, daff, dstates, n, method, dout,
-                                       rt->workspace.ptr, ws, rt->stream),
+    TGK_HIP(tgk::launch_seeds_from_fod_prepared(model, n_parameters, dfod, dvox, nnz, daff, dstates, n, method, dout,
+                                                rt->seeds_table.ptr, rt->stream),
             "FOD seeding kernel");
     TGK_HIP(hipMemcpyAsync(states, dstates, (size_t)n * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, rt->stream),
             "state copy");
     return fetch(out, dout, (size_t)n * 8, rt->stream);
+}
+
+int tgk_seeds_prepare(tgk_runtime *rt, int32_t model, int32_t n_parameters, int32_t method)
+{
+    if (!rt)
+        return fail(TGK_ERR_INVALID, "runtime is NULL");
+    if (int r = check_seeds_args(model, n_parameters, method))
+        return r;
+    if (int r = select_device(rt))
+        return r;
+    if (int r = prepare_seeds(rt, model, n_parameters, method))
+        return r;
+    TGK_HIP(hipStreamSynchronize(rt->stream), "seeding table");
+    return TGK_OK;
+}
+
+int tgk_seeds_from_fod_device(tgk_runtime *rt, int32_t model, int32_t n_parameters, const float *d_fod,
+                              const float *d_voxels, uint32_t nnz, const float *d_affine, uint32_t *d_states,
+                              int64_t n, int32_t method, float *d_out)
+{
+    if (!rt || n < 0 || (n > 0 && (!d_fod || !d_voxels || !d_affine || !d_states || !d_out)))
+        return fail(TGK_ERR_INVALID, "bad arguments");
+    if (int r = check_seeds_args(model, n_parameters, method))
+        return r;
+    if (n > 0 && nnz == 0)
+        return fail(TGK_ERR_INVALID, "the FOD has no nonzero voxel");
+    if (n == 0)
+        return TGK_OK;
+    if (int r = select_device(rt))
+        return r;
+    if (int r = prepare_seeds(rt, model, n_parameters, method))
+        return r;
+    TGK_HIP(tgk::launch_seeds_from_fod_prepared(model, n_parameters, d_fod, d_voxels, nnz, d_affine, d_states, n,
+                                                method, d_out, rt->seeds_table.ptr, rt->stream),
+            "FOD seeding kernel");
+    return TGK_OK;
+}
+
+int tgk_track_prepare(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field,
+                      const tgk_algorithm *algorithm, int32_t path, int64_t max_seeds)
+{
+    if (!rt)
+        return fail(TGK_ERR_INVALID, "runtime is NULL");
+    if (rt->capturing)
+        return fail(TGK_ERR_INVALID, "prepare before recording a graph");
+    if (int r = check_track_args(domain, field, algorithm, path, max_seeds, 1, 1, 1.0f))
+        return r;
+    if (int r = select_device(rt))
+        return r;
+    if (int r = ensure_workspace(rt, tgk::offsets_workspace_bytes(max_seeds)))
+        return r;
+    if (path == TGK_PATH_FAST) {
+        const tgk::TrackLaunch p = make_launch(domain, field, algorithm, nullptr, 0, 0, nullptr, 1, 1, 1.0f, nullptr,
+                                               nullptr, nullptr, nullptr);
+        tgk::FastLaunch f;
+        if (int r = fast_launch(rt, const_cast<tgk_field *>(field), algorithm, p, &f))
+            return r;
+        TGK_HIP(tgk::prepare_track_fast(f), "fast kernel occupancy");
+    }
+    TGK_HIP(hipStreamSynchronize(rt->stream), "track preparation");
+    return TGK_OK;
+}
+
+int tgk_graph_begin(tgk_runtime *rt)
+{
+    if (!rt)
+        return fail(TGK_ERR_INVALID, "runtime is NULL");
+    if (rt->capturing)
+        return fail(TGK_ERR_INVALID, "a graph is already being recorded on this runtime");
+    if (int r = select_device(rt))
+        return r;
+    TGK_HIP(hipStreamBeginCapture(rt->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    rt->capturing = true;
+    return TGK_OK;
+}
+
+int tgk_graph_end(tgk_runtime *rt, tgk_graph **out)
+{
+    if (!rt || !out)
+        return fail(TGK_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    if (!rt->capturing)
+        return fail(TGK_ERR_INVALID, "no graph is being recorded on this runtime");
+    rt->capturing = false;
+    hipGraph_t graph = nullptr;
+    TGK_HIP(hipStreamEndCapture(rt->stream, &graph), "hipStreamEndCapture");
+    std::unique_ptr<tgk_graph> g(new (std::nothrow) tgk_graph());
+    if (!g) {
+        (void)hipGraphDestroy(graph);
+        return fail(TGK_ERR_NO_MEMORY, "graph allocation");
+    }
+    g->rt = rt;
+    g->graph = graph;
+    const hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    *out = g.release();
+    return TGK_OK;
+}
+
+int tgk_graph_launch(tgk_graph *graph)
+{
+    if (!graph)
+        return fail(TGK_ERR_INVALID, "graph is NULL");
+    if (int r = select_device(graph->rt))
+        return r;
+    TGK_HIP(hipGraphLaunch(graph->exec, graph->rt->stream), "hipGraphLaunch");
+    return TGK_OK;
+}
+
+int tgk_graph_destroy(tgk_graph *graph)
+{
+    if (!graph)
+        return TGK_OK;
+    if (graph->exec)
+        (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph)
+        (void)hipGraphDestroy(graph->graph);
+    delete graph;
+    return TGK_OK;
 }
 
 }  // extern "C"

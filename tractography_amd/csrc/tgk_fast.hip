@@ -215,12 +215,16 @@ __global__ __launch_bounds__(256) void compact_seed_major_kernel(const float *__
 {
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t capacity = n_seeds * (int64_t)n_points * 3;
     for (int64_t g = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < n_seeds; g += waves) {
-        const int64_t count = (int64_t)lengths[g] * 3;
+        // Lengths are clamped to the row and writes to the output capacity,
+        // whatever the caller's lengths and offsets hold.
+        const int64_t count = (int64_t)min(lengths[g], n_points) * 3;
         const float *src = dense + g * (int64_t)n_points * 3;
-        float *dst = out + offsets[g] * 3;
+        const int64_t first = offsets[g] * 3;
         for (int64_t i = lane; i < count; i += 64)
-            dst[i] = src[i];
+            if (first + i >= 0 && first + i < capacity)
+                out[first + i] = src[i];
     }
 }
 
@@ -323,22 +327,40 @@ template <int L> hipError_t device_matrix(const double **out)
     return hipSuccess;
 }
 
-template <int MODEL, int ALGO, int DOMAIN, int LMAX>
-hipError_t launch_fast(const FastLaunch &f, int grid_blocks, hipStream_t stream)
+template <int MODEL, int ALGO, int DOMAIN, int LMAX> hipError_t occupancy_grid(int *blocks)
 {
-    if (grid_blocks <= 0) {
+    static int cache[64] = {};
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess)
+        return e;
+    if (device < 0 || device >= 64)
+        return hipErrorInvalidDevice;
+    if (cache[device] == 0) {
         int per_cu = 0;
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, reinterpret_cast<const void *>(track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX>), 256, 0);
         if (e != hipSuccess)
             return e;
-        int device = 0;
-        hipGetDevice(&device);
         hipDeviceProp_t prop;
         e = hipGetDeviceProperties(&prop, device);
         if (e != hipSuccess)
             return e;
-        grid_blocks = prop.multiProcessorCount * (per_cu > 0 ? per_cu : 1);
+        cache[device] = prop.multiProcessorCount * (per_cu > 0 ? per_cu : 1);
+    }
+    *blocks = cache[device];
+    return hipSuccess;
+}
+
+template <int MODEL, int ALGO, int DOMAIN, int LMAX>
+hipError_t launch_fast(const FastLaunch &f, int grid_blocks, bool query, hipStream_t stream)
+{
+    if (grid_blocks <= 0 || query) {
+        int blocks = 0;
+        const hipError_t e = occupancy_grid<MODEL, ALGO, DOMAIN, LMAX>(&blocks);
+        if (e != hipSuccess || query)
+            return e;
+        grid_blocks = blocks;
     }
     zero_counter<<<1, 1, 0, stream>>>(f.queue);
     track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX><<<grid_blocks, 256, 0, stream>>>(f);
@@ -346,29 +368,36 @@ hipError_t launch_fast(const FastLaunch &f, int grid_blocks, hipStream_t stream)
 }
 
 template <int MODEL, int ALGO, int DOMAIN>
-hipError_t launch_fast_order(const FastLaunch &f, int grid, hipStream_t stream)
+hipError_t launch_fast_order(const FastLaunch &f, int grid, bool query, hipStream_t stream)
 {
     if constexpr (MODEL == MODEL_SH) {
         if (f.sh_order == 12)
-            return launch_fast<MODEL, ALGO, DOMAIN, 12>(f, grid, stream);
+            return launch_fast<MODEL, ALGO, DOMAIN, 12>(f, grid, query, stream);
     }
-    return launch_fast<MODEL, ALGO, DOMAIN, 8>(f, grid, stream);
+    return launch_fast<MODEL, ALGO, DOMAIN, 8>(f, grid, query, stream);
 }
 
 template <int MODEL, int ALGO>
-hipError_t launch_fast_domain(const FastLaunch &f, int grid, hipStream_t stream)
+hipError_t launch_fast_domain(const FastLaunch &f, int grid, bool query, hipStream_t stream)
 {
     if (f.track.domain_kind == DOMAIN_MESH)
-        return launch_fast_order<MODEL, ALGO, DOMAIN_MESH>(f, grid, stream);
-    return launch_fast_order<MODEL, ALGO, DOMAIN_MASK>(f, grid, stream);
+        return launch_fast_order<MODEL, ALGO, DOMAIN_MESH>(f, grid, query, stream);
+    return launch_fast_order<MODEL, ALGO, DOMAIN_MASK>(f, grid, query, stream);
 }
 
 template <int MODEL>
-hipError_t launch_fast_algo(const FastLaunch &f, int grid, hipStream_t stream)
+hipError_t launch_fast_algo(const FastLaunch &f, int grid, bool query, hipStream_t stream)
 {
     if (f.track.algo.algorithm == ALGO_DIFFUSION)
-        return launch_fast_domain<MODEL, ALGO_DIFFUSION>(f, grid, stream);
-    return launch_fast_domain<MODEL, ALGO_TRANSPORT>(f, grid, stream);
+        return launch_fast_domain<MODEL, ALGO_DIFFUSION>(f, grid, query, stream);
+    return launch_fast_domain<MODEL, ALGO_TRANSPORT>(f, grid, query, stream);
+}
+
+hipError_t dispatch_fast(const FastLaunch &f, int grid, bool query, hipStream_t stream)
+{
+    if (f.track.algo.model == MODEL_DTI)
+        return launch_fast_algo<MODEL_DTI>(f, grid, query, stream);
+    return launch_fast_algo<MODEL_SH>(f, grid, query, stream);
 }
 
 }  // namespace
@@ -377,10 +406,10 @@ hipError_t launch_track_fast(const FastLaunch &f, int grid_blocks, hipStream_t s
 {
     if (f.track.n_seeds == 0)
         return hipSuccess;
-    if (f.track.algo.model == MODEL_DTI)
-        return launch_fast_algo<MODEL_DTI>(f, grid_blocks, stream);
-    return launch_fast_algo<MODEL_SH>(f, grid_blocks, stream);
+    return dispatch_fast(f, grid_blocks, false, stream);
 }
+
+hipError_t prepare_track_fast(const FastLaunch &f) { return dispatch_fast(f, 0, true, nullptr); }
 
 hipError_t launch_compact_seed_major(const float *dense, int64_t n_seeds, uint32_t n_points,
                                      const uint32_t *lengths, const int64_t *offsets, float *out,

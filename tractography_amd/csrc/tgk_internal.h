@@ -24,8 +24,13 @@ struct FastLaunch {
     uint32_t coeff_stride;  // elements per voxel in `coeffs`
     int32_t sh_order;       // 8 (K <= 45) or 12 (K <= 91): polynomial layout of `coeffs`
 };
-// grid_blocks <= 0 sizes the persistent grid to the device's occupancy.
+// grid_blocks <= 0 sizes the persistent grid to the device's occupancy
+// (computed once per kernel and device, then cached).
 hipError_t launch_track_fast(const FastLaunch &p, int grid_blocks, hipStream_t stream);
+// Computes and caches that occupancy for the kernel `p` selects, launching
+// nothing (so later launches make no device queries, e.g. while a graph is
+// being recorded).
+hipError_t prepare_track_fast(const FastLaunch &p);
 hipError_t launch_compact_seed_major(const float *dense, int64_t n_seeds, uint32_t n_points,
                                      const uint32_t *lengths, const int64_t *offsets, float *out,
                                      hipStream_t stream);
@@ -54,6 +59,14 @@ hipError_t launch_seeds_from_fod(int model, int K, const float *fod, const float
                                  const float *affine, uint32_t *states, int64_t n, int32_t method, float *out,
                                  void *workspace, size_t workspace_bytes, hipStream_t stream);
 size_t seeds_from_fod_workspace_bytes(int model, int K, int64_t n, int32_t method);
+// The two halves of launch_seeds_from_fod: the per-(model, K, method) table
+// (built and uploaded once; synchronises the stream for the MFMA table), and
+// the seeding kernel over a prepared table (enqueue only, graph-safe).
+hipError_t prepare_seeds_from_fod(int model, int K, int32_t method, void *table, size_t table_bytes,
+                                  hipStream_t stream);
+hipError_t launch_seeds_from_fod_prepared(int model, int K, const float *fod, const float *voxels, uint32_t nnz,
+                                          const float *affine, uint32_t *states, int64_t n, int32_t method,
+                                          float *out, const void *table, hipStream_t stream);
 
 // Exclusive scan of u32 lengths into i64 offsets (offsets[n] = total).
 hipError_t launch_offsets(const uint32_t *lengths, int64_t n, int64_t *offsets, void *workspace,

@@ -115,26 +115,65 @@ hipError_t launch_plain_algo(const TrackLaunch &p, hipStream_t stream)
     return launch_plain_domain<MODEL, ALGO_TRANSPORT>(p, stream);
 }
 
-// Dense point-major -> ragged (offsets[g] + n). Threads walk the dense array
-// in order so reads are coalesced.
+// Point-major dense [n][seed][3] -> ragged points in seed order. A block owns
+// 64 consecutive seeds and walks their points in chunks of 16 up to the
+// block's longest streamline: it reads a chunk as rows of 64 seeds (768
+// contiguous bytes per point index, skipping points past each length), turns
+// it around in LDS and writes each seed's run of up to 16 points (192 bytes)
+// contiguously. Lengths are clamped to n_points.
+constexpr int COMPACT_SEEDS = 64;
+constexpr int COMPACT_POINTS = 16;
+
 __global__ __launch_bounds__(256) void compact_point_major_kernel(const float *__restrict__ dense, int64_t n_seeds,
                                                                   uint32_t n_points,
                                                                   const uint32_t *__restrict__ lengths,
                                                                   const int64_t *__restrict__ offsets,
                                                                   float *__restrict__ out)
 {
-    const int64_t total = n_seeds * (int64_t)n_points;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t n = t / n_seeds;
-        const int64_t g = t - n * n_seeds;
-        if (n < (int64_t)lengths[g]) {
-            const float *src = dense + t * 3;
-            float *dst = out + (offsets[g] + n) * 3;
-            dst[0] = src[0];
-            dst[1] = src[1];
-            dst[2] = src[2];
+    __shared__ float tile[COMPACT_SEEDS][COMPACT_POINTS * 3 + 1];
+    __shared__ uint32_t s_len[COMPACT_SEEDS];
+    __shared__ int64_t s_off[COMPACT_SEEDS];
+    __shared__ uint32_t s_longest;
+    const int64_t g0 = (int64_t)blockIdx.x * COMPACT_SEEDS;
+    const int t = threadIdx.x;
+    if (t == 0)
+        s_longest = 0;
+    __syncthreads();
+    if (t < COMPACT_SEEDS) {
+        const int64_t g = g0 + t;
+        const uint32_t len = g < n_seeds ? min(lengths[g], n_points) : 0u;
+        s_len[t] = len;
+        s_off[t] = g < n_seeds ? offsets[g] : 0;
+        atomicMax(&s_longest, len);
+    }
+    __syncthreads();
+    const uint32_t longest = s_longest;
+    const int width = (int)(n_seeds - g0 < COMPACT_SEEDS ? n_seeds - g0 : COMPACT_SEEDS);
+    const int row = width * 3;
+    for (uint32_t n0 = 0; n0 < longest; n0 += COMPACT_POINTS) {
+        for (int e = t; e < COMPACT_POINTS * row; e += 256) {
+            const int pn = e / row;
+            const int rem = e - pn * row;
+            const int gl = rem / 3;
+            const uint32_t n = n0 + (uint32_t)pn;
+            if (n < s_len[gl])
+                tile[gl][pn * 3 + (rem - gl * 3)] = dense[((int64_t)n * n_seeds + g0) * 3 + rem];
         }
+        __syncthreads();
+        const int gl = t >> 2, quarter = t & 3;
+        if (gl < width) {
+            const uint32_t len = s_len[gl];
+            const int valid = len > n0 ? (int)min(len - n0, (uint32_t)COMPACT_POINTS) * 3 : 0;
+            // Offsets come from the caller's lengths: stay inside the
+            // n_seeds * n_points capacity whatever they hold.
+            const int64_t first = (s_off[gl] + (int64_t)n0) * 3;
+            const int64_t capacity = n_seeds * (int64_t)n_points * 3;
+            const int end = min(12 * quarter + 12, valid);
+            for (int i = 12 * quarter; i < end; i++)
+                if (first + i >= 0 && first + i < capacity)
+                    out[first + i] = tile[gl][i];
+        }
+        __syncthreads();
     }
 }
 
@@ -328,10 +367,9 @@ hipError_t launch_compact_point_major(const float *dense, int64_t n_seeds, uint3
 {
     if (n_seeds == 0)
         return hipSuccess;
-    const int64_t total = n_seeds * (int64_t)n_points;
-    const int64_t blocks = (total + 255) / 256;
-    compact_point_major_kernel<<<(unsigned)(blocks < 262144 ? blocks : 262144), 256, 0, stream>>>(
-        dense, n_seeds, n_points, lengths, offsets, out);
+    const int64_t blocks = (n_seeds + COMPACT_SEEDS - 1) / COMPACT_SEEDS;
+    compact_point_major_kernel<<<(unsigned)blocks, 256, 0, stream>>>(dense, n_seeds, n_points, lengths, offsets,
+                                                                    out);
     return hipGetLastError();
 }
 

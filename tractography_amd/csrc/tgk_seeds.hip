@@ -410,55 +410,77 @@ size_t seeds_from_fod_workspace_bytes(int model, int K, int64_t n, int32_t metho
     return (size_t)N_DIRECTIONS * TABLE_STRIDE * sizeof(float);
 }
 
+hipError_t prepare_seeds_from_fod(int model, int K, int32_t method, void *table, size_t table_bytes,
+                                  hipStream_t stream)
+{
+    if (table_bytes < seeds_from_fod_workspace_bytes(model, K, 0, method))
+        return hipErrorInvalidValue;
+    if (method == SEEDS_MFMA) {
+        if (K < 1 || mfma_ksteps(K) > 6 || (model == MODEL_DTI && K != 6))
+            return hipErrorInvalidValue;
+        std::vector<uint16_t> host_table;
+        std::vector<float> dirs;
+        build_mfma_table(model, K, host_table, dirs);
+        char *ws = static_cast<char *>(table);
+        hipError_t e = hipMemcpyAsync(ws, host_table.data(), host_table.size() * sizeof(uint16_t),
+                                      hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ws + mfma_table_bytes(K), dirs.data(), dirs.size() * sizeof(float),
+                               hipMemcpyHostToDevice, stream);
+        // The host tables must outlive the copies.
+        const hipError_t s = hipStreamSynchronize(stream);
+        return e != hipSuccess ? e : s;
+    }
+    if (K < 1 || K > 45 || (model == MODEL_DTI && K != 6))
+        return hipErrorInvalidValue;
+    basis_table_kernel<<<(N_DIRECTIONS + 255) / 256, 256, 0, stream>>>(static_cast<float *>(table));
+    return hipGetLastError();
+}
+
+hipError_t launch_seeds_from_fod_prepared(int model, int K, const float *fod, const float *voxels, uint32_t nnz,
+                                          const float *affine, uint32_t *states, int64_t n, int32_t method,
+                                          float *out, const void *table, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    if (nnz == 0)
+        return hipErrorInvalidValue;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (method == SEEDS_MFMA) {
+        const void *ddirs = static_cast<const char *>(table) + mfma_table_bytes(K);
+        if (model == MODEL_DTI)
+            launch_mfma<MODEL_DTI, 1>(fod, K, voxels, nnz, affine, table, ddirs, states, n, out, stream);
+        else if (mfma_ksteps(K) <= 1)
+            launch_mfma<MODEL_SH, 1>(fod, K, voxels, nnz, affine, table, ddirs, states, n, out, stream);
+        else if (mfma_ksteps(K) <= 3)
+            launch_mfma<MODEL_SH, 3>(fod, K, voxels, nnz, affine, table, ddirs, states, n, out, stream);
+        else
+            launch_mfma<MODEL_SH, 6>(fod, K, voxels, nnz, affine, table, ddirs, states, n, out, stream);
+        return hipGetLastError();
+    }
+    const float *t = static_cast<const float *>(table);
+    if (model == MODEL_DTI)
+        seeds_from_fod_kernel<MODEL_DTI><<<blocks, 256, 0, stream>>>(
+            fod, K, reinterpret_cast<const float4 *>(voxels), nnz, affine, t, reinterpret_cast<uint2 *>(states), n,
+            out);
+    else
+        seeds_from_fod_kernel<MODEL_SH><<<blocks, 256, 0, stream>>>(
+            fod, K, reinterpret_cast<const float4 *>(voxels), nnz, affine, t, reinterpret_cast<uint2 *>(states), n,
+            out);
+    return hipGetLastError();
+}
+
 hipError_t launch_seeds_from_fod(int model, int K, const float *fod, const float *voxels, uint32_t nnz,
                                  const float *affine, uint32_t *states, int64_t n, int32_t method, float *out,
                                  void *workspace, size_t workspace_bytes, hipStream_t stream)
 {
     if (n == 0)
         return hipSuccess;
-    if (nnz == 0 || workspace_bytes < seeds_from_fod_workspace_bytes(model, K, n, method))
-        return hipErrorInvalidValue;
-    if (method == SEEDS_MFMA) {
-        if (K < 1 || mfma_ksteps(K) > 6 || (model == MODEL_DTI && K != 6))
-            return hipErrorInvalidValue;
-        std::vector<uint16_t> table;
-        std::vector<float> dirs;
-        build_mfma_table(model, K, table, dirs);
-        char *ws = static_cast<char *>(workspace);
-        void *dtable = ws, *ddirs = ws + mfma_table_bytes(K);
-        hipError_t e = hipMemcpyAsync(dtable, table.data(), table.size() * sizeof(uint16_t), hipMemcpyHostToDevice,
-                                      stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(ddirs, dirs.data(), dirs.size() * sizeof(float), hipMemcpyHostToDevice, stream);
-        if (e != hipSuccess)
-            return e;
-        if (model == MODEL_DTI)
-            launch_mfma<MODEL_DTI, 1>(fod, K, voxels, nnz, affine, dtable, ddirs, states, n, out, stream);
-        else if (mfma_ksteps(K) <= 1)
-            launch_mfma<MODEL_SH, 1>(fod, K, voxels, nnz, affine, dtable, ddirs, states, n, out, stream);
-        else if (mfma_ksteps(K) <= 3)
-            launch_mfma<MODEL_SH, 3>(fod, K, voxels, nnz, affine, dtable, ddirs, states, n, out, stream);
-        else
-            launch_mfma<MODEL_SH, 6>(fod, K, voxels, nnz, affine, dtable, ddirs, states, n, out, stream);
-        e = hipGetLastError();
-        // The host tables must outlive the copies.
-        const hipError_t s = hipStreamSynchronize(stream);
-        return e != hipSuccess ? e : s;
-    }
-    if (K < 1 || K > 45)
-        return hipErrorInvalidValue;
-    float *table = static_cast<float *>(workspace);
-    basis_table_kernel<<<(N_DIRECTIONS + 255) / 256, 256, 0, stream>>>(table);
-    const unsigned blocks = (unsigned)((n + 255) / 256);
-    if (model == MODEL_DTI)
-        seeds_from_fod_kernel<MODEL_DTI><<<blocks, 256, 0, stream>>>(
-            fod, K, reinterpret_cast<const float4 *>(voxels), nnz, affine, table, reinterpret_cast<uint2 *>(states),
-            n, out);
-    else
-        seeds_from_fod_kernel<MODEL_SH><<<blocks, 256, 0, stream>>>(
-            fod, K, reinterpret_cast<const float4 *>(voxels), nnz, affine, table, reinterpret_cast<uint2 *>(states),
-            n, out);
-    return hipGetLastError();
+    hipError_t e = prepare_seeds_from_fod(model, K, method, workspace, workspace_bytes, stream);
+    if (e != hipSuccess)
+        return e;
+    return launch_seeds_from_fod_prepared(model, K, fod, voxels, nnz, affine, states, n, method, out, workspace,
+                                          stream);
 }
 
 }  // namespace tgk
