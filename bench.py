@@ -370,6 +370,29 @@ def brain145_shell_diffusion(seed: int = 0, n_seeds: int = 2_000_000, shape=SHAP
     return cfg
 
 
+SH91_CHUNKS = (300_000, 700_000, 1_100_000, 1_500_000, 2_500_000, 1_900_000, 1_200_000, 800_000)
+
+
+def brain145_sh91_fod_seeds(seed: int = 0, n_seeds: int = 10_000_000, shape=SHAPE_145):
+    """brain145_diffusion's fibres projected to SH order 12 (91
+    coefficients); the seeds are drawn on the device from the FOD
+    (seeds.from_fod, split-bfloat16 MFMA sampler) in 8 uneven chunks summing
+    to n_seeds. Diffusion gamma 1, noise variance 0.1, step 0.1, save 0.25,
+    max 300."""
+    rng = np.random.default_rng(seed)
+    directions = smooth_direction_fields(rng, shape)
+    _, weights = _random_fibres(rng, int(np.prod(shape)), base_directions=directions)
+    fod = fod_from_fibres(directions, weights, lmax=12).reshape(*shape, 91)
+    mask = ellipsoid_mask(shape, _semi_axes(shape, (60, 75, 60)))
+    fod[~mask] = 0
+    scale = n_seeds / sum(SH91_CHUNKS)
+    chunks = [max(1, int(round(c * scale))) for c in SH91_CHUNKS]
+    chunks[-1] = max(1, n_seeds - sum(chunks[:-1]))
+    return {"fod": fod, "mask": mask.astype(np.uint8), "seeds": None, "n_seeds": int(sum(chunks)),
+            "chunks": chunks, "algorithm": "diffusion", "step_size": 0.1, "gamma": 1.0, "noise_variance": 0.1,
+            "save_at": 0.25, "max_length": 300.0, "state_seed": seed + 2}
+
+
 # --------------------------------------------------------------------------
 # Measurement
 # --------------------------------------------------------------------------
@@ -482,6 +505,127 @@ class DeviceRun:
         return lengths, steps, points
 
 
+class GraphRun:
+    """FOD seeding on the device, fast tracking and compaction over uneven
+    chunks of seeds, recorded once into a HIP graph; one bench step is one
+    replay of the whole graph. Chunks run in order on one stream and share
+    one dense and one ragged point buffer sized for the largest chunk."""
+
+    def __init__(self, tg, torch, runtime, cfg, method: int = 1, batch_size: int = 50_000):
+        from tractography_amd import _native
+
+        self.tg, self.torch, self.runtime, self.cfg = tg, torch, runtime, cfg
+        self.native, self.lib = _native, _native.lib()
+        self.method = method
+        self.chunks = list(cfg["chunks"])
+        self.n = sum(self.chunks)
+        fod = cfg["fod"]
+        self.K = fod.shape[-1]
+        self.field = tg.field.Image(tg.Nifti1Image(fod, np.eye(4)))
+        self.domain = tg.domain.Image(tg.Nifti1Image(cfg["mask"], np.eye(4)))
+        self.algorithm = make_algorithm(tg, cfg)
+        self.n_points = tg.tracking.point_capacity(cfg["max_length"], cfg["save_at"])
+        inline, voxels, affine = tg.seeds.sparse_fod(tg.Nifti1Image(fod, np.eye(4)))
+        self.nnz = len(voxels)
+        dev = torch.device("cuda", runtime.device)
+        self.d_fod = torch.from_numpy(inline).to(dev)
+        self.d_voxels = torch.from_numpy(voxels).to(dev)
+        self.d_affine = torch.from_numpy(affine).to(dev)
+        del inline, voxels
+        states = np.random.default_rng(cfg["state_seed"]).integers(0, 2**32 - 1, (self.n, 2), dtype=np.uint32)
+        self.d_states = torch.from_numpy(states.view(np.int32)).to(dev)
+        self.d_seeds = torch.zeros((self.n, 8), dtype=torch.float32, device=dev)
+        n_batches = (self.n + batch_size - 1) // batch_size
+        rng_seeds = np.array([self.algorithm.get_rng_seed() for _ in range(n_batches)], dtype=np.uint32)
+        self.batch_size = batch_size
+        self.d_rng = torch.from_numpy(rng_seeds.view(np.int32)).to(dev)
+        self.d_lengths = torch.zeros(self.n, dtype=torch.int32, device=dev)
+        self.d_steps = torch.zeros(self.n, dtype=torch.int32, device=dev)
+        self.d_hits = torch.zeros(self.n, dtype=torch.int32, device=dev)
+        largest = max(self.chunks)
+        self.d_dense = torch.empty(largest * self.n_points * 3, dtype=torch.float32, device=dev)
+        self.d_offsets = torch.zeros(largest + 1, dtype=torch.int64, device=dev)
+        self.d_points = torch.empty(largest * self.n_points * 3, dtype=torch.float32, device=dev)
+        self.field_handle = self.field.upload(runtime)
+        self.domain_handle = self.domain.upload(runtime)
+        torch.cuda.synchronize()
+        self.native.check(self.lib.tgk_track_prepare(runtime.handle, self.domain_handle.handle,
+                                                     self.field_handle.handle, self.algorithm.descriptor(),
+                                                     _native.PATH_FAST, largest))
+        self.native.check(self.lib.tgk_seeds_prepare(runtime.handle, 0, self.K, method))
+        self.stream = torch.cuda.ExternalStream(runtime.stream(), device=dev)
+        self.graph = None
+
+    def enqueue(self, events=None):
+        """The chunk loop on the runtime stream; with `events`, a (start, stop)
+        pair is recorded around each tracking launch."""
+        check, lib, h = self.native.check, self.lib, self.runtime.handle
+        first = 0
+        for count in self.chunks:
+            seeds = self.d_seeds.data_ptr() + first * 32
+            check(lib.tgk_seeds_from_fod_device(h, 0, self.K, self.d_fod.data_ptr(), self.d_voxels.data_ptr(),
+                                                self.nnz, self.d_affine.data_ptr(),
+                                                self.d_states.data_ptr() + first * 8, count, self.method, seeds))
+            if events is not None:
+                events.append((self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True)))
+                events[-1][0].record(self.stream)
+            check(lib.tgk_track_device(h, self.domain_handle.handle, self.field_handle.handle,
+                                       self.algorithm.descriptor(), self.native.PATH_FAST, seeds, first, count,
+                                       self.d_rng.data_ptr(), self.batch_size, self.n_points,
+                                       float(np.float32(self.cfg["save_at"])), self.d_lengths.data_ptr() + first * 4,
+                                       self.d_steps.data_ptr() + first * 4, self.d_hits.data_ptr() + first * 4,
+                                       self.d_dense.data_ptr()))
+            if events is not None:
+                events[-1][1].record(self.stream)
+            check(lib.tgk_compact_device(h, self.native.PATH_FAST, self.d_dense.data_ptr(), count, self.n_points,
+                                         self.d_lengths.data_ptr() + first * 4, self.d_offsets.data_ptr(),
+                                         self.d_points.data_ptr()))
+            first += count
+
+    def record(self):
+        import ctypes
+
+        self.native.check(self.lib.tgk_graph_begin(self.runtime.handle))
+        try:
+            self.enqueue()
+        finally:
+            graph = ctypes.c_void_p()
+            status = self.lib.tgk_graph_end(self.runtime.handle, ctypes.byref(graph))
+        self.native.check(status)
+        self.graph = graph
+
+    def total_steps(self) -> int:
+        self.runtime.finish()
+        return int(self.d_steps.to(self.torch.int64).sum().item())
+
+    def timed(self, warmup: int, steps: int):
+        """(ms per graph replay, track-kernel ms per replay from an eager
+        pass, orientation updates per replay)."""
+        torch = self.torch
+        events = []
+        self.enqueue(events)  # eager pass: tracking kernel time per chunk
+        self.runtime.finish()
+        kernel = float(sum(a.elapsed_time(b) for a, b in events))
+        self.record()
+        for _ in range(warmup):
+            self.native.check(self.lib.tgk_graph_launch(self.graph))
+        self.runtime.finish()
+        times, updates = [], 0
+        for _ in range(steps):
+            start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record(self.stream)
+            self.native.check(self.lib.tgk_graph_launch(self.graph))
+            stop.record(self.stream)
+            updates += self.total_steps()
+            times.append(start.elapsed_time(stop))
+        return float(np.mean(times)), kernel, updates // steps
+
+    def close(self):
+        if self.graph is not None:
+            self.lib.tgk_graph_destroy(self.graph)
+            self.graph = None
+
+
 def host_leg(cfg, gpu_results: dict, seconds: float = 15.0, n_accuracy: int = 400):
     """The host restatement's leg of the benchmark.
 
@@ -539,6 +683,10 @@ WORKLOADS = {
     "shell": (brain145_shell_diffusion, 2_000_000,
               "145x174x145 SH45 Diffusion gamma 1 sigma^2 0.1 between two displaced ellipsoid meshes "
               "(2 x ~200k triangles, BVH termination), {n} seeds, step 0.1, save 0.25, max 300"),
+    "sh91": (brain145_sh91_fod_seeds, 10_000_000,
+             "145x174x145 SH91 (lmax 12) Diffusion gamma 1 sigma^2 0.1, {n} seeds drawn on the device from the "
+             "FOD (split-bf16 MFMA sampler) in 8 uneven chunks, one HIP graph per replay, fast path, step 0.1, "
+             "save 0.25, max 300"),
 }
 
 
@@ -551,6 +699,9 @@ def main():
     parser.add_argument("--seeds", type=int, default=None)
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--paths", default="fast,plain")
+    parser.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                        help="before timing, run one elementwise copy that reads and writes BYTES each "
+                             "(reference dispatch for hardware byte counters)")
     args = parser.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (seeds shard with no "
@@ -565,9 +716,15 @@ def main():
     t0 = time.perf_counter()
     generator, default_seeds, description = WORKLOADS[args.workload]
     cfg = generator(n_seeds=args.seeds or default_seeds)
-    workload = f"{args.workload}: {description.format(n=len(cfg['seeds']))}"
+    n_seeds = cfg["n_seeds"] if cfg.get("seeds") is None else len(cfg["seeds"])
+    workload = f"{args.workload}: {description.format(n=n_seeds)}"
     print(f"[bench] workload generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    if cfg.get("chunks") is not None:
+        run_graph_workload(args, cfg, workload)
+        return
 
+    if args.calibrate_copy > 0:
+        calibration_copy(torch, args.calibrate_copy)
     runtime = tg.HipRuntime()
     run = DeviceRun(tg, torch, runtime, cfg)
     paths = [p for p in args.paths.split(",") if p]
@@ -615,6 +772,55 @@ def main():
             results["plain"]["steps"] / results["plain"]["ms_per_step"])
     if not args.no_cpu_baseline:
         line["cpu_baseline"], line["accuracy_vs_fp64"] = host_leg(cfg, gpu_results)
+    print(json.dumps(line), flush=True)
+
+
+def calibration_copy(torch, n_bytes: int):
+    """out = src + 0 over n_bytes of float32: a kernel that reads and writes
+    exactly n_bytes each, to scale hardware byte counters."""
+    n = n_bytes // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    torch.add(src, 0.0, out=dst)
+    torch.cuda.synchronize()
+    print(f"[bench] calibration copy: {n * 4} bytes read and written", file=sys.stderr, flush=True)
+    del src, dst
+
+
+def run_graph_workload(args, cfg, workload):
+    import torch
+
+    import tractography_amd as tg
+
+    runtime = tg.HipRuntime()
+    run = GraphRun(tg, torch, runtime, cfg)
+    ms, kernel, updates = run.timed(args.warmup, args.steps)
+    run.close()
+    print(f"[bench] fast (graph): {ms:.1f} ms/replay, track kernels {kernel:.1f} ms, {updates} orientation updates, "
+          f"{updates / ms * 1e3:.3e} steps/s", file=sys.stderr, flush=True)
+    K = cfg["fod"].shape[-1]
+    bps = bytes_per_step(K, cfg["step_size"], cfg["save_at"])
+    achieved = bps * updates / (kernel * 1e-3) / 1e9
+    line = {
+        "metric": "streamline steps/s on 1 MI355X, mixed vs FP32 path, at <=1e-3 voxel max deviation",
+        "value": updates / (ms * 1e-3),
+        "unit": "streamline steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded numpy generators; seeds drawn on the device)",
+        "config": {"workload": workload, "path": "fast", "seeds": run.n, "chunks": run.chunks,
+                   "orientation_updates_per_step": updates},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / MI355X_HBM_PEAK_GBS, "traffic": None, "kernel": "track_fast",
+                     "bytes_per_step": bps},
+    }
     print(json.dumps(line), flush=True)
 
 
