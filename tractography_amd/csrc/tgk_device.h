@@ -156,8 +156,10 @@ __device__ __forceinline__ bool mask_contains(const MaskView &m, V3 p)
     const float rx = floorf(affine_row(m.inv, 0, p) + 0.5f);
     const float ry = floorf(affine_row(m.inv, 1, p) + 0.5f);
     const float rz = floorf(affine_row(m.inv, 2, p) + 0.5f);
-    if (!(rx >= 0.0f && rx < (float)m.dims[0] && ry >= 0.0f && ry < (float)m.dims[1] && rz >= 0.0f &&
-          rz < (float)m.dims[2]))
+    // Bitwise &: all three coordinates are computed anyway, and a branch per
+    // test costs more than the tests.
+    if (!((rx >= 0.0f) & (rx < (float)m.dims[0]) & (ry >= 0.0f) & (ry < (float)m.dims[1]) & (rz >= 0.0f) &
+          (rz < (float)m.dims[2])))
         return false;
     const uint64_t idx = ((uint64_t)(uint32_t)rx * m.dims[1] + (uint32_t)ry) * m.dims[2] + (uint32_t)rz;
     return m.mask[idx] != 0;
@@ -170,8 +172,10 @@ __device__ __forceinline__ int64_t field_voxel(const FieldView &f, V3 p)
     const float rx = rintf(affine_row(f.inv, 0, p));
     const float ry = rintf(affine_row(f.inv, 1, p));
     const float rz = rintf(affine_row(f.inv, 2, p));
-    if (!(rx >= 0.0f && rx < (float)f.dims[0] && ry >= 0.0f && ry < (float)f.dims[1] && rz >= 0.0f &&
-          rz < (float)f.dims[2]))
+    // Bitwise &: all three coordinates are computed anyway, and a branch per
+    // test costs more than the tests.
+    if (!((rx >= 0.0f) & (rx < (float)f.dims[0]) & (ry >= 0.0f) & (ry < (float)f.dims[1]) & (rz >= 0.0f) &
+          (rz < (float)f.dims[2])))
         return -1;
     return ((int64_t)(uint32_t)rx * f.dims[1] + (uint32_t)ry) * f.dims[2] + (uint32_t)rz;
 }
@@ -288,8 +292,10 @@ __device__ __forceinline__ bool mesh_far(const MeshView &m, V3 start, float len2
     const float rx = floorf(affine_row(m.near_inv, 0, start) + 0.5f);
     const float ry = floorf(affine_row(m.near_inv, 1, start) + 0.5f);
     const float rz = floorf(affine_row(m.near_inv, 2, start) + 0.5f);
-    if (!(rx >= 0.0f && rx < (float)m.near_dims[0] && ry >= 0.0f && ry < (float)m.near_dims[1] && rz >= 0.0f &&
-          rz < (float)m.near_dims[2]))
+    // Bitwise &: all three coordinates are computed anyway, and a branch per
+    // test costs more than the tests.
+    if (!((rx >= 0.0f) & (rx < (float)m.near_dims[0]) & (ry >= 0.0f) & (ry < (float)m.near_dims[1]) & (rz >= 0.0f) &
+          (rz < (float)m.near_dims[2])))
         return false;
     const uint64_t idx =
         ((uint64_t)(uint32_t)rx * m.near_dims[1] + (uint32_t)ry) * m.near_dims[2] + (uint32_t)rz;

@@ -41,8 +41,10 @@ __device__ __forceinline__ int64_t mask_cell(const MaskView &m, V3 p)
     const float rx = floorf(affine_row(m.inv, 0, p) + 0.5f);
     const float ry = floorf(affine_row(m.inv, 1, p) + 0.5f);
     const float rz = floorf(affine_row(m.inv, 2, p) + 0.5f);
-    if (!(rx >= 0.0f && rx < (float)m.dims[0] && ry >= 0.0f && ry < (float)m.dims[1] && rz >= 0.0f &&
-          rz < (float)m.dims[2]))
+    // Bitwise &: all three coordinates are computed anyway, and a branch per
+    // test costs more than the tests.
+    if (!((rx >= 0.0f) & (rx < (float)m.dims[0]) & (ry >= 0.0f) & (ry < (float)m.dims[1]) & (rz >= 0.0f) &
+          (rz < (float)m.dims[2])))
         return -1;
     return ((int64_t)(uint32_t)rx * m.dims[1] + (uint32_t)ry) * m.dims[2] + (uint32_t)rz;
 }
@@ -62,11 +64,20 @@ struct Lane {
 // One iteration of the reference loop (tracking.py:121-147) for a lane with
 // a streamline in flight; on termination the lane's results are written and
 // the lane becomes idle (L.g = -1).
+// Where a lane keeps its cell's coefficients between steps: SH rows of
+// order 8 in LDS (one float4 column per lane, read back every step), so no
+// register array is carried around the loop; DTI rows and order-12 rows in
+// registers.
+template <int MODEL, int LMAX> constexpr bool rows_in_lds() { return MODEL == MODEL_SH && LMAX == 8; }
+// Threads per block of the fast kernel.
+template <int LMAX> constexpr int fast_block() { return 256; }
+
 template <int MODEL, int ALGO, int DOMAIN, int LMAX>
 __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__restrict__ coeffs, uint32_t cstride,
-                                        Lane &L, float *q)
+                                        Lane &L, float *q, float4 *__restrict__ lds_rows)
 {
     constexpr int TERMS = Order<LMAX>::TERMS;
+    constexpr int VEC = Order<LMAX>::STRIDE / 4;
     bool stop = !(L.n < p.n_points && L.steps < p.max_steps);
     uint32_t hit = NO_HIT;
     if (!stop) {
@@ -104,9 +115,13 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
     if (cell != L.field_cell) {
         L.field_cell = cell;
         const float4 *row = reinterpret_cast<const float4 *>(coeffs + cell * (int64_t)cstride);
-        if constexpr (MODEL == MODEL_SH) {
+        if constexpr (rows_in_lds<MODEL, LMAX>()) {
 #pragma unroll
-            for (int j = 0; j < Order<LMAX>::STRIDE / 4; j++) {
+            for (int j = 0; j < VEC; j++)
+                lds_rows[j * fast_block<LMAX>() + threadIdx.x] = row[j];
+        } else if constexpr (MODEL == MODEL_SH) {
+#pragma unroll
+            for (int j = 0; j < VEC; j++) {
                 const float4 v = row[j];
                 if (4 * j < TERMS)
                     q[4 * j] = v.x;
@@ -125,6 +140,20 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
             q[3] = a.w;
             q[4] = b.x;
             q[5] = b.y;
+        }
+    }
+    if constexpr (rows_in_lds<MODEL, LMAX>()) {
+#pragma unroll
+        for (int j = 0; j < VEC; j++) {
+            const float4 v = lds_rows[j * fast_block<LMAX>() + threadIdx.x];
+            if (4 * j < TERMS)
+                q[4 * j] = v.x;
+            if (4 * j + 1 < TERMS)
+                q[4 * j + 1] = v.y;
+            if (4 * j + 2 < TERMS)
+                q[4 * j + 2] = v.z;
+            if (4 * j + 3 < TERMS)
+                q[4 * j + 3] = v.w;
         }
     }
     const fast::Direction dir = fast::direction_of(L.o);
@@ -149,8 +178,9 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
 }
 
 template <int MODEL, int ALGO, int DOMAIN, int LMAX>
-__global__ __launch_bounds__(256) void track_fast_kernel(FastLaunch f)
+__global__ __launch_bounds__(fast_block<LMAX>()) void track_fast_kernel(FastLaunch f)
 {
+    __shared__ float4 lds_rows[rows_in_lds<MODEL, LMAX>() ? Order<LMAX>::STRIDE / 4 * fast_block<LMAX>() : 1];
     const TrackLaunch &p = f.track;
     const int lane_id = __lane_id();
     const float *coeffs = static_cast<const float *>(f.coeffs);
@@ -199,7 +229,7 @@ __global__ __launch_bounds__(256) void track_fast_kernel(FastLaunch f)
         if (__ballot(L.g >= 0) == 0)
             break;
         if (L.g >= 0)
-            advance<MODEL, ALGO, DOMAIN, LMAX>(p, coeffs, cstride, L, q);
+            advance<MODEL, ALGO, DOMAIN, LMAX>(p, coeffs, cstride, L, q, lds_rows);
     }
 }
 
@@ -339,7 +369,7 @@ template <int MODEL, int ALGO, int DOMAIN, int LMAX> hipError_t occupancy_grid(i
     if (cache[device] == 0) {
         int per_cu = 0;
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void *>(track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX>), 256, 0);
+            &per_cu, reinterpret_cast<const void *>(track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX>), fast_block<LMAX>(), 0);
         if (e != hipSuccess)
             return e;
         hipDeviceProp_t prop;
@@ -363,7 +393,7 @@ hipError_t launch_fast(const FastLaunch &f, int grid_blocks, bool query, hipStre
         grid_blocks = blocks;
     }
     zero_counter<<<1, 1, 0, stream>>>(f.queue);
-    track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX><<<grid_blocks, 256, 0, stream>>>(f);
+    track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX><<<grid_blocks, fast_block<LMAX>(), 0, stream>>>(f);
     return hipGetLastError();
 }
 

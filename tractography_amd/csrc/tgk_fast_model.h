@@ -41,16 +41,22 @@ template <int L> struct Order {
     __host__ __device__ static constexpr int b_offset(int m) { return a_offset(m) + poly_terms(m); }
 };
 
-// q(w) and q'(w) for q = sum_j c[j] w^j (Horner).
+// q(w) and q'(w) for q = sum_j c[j] w^j (Horner; the derivative's first
+// step is peeled, 0 * w + v not being foldable under IEEE rules).
 template <int N>
 __device__ __forceinline__ void horner(const float *c, float w, float &v, float &d)
 {
     v = c[N - 1];
-    d = 0.0f;
+    if constexpr (N == 1) {
+        d = 0.0f;
+    } else {
+        d = v;
+        v = v * w + c[N - 2];
 #pragma unroll
-    for (int j = N - 2; j >= 0; j--) {
-        d = d * w + v;
-        v = v * w + c[j];
+        for (int j = N - 3; j >= 0; j--) {
+            d = d * w + v;
+            v = v * w + c[j];
+        }
     }
 }
 
