@@ -699,6 +699,12 @@ def main():
     parser.add_argument("--seeds", type=int, default=None)
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--paths", default="fast,plain")
+    parser.add_argument("--fast-grid", type=int, default=0,
+                        help="blocks of the fast kernel (default: occupancy-sized)")
+    parser.add_argument("--autotune", action="store_true",
+                        help="choose the fast kernel's grid on 100k seeds of the workload before timing")
+    parser.add_argument("--api", action="store_true",
+                        help="also time one tg.tractogram call on host arrays (transfers included)")
     parser.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                         help="before timing, run one elementwise copy that reads and writes BYTES each "
                              "(reference dispatch for hardware byte counters)")
@@ -726,6 +732,16 @@ def main():
     if args.calibrate_copy > 0:
         calibration_copy(torch, args.calibrate_copy)
     runtime = tg.HipRuntime()
+    if args.autotune:
+        tuner = tg.HipRuntime(path="fast")
+        sample = cfg["seeds"][:100_000]
+        tuned = tg.autotune.tune_fast_grid(make_algorithm(tg, cfg), make_domain(tg, cfg),
+                                           tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4))), sample,
+                                           runtime=tuner, save_at=cfg["save_at"],
+                                           maximum_streamline_length=cfg["max_length"])
+        args.fast_grid = tuned["blocks"]
+        print(f"[bench] autotuned fast grid: {tuned}", file=sys.stderr, flush=True)
+    runtime.set_fast_grid(args.fast_grid)
     run = DeviceRun(tg, torch, runtime, cfg)
     paths = [p for p in args.paths.split(",") if p]
     results = {}
@@ -770,9 +786,30 @@ def main():
     if "fast" in results and "plain" in results:
         line["speedup_fast_over_plain"] = (results["fast"]["steps"] / results["fast"]["ms_per_step"]) / (
             results["plain"]["steps"] / results["plain"]["ms_per_step"])
+    if args.api:
+        line["api_end_to_end"] = api_leg(tg, cfg, headline)
     if not args.no_cpu_baseline:
         line["cpu_baseline"], line["accuracy_vs_fp64"] = host_leg(cfg, gpu_results)
     print(json.dumps(line), flush=True)
+
+
+def api_leg(tg, cfg, path: str):
+    """One tg.tractogram call on host arrays (seeds in, ragged streamlines
+    out, so PCIe transfers and the host-side split are included), after one
+    untimed call on a small prefix."""
+    runtime = tg.HipRuntime(path=path)
+    field = tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4)))
+    domain = make_domain(tg, cfg)
+    algorithm = make_algorithm(tg, cfg)
+    kwargs = dict(save_at=cfg["save_at"], maximum_streamline_length=cfg["max_length"], runtime=runtime)
+    tg.tractogram(field, cfg["seeds"][:1000], domain, algorithm, **kwargs)
+    t0 = time.perf_counter()
+    result = tg.tractogram(field, cfg["seeds"], domain, algorithm, **kwargs)
+    seconds = time.perf_counter() - t0
+    points = int(np.sum(result.streamlines.lengths))
+    print(f"[bench] tg.tractogram on host arrays: {seconds:.2f} s, {len(result.streamlines)} streamlines, "
+          f"{points} points", file=sys.stderr, flush=True)
+    return {"seconds": seconds, "streamlines": len(result.streamlines), "points": points, "path": path}
 
 
 def calibration_copy(torch, n_bytes: int):

@@ -83,6 +83,11 @@ int tgk_runtime_synchronize(tgk_runtime *rt);
 int tgk_runtime_info(tgk_runtime *rt, char *name, int32_t name_len, int32_t *compute_units, int64_t *total_bytes);
 /* The runtime's HIP stream (hipStream_t) for callers sharing device buffers. */
 int tgk_runtime_stream(tgk_runtime *rt, void **stream);
+/* Grid of the FAST path's persistent kernel in blocks of 256 lanes; <= 0
+ * (the default) sizes it to the device's occupancy. No counterpart in the
+ * reference, whose kernel is launched one work-item per seed; set by
+ * tractography_amd.autotune. */
+int tgk_runtime_set_fast_grid(tgk_runtime *rt, int32_t blocks);
 
 /* ------------------------------------------------------ field, domain --- */
 

@@ -195,3 +195,22 @@ def test_plain_path_refuses_order12(runtime):
     algorithm = tg.algorithm.Transport(tg.local.RealSymmetricSphericalHarmonics(91), step_size=0.1, gamma=1.0)
     with pytest.raises(NotImplementedError, match="91"):
         tg.tracking.track(algorithm, domain, field, seeds, [0], 8, 0.25, 10.0, runtime)
+
+
+def test_fast_results_do_not_depend_on_the_grid(scene):
+    # Lanes pull streamlines from a queue, so the grid changes which lane
+    # tracks a seed but never the result; the tuner only picks the fastest.
+    cfg, field, domain = scene
+    algorithm = tg.algorithm.Diffusion(tg.local.RealSymmetricSphericalHarmonics(45), step_size=0.1, gamma=1.0,
+                                       noise_variance=0.1, seed=41)
+    rng = [algorithm.get_rng_seed()]
+    runtime = tg.HipRuntime(path="fast")
+    runtime.set_fast_grid(1)
+    one_block = tg.tracking.track(algorithm, domain, field, cfg["seeds"], rng, 2000, 0.25, 80.0, runtime)
+    tuned = tg.autotune.tune_fast_grid(algorithm, domain, field, cfg["seeds"][:500], runtime=runtime, save_at=0.25,
+                                       maximum_streamline_length=80.0, repeats=1)
+    assert tuned["blocks"] in tuned["timings"]
+    got = tg.tracking.track(algorithm, domain, field, cfg["seeds"], rng, 2000, 0.25, 80.0, runtime)
+    np.testing.assert_array_equal(got.lengths, one_block.lengths)
+    np.testing.assert_array_equal(got.steps, one_block.steps)
+    np.testing.assert_array_equal(got.points, one_block.points)

@@ -13,7 +13,7 @@ import numpy as np
 import numpy.typing as npt
 
 from . import algorithm as algorithm_module
-from . import core, local, seeds, tracking, utils
+from . import autotune, core, local, seeds, tracking, utils
 from . import domain as domain_module
 from . import field as field_module
 from .algorithm import BaseAlgorithm

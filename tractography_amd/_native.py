@@ -50,6 +50,7 @@ _SIGNATURES = {
     "tgk_last_error": (c_char_p, []),
     "tgk_version": (c_int, []),
     "tgk_runtime_create": (c_int, [c_int32, POINTER(c_void_p)]),
+    "tgk_runtime_set_fast_grid": (c_int, [_P, c_int32]),
     "tgk_runtime_destroy": (c_int, [_P]),
     "tgk_runtime_synchronize": (c_int, [_P]),
     "tgk_runtime_info": (c_int, [_P, c_char_p, c_int32, POINTER(c_int32), POINTER(c_int64)]),

@@ -86,6 +86,7 @@ struct tgk_runtime {
     DeviceBuffer seeds_table;  // prepared FOD-seeding table for seeds_key
     int32_t seeds_key[3] = {-1, -1, -1};  // model, K, method
     bool capturing = false;    // the stream is being recorded into a graph
+    int32_t fast_grid = 0;     // blocks of the fast kernel; <= 0: sized by occupancy
 };
 
 struct tgk_graph {
@@ -304,6 +305,14 @@ int tgk_runtime_info(tgk_runtime *rt, char *name, int32_t name_len, int32_t *com
         *compute_units = prop.multiProcessorCount;
     if (total_bytes)
         *total_bytes = (int64_t)prop.totalGlobalMem;
+    return TGK_OK;
+}
+
+int tgk_runtime_set_fast_grid(tgk_runtime *rt, int32_t blocks)
+{
+    if (!rt)
+        return fail(TGK_ERR_INVALID, "runtime is NULL");
+    rt->fast_grid = blocks > 0 ? blocks : 0;
     return TGK_OK;
 }
 
@@ -781,10 +790,7 @@ int run_track(tgk_runtime *rt, const tgk_domain *domain, tgk_field *field, const
     tgk::FastLaunch f;
     if (int r = fast_launch(rt, field, algorithm, p, &f))
         return r;
-    int grid = 0;
-    if (const char *env = getenv("TGK_FAST_GRID"))
-        grid = atoi(env);
-    TGK_HIP(tgk::launch_track_fast(f, grid, rt->stream), "fast tracking kernel");
+    TGK_HIP(tgk::launch_track_fast(f, rt->fast_grid, rt->stream), "fast tracking kernel");
     return TGK_OK;
 }
 

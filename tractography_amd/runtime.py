@@ -65,6 +65,11 @@ class HipRuntime:
         )
         return {"name": name.value.decode(), "compute_units": cus.value, "total_bytes": total.value}
 
+    def set_fast_grid(self, blocks: int) -> None:
+        """Blocks of the fast path's persistent kernel (<= 0: occupancy)."""
+        _native.check(_native.lib().tgk_runtime_set_fast_grid(self._handle, int(blocks)))
+        self.fast_grid = int(blocks) if blocks > 0 else 0
+
     def stream(self) -> int:
         s = ctypes.c_void_p()
         _native.check(_native.lib().tgk_runtime_stream(self._handle, ctypes.byref(s)))
