@@ -212,7 +212,9 @@ __device__ __forceinline__ float randn(uint2 &rng)
 }
 
 // exps2(p, x, 1) (core.cl:101-109) with cos(n) and sin(n)/n as Taylor
-// polynomials in w = |x|^2 for |x| < 0.5 (truncation below 2e-8).
+// polynomials in w = |x|^2 for |x| < 0.5 (truncation below 2e-8); larger
+// turns, rare and where trajectories are most sensitive, use the accurate
+// sincos of tgk_math.h rather than the hardware approximation.
 __device__ __forceinline__ V3 exps2(V3 p, V3 x)
 {
     const float w = x.x * x.x + x.y * x.y + x.z * x.z;
@@ -225,7 +227,7 @@ __device__ __forceinline__ V3 exps2(V3 p, V3 x)
     } else {
         const float n = __builtin_sqrtf(w);
         float s;
-        __sincosf(n, &s, &c);
+        tgk_sincosf(n, &s, &c);
         sn = s / n;
     }
     return v3(c * p.x + sn * x.x, c * p.y + sn * x.y, c * p.z + sn * x.z);
