@@ -708,11 +708,18 @@ def main():
     parser.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                         help="before timing, run one elementwise copy that reads and writes BYTES each "
                              "(reference dispatch for hardware byte counters)")
+    parser.add_argument("--no-traffic", action="store_true",
+                        help="skip the two hardware-counter runs that measure the tracking kernel's HBM bytes")
     args = parser.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (seeds shard with no "
               "communication, see the multi-GPU section of the design notes)", file=sys.stderr)
         sys.exit(2)
+    # Counter runs are child processes started before this process touches
+    # the GPU.
+    traffic = None
+    if not args.no_traffic and args.calibrate_copy == 0 and args.workload != "sh91" and "fast" in args.paths:
+        traffic = traffic_leg(args)
 
     import torch
 
@@ -780,9 +787,12 @@ def main():
         "paths": {k: {"steps_per_s": v["steps"] / (v["ms_per_step"] * 1e-3), "ms_per_step": v["ms_per_step"],
                       "kernel_ms": v["kernel_ms"]} for k, v in results.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / MI355X_HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / MI355X_HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic and headline == "fast" else None,
                      "kernel": f"track_{headline}", "bytes_per_step": bps},
     }
+    if traffic and headline == "fast":
+        line["roofline"]["traffic_detail"] = traffic
     if "fast" in results and "plain" in results:
         line["speedup_fast_over_plain"] = (results["fast"]["steps"] / results["fast"]["ms_per_step"]) / (
             results["plain"]["steps"] / results["plain"]["ms_per_step"])
@@ -810,6 +820,67 @@ def api_leg(tg, cfg, path: str):
     print(f"[bench] tg.tractogram on host arrays: {seconds:.2f} s, {len(result.streamlines)} streamlines, "
           f"{points} points", file=sys.stderr, flush=True)
     return {"seconds": seconds, "streamlines": len(result.streamlines), "points": points, "path": path}
+
+
+TRAFFIC_CALIBRATION_BYTES = 1 << 30
+
+
+def _counter_values(csv_path: Path, counter: str) -> dict:
+    """Kernel name -> list of values of `counter`, one per dispatch."""
+    import csv
+
+    values = {}
+    with open(csv_path) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] == counter:
+                values.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    return values
+
+
+def traffic_leg(args):
+    """HBM bytes per launch of the fast tracking kernel from two rocprofv3
+    counter runs of this script (FETCH_SIZE, then WRITE_SIZE, KiB per
+    dispatch), each scaled by what the counter reads for an elementwise copy
+    that reads and writes 1 GiB in the same run. Returns None if the profiler
+    is unavailable or a run fails."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    profiler = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not Path(profiler).exists():
+        return None
+    measured = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = Path(tmp) / counter
+            cmd = [profiler, "--pmc", counter, "--output-format", "csv", "-d", str(out), "-o", "run", "--",
+                   sys.executable, str(ROOT / "bench.py"), "--workload", args.workload, "--paths", "fast",
+                   "--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--no-traffic",
+                   "--calibrate-copy", str(TRAFFIC_CALIBRATION_BYTES)]
+            if args.seeds:
+                cmd += ["--seeds", str(args.seeds)]
+            print(f"[bench] counter run {counter}", file=sys.stderr, flush=True)
+            try:
+                subprocess.run(cmd, cwd="/tmp", stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                               timeout=240, check=True, env={**os.environ, "TMPDIR": "/tmp"})
+            except (subprocess.SubprocessError, OSError) as e:
+                print(f"[bench] counter run {counter} failed: {e}", file=sys.stderr, flush=True)
+                return None
+            files = sorted(out.rglob("*counter_collection.csv"))
+            if not files:
+                return None
+            values = _counter_values(files[0], counter)
+            kernel = [v for k, v in values.items() if "track_fast_kernel" in k]
+            copy = [max(v) for k, v in values.items() if "CUDAFunctorOnSelf_add" in k]
+            if not kernel or not copy or copy[0] <= 0:
+                return None
+            scale = TRAFFIC_CALIBRATION_BYTES / (copy[0] * 1024.0)
+            measured[counter] = {"kib_per_dispatch": float(np.mean(kernel[0])), "scale": scale,
+                                 "bytes": float(np.mean(kernel[0])) * 1024.0 * scale}
+    return {"bytes_per_launch": measured["FETCH_SIZE"]["bytes"] + measured["WRITE_SIZE"]["bytes"],
+            "fetch_bytes": measured["FETCH_SIZE"]["bytes"], "write_bytes": measured["WRITE_SIZE"]["bytes"],
+            "fetch_scale": measured["FETCH_SIZE"]["scale"], "write_scale": measured["WRITE_SIZE"]["scale"]}
 
 
 def calibration_copy(torch, n_bytes: int):
