@@ -653,9 +653,10 @@ def host_leg(cfg, gpu_results: dict, seconds: float = 15.0, n_accuracy: int = 40
         r = oracle.track(domain, field, algorithm, seeds, rng_seeds, 50_000, cfg["save_at"], cfg["max_length"],
                          threads=threads)
         dt = time.perf_counter() - t0
-        if dt > seconds / 4 or sample >= len(cfg["seeds"]):
+        if dt > 0.6 * seconds or sample >= len(cfg["seeds"]):
             break
-        sample = min(len(cfg["seeds"]), int(sample * max(2.0, seconds / max(dt, 1e-3))))
+        # Steps/s is additive over seeds: size the next sample to ~`seconds`.
+        sample = min(len(cfg["seeds"]), int(sample * min(64.0, max(2.0, seconds / max(dt, 1e-3)))))
     steps = int(r["steps"].astype(np.int64).sum())
     baseline = {"value": steps / dt, "unit": "streamline steps/s", "cores": threads, "kind": "port",
                 "sample": f"first {len(seeds)} seeds of the workload, {steps} steps in {dt:.1f} s "
