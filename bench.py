@@ -367,6 +367,8 @@ def brain145_shell_diffusion(seed: int = 0, n_seeds: int = 2_000_000, shape=SHAP
     extent = _semi_axes(shape, (60, 75, 60)) + amplitude + 1.0
     cfg["seeds"] = seeds_in_shell(rng, centre, radii, extent, n_seeds)
     cfg["mesh"] = (vertices, faces)
+    # Culling cells of half a voxel certifying segments up to 1.5 steps.
+    cfg["mesh_grid"] = (0.5, 1.5 * cfg["step_size"])
     return cfg
 
 
@@ -422,7 +424,8 @@ def make_domain(tg, cfg):
     """The workload's tracking domain: its boundary meshes if it has any,
     else its mask image."""
     if cfg.get("mesh") is not None:
-        return tg.domain.Mesh(*cfg["mesh"])
+        cell_size, reach = cfg.get("mesh_grid", (1.0, 0.5))
+        return tg.domain.Mesh(*cfg["mesh"], cell_size=cell_size, reach=reach)
     return tg.domain.Image(tg.Nifti1Image(cfg["mask"], np.eye(4)))
 
 

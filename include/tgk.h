@@ -119,6 +119,15 @@ int tgk_mesh_domain_create(tgk_runtime *rt, const float *vertices, uint32_t n_ve
  * to a mask domain; mesh domains already carry theirs. */
 int tgk_domain_set_boundary(tgk_domain *domain, const float *vertices, uint32_t n_vertices,
                             const uint32_t *triangles, const float *cumulative, uint32_t n_triangles);
+/* Per-cell candidate triangles on the culling grid's cells (the grid given
+ * to tgk_mesh_domain_create): cell_start (n_cells + 1) u32 CSR offsets,
+ * entries (n_entries, 12) f32 rows (a.xyz, slot bits, (b - a).xyz, 0,
+ * (c - a).xyz, 0) per candidate, in BVH leaf order within a cell. The FAST
+ * path answers segments up to `reach` long from these lists instead of a
+ * traversal. No counterpart in the reference, whose kernel traverses the BVH
+ * for every step (bvh.cl:83-131). */
+int tgk_domain_set_segment_grid(tgk_domain *domain, const uint32_t *cell_start, uint64_t n_cells,
+                                const float *entries, uint64_t n_entries);
 int tgk_domain_destroy(tgk_domain *domain);
 
 /* ------------------------------------------------------------ tracking --- */
@@ -217,6 +226,10 @@ int tgk_domain_contains(tgk_runtime *rt, const tgk_domain *domain, const float *
 /* Closest boundary hit per segment (bvh.cl:83-131) of a mesh domain. */
 int tgk_bvh_segment_hit(tgk_runtime *rt, const tgk_domain *domain, const float *starts, const float *ends, int64_t n,
                         uint8_t *found, uint32_t *hit, float *t);
+/* The same query answered from the candidate lists where they decide (the
+ * FAST path's method); for the parity tests against the traversal. */
+int tgk_grid_segment_hit(tgk_runtime *rt, const tgk_domain *domain, const float *starts, const float *ends,
+                         int64_t n, uint8_t *found, uint32_t *hit, float *t);
 /* Evaluates the device elementary functions of the numeric contract:
  * fn 0 sin, 1 cos, 2 atan2(x, y), 3 acos, 4 exp, 5 log, 6 sqrt, 7 x / y. */
 int tgk_math(tgk_runtime *rt, int32_t fn, const float *x, const float *y, int64_t n, float *out);

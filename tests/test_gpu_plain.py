@@ -311,3 +311,44 @@ def test_plain_mesh_domain_bit_exact(runtime, cfg0, name, oracle_threads):
     hit = got.hits != 0xFFFFFFFF
     print(f"[mesh {name}] {hit.mean():.3f} of streamlines end on the boundary, {int(got.steps.sum())} steps")
     assert hit.mean() > 0.3
+
+
+def _segment_hits(runtime, device, fn, starts, ends):
+    from tractography_amd import _native
+
+    starts = np.ascontiguousarray(starts, np.float32)
+    ends = np.ascontiguousarray(ends, np.float32)
+    n = len(starts)
+    found = np.zeros(n, np.uint8)
+    hit = np.zeros(n, np.uint32)
+    t = np.zeros(n, np.float32)
+    _native.check(getattr(_native.lib(), fn)(runtime.handle, device.handle, _native.ptr(starts), _native.ptr(ends), n,
+                                             _native.ptr(found), _native.ptr(hit), _native.ptr(t)))
+    return found.astype(bool), hit, t
+
+
+def test_candidate_lists_answer_as_the_bvh(runtime):
+    # Segments near the surfaces (where the lists are used), longer than the
+    # lists' reach (traversal fallback) and off the grid: identical answers.
+    mesh, seeds, centre, radii = fixtures.shell_scene(n_seeds=10, seed=21)
+    device = mesh.upload(runtime)
+    data = mesh._ensure_boundary_data()
+    rng = np.random.default_rng(5)
+    tri = data["vertices"][data["indices"][:, :3].astype(np.int64), :3]
+    pick = rng.integers(0, len(tri), 200_000)
+    w = rng.dirichlet([1, 1, 1], len(pick))
+    on_surface = np.einsum("nk,nkc->nc", w, tri[pick])
+    direction = rng.standard_normal((len(pick), 3))
+    direction /= np.linalg.norm(direction, axis=1, keepdims=True)
+    length = np.concatenate([rng.uniform(0.001, mesh.reach, len(pick) - 20_000), rng.uniform(0.6, 3.0, 20_000)])
+    starts = on_surface - direction * length[:, None] * rng.uniform(0, 1, (len(pick), 1))
+    ends = starts + direction * length[:, None]
+    starts[:1000] += 100.0  # off the grid
+    ends[:1000] += 100.0
+    bvh = _segment_hits(runtime, device, "tgk_bvh_segment_hit", starts, ends)
+    grid = _segment_hits(runtime, device, "tgk_grid_segment_hit", starts, ends)
+    print(f"[candidate lists] {bvh[0].mean():.3f} of segments hit", flush=True)
+    assert bvh[0].mean() > 0.3
+    np.testing.assert_array_equal(grid[0], bvh[0])
+    np.testing.assert_array_equal(grid[1][bvh[0]], bvh[1][bvh[0]])
+    np.testing.assert_array_equal(grid[2][bvh[0]], bvh[2][bvh[0]])

@@ -7,7 +7,7 @@ host fallback: without libtgk.so every device entry point raises.
 from __future__ import annotations
 
 import ctypes
-from ctypes import POINTER, c_char_p, c_float, c_int, c_int32, c_int64, c_uint32, c_void_p
+from ctypes import POINTER, c_char_p, c_float, c_int, c_int32, c_int64, c_uint32, c_uint64, c_void_p
 from pathlib import Path
 
 import numpy as np
@@ -94,6 +94,8 @@ _SIGNATURES = {
     "tgk_field_evaluate": (c_int, [_P, _P, _P, c_int64, _P]),
     "tgk_domain_contains": (c_int, [_P, _P, _P, c_int64, _P]),
     "tgk_bvh_segment_hit": (c_int, [_P, _P, _P, _P, c_int64, _P, _P, _P]),
+    "tgk_grid_segment_hit": (c_int, [_P, _P, _P, _P, c_int64, _P, _P, _P]),
+    "tgk_domain_set_segment_grid": (c_int, [_P, _P, c_uint64, _P, c_uint64]),
     "tgk_math": (c_int, [_P, c_int32, _P, _P, c_int64, _P]),
 }
 

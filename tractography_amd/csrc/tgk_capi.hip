@@ -116,7 +116,7 @@ struct tgk_domain {
     // boundary surface (seeding) and BVH (mesh domains)
     DeviceBuffer vertices, triangles, cumulative;
     uint32_t n_vertices = 0, n_triangles = 0;
-    DeviceBuffer bvh_min, bvh_max, bvh_nodes, bvh_order, near;
+    DeviceBuffer bvh_min, bvh_max, bvh_nodes, bvh_order, near, cell_start, cell_entries;
     uint32_t n_nodes = 0;
     uint32_t near_dims[3] = {0, 0, 0};
     float near_inverse_affine[16];
@@ -174,6 +174,8 @@ tgk::MeshView mesh_view(const tgk_domain *d)
         v.near_dims[i] = d->near_dims[i];
     copy_rows3(d->near_inverse_affine, v.near_inv);
     v.reach = d->near.ptr ? d->reach : -1.0f;
+    v.cell_start = d->cell_start.as<const uint32_t>();
+    v.cell_entries = d->cell_entries.as<const float4>();
     return v;
 }
 
@@ -482,6 +484,40 @@ int tgk_mesh_domain_create(tgk_runtime *rt, const float *vertices, uint32_t n_ve
     }
     TGK_HIP(hipStreamSynchronize(s), "mesh upload");
     *out = d.release();
+    return TGK_OK;
+}
+
+int tgk_domain_set_segment_grid(tgk_domain *domain, const uint32_t *cell_start, uint64_t n_cells,
+                                const float *entries, uint64_t n_entries)
+{
+    if (!domain || !cell_start || (n_entries > 0 && !entries))
+        return fail(TGK_ERR_INVALID, "NULL argument");
+    if (domain->kind != tgk::DOMAIN_MESH || !domain->near.ptr)
+        return fail(TGK_ERR_INVALID, "candidate lists need a mesh domain with a culling grid");
+    const uint64_t cells = (uint64_t)domain->near_dims[0] * domain->near_dims[1] * domain->near_dims[2];
+    if (n_cells != cells)
+        return fail(TGK_ERR_INVALID, "candidate lists for %llu cells, the culling grid has %llu",
+                    (unsigned long long)n_cells, (unsigned long long)cells);
+    if (cell_start[0] != 0 || cell_start[n_cells] != n_entries || n_entries > 0xffffffffull)
+        return fail(TGK_ERR_INVALID, "candidate offsets do not span the entries");
+    for (uint64_t i = 0; i < n_cells; i++)
+        if (cell_start[i + 1] < cell_start[i])
+            return fail(TGK_ERR_INVALID, "candidate offsets decrease at cell %llu", (unsigned long long)i);
+    for (uint64_t i = 0; i < n_entries; i++) {
+        uint32_t slot;
+        memcpy(&slot, entries + 12 * i + 3, sizeof slot);
+        if (slot >= domain->n_triangles)
+            return fail(TGK_ERR_INVALID, "candidate %llu names triangle %u of %u", (unsigned long long)i, slot,
+                        domain->n_triangles);
+    }
+    if (int r = select_device(domain->rt))
+        return r;
+    hipStream_t s = domain->rt->stream;
+    if (int r = upload(domain->cell_start, cell_start, (size_t)n_cells + 1, s, "candidate upload"))
+        return r;
+    if (int r = upload(domain->cell_entries, entries, (size_t)n_entries * 12, s, "candidate upload"))
+        return r;
+    TGK_HIP(hipStreamSynchronize(s), "candidate upload");
     return TGK_OK;
 }
 
@@ -915,13 +951,19 @@ int tgk_domain_contains(tgk_runtime *rt, const tgk_domain *domain, const float *
     return fetch(out, dout, (size_t)n, rt->stream);
 }
 
-int tgk_bvh_segment_hit(tgk_runtime *rt, const tgk_domain *domain, const float *starts, const float *ends, int64_t n,
-                        uint8_t *found, uint32_t *hit, float *t)
+}  // extern "C"
+
+namespace {
+
+int segment_hit(tgk_runtime *rt, const tgk_domain *domain, int method, const float *starts, const float *ends,
+                int64_t n, uint8_t *found, uint32_t *hit, float *t)
 {
     if (!rt || !domain || n < 0 || (n > 0 && (!starts || !ends || !found || !hit || !t)))
         return fail(TGK_ERR_INVALID, "bad arguments");
     if (domain->n_nodes == 0)
         return fail(TGK_ERR_INVALID, "domain has no BVH");
+    if (method == 1 && !domain->cell_start.ptr)
+        return fail(TGK_ERR_INVALID, "domain has no candidate lists");
     if (n == 0)
         return TGK_OK;
     if (int r = select_device(rt))
@@ -941,12 +983,29 @@ int tgk_bvh_segment_hit(tgk_runtime *rt, const tgk_domain *domain, const float *
         return r;
     if (int r = st.out((size_t)n, &dt))
         return r;
-    TGK_HIP(tgk::launch_bvh_hit(mesh_view(domain), ds, de, n, dfound, dhit, dt, rt->stream), "BVH kernel");
+    TGK_HIP(tgk::launch_bvh_hit(mesh_view(domain), method, ds, de, n, dfound, dhit, dt, rt->stream),
+            "segment hit kernel");
     if (int r = fetch(found, dfound, (size_t)n, rt->stream))
         return r;
     if (int r = fetch(hit, dhit, (size_t)n, rt->stream))
         return r;
     return fetch(t, dt, (size_t)n, rt->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tgk_bvh_segment_hit(tgk_runtime *rt, const tgk_domain *domain, const float *starts, const float *ends, int64_t n,
+                        uint8_t *found, uint32_t *hit, float *t)
+{
+    return segment_hit(rt, domain, 0, starts, ends, n, found, hit, t);
+}
+
+int tgk_grid_segment_hit(tgk_runtime *rt, const tgk_domain *domain, const float *starts, const float *ends,
+                         int64_t n, uint8_t *found, uint32_t *hit, float *t)
+{
+    return segment_hit(rt, domain, 1, starts, ends, n, found, hit, t);
 }
 
 int tgk_math(tgk_runtime *rt, int32_t fn, const float *x, const float *y, int64_t n, float *out)

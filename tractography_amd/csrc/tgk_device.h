@@ -53,6 +53,10 @@ struct MeshView {
     uint32_t near_dims[3];
     float near_inv[12];
     float reach;
+    // Candidate triangles per culling-grid cell (CSR, leaf order): entries
+    // are 3 float4 rows (a, slot bits), (b - a), (c - a). Null when absent.
+    const uint32_t *cell_start;
+    const float4 *cell_entries;
 };
 
 struct AlgoParams {
@@ -278,6 +282,86 @@ __device__ inline bool bvh_segment_hit(const MeshView &m, V3 start, V3 end, uint
                 return found;
             stack[size++] = nd.y;
             stack[size++] = nd.x;
+        }
+    }
+    return found;
+}
+
+// The culling-grid cell of p, or -1 outside the grid.
+__device__ __forceinline__ int64_t mesh_grid_cell(const MeshView &m, V3 p)
+{
+    const float rx = floorf(affine_row(m.near_inv, 0, p) + 0.5f);
+    const float ry = floorf(affine_row(m.near_inv, 1, p) + 0.5f);
+    const float rz = floorf(affine_row(m.near_inv, 2, p) + 0.5f);
+    if (!((rx >= 0.0f) & (rx < (float)m.near_dims[0]) & (ry >= 0.0f) & (ry < (float)m.near_dims[1]) & (rz >= 0.0f) &
+          (rz < (float)m.near_dims[2])))
+        return -1;
+    return ((int64_t)(uint32_t)rx * m.near_dims[1] + (uint32_t)ry) * m.near_dims[2] + (uint32_t)rz;
+}
+
+// Moller-Trumbore on a triangle given as a, b - a, c - a: the arithmetic of
+// bvh_triangle_hit, with the two edges precomputed (same float32 values).
+__device__ __forceinline__ bool grid_triangle_hit(float4 a4, float4 e14, float4 e24, const float st[3],
+                                                  const float dir[3], float max_t, float &hit_t)
+{
+    const float e1[3] = {e14.x, e14.y, e14.z};
+    const float e2[3] = {e24.x, e24.y, e24.z};
+    const float p[3] = {dir[1] * e2[2] - dir[2] * e2[1], dir[2] * e2[0] - dir[0] * e2[2],
+                        dir[0] * e2[1] - dir[1] * e2[0]};
+    const float det = (e1[0] * p[0] + e1[1] * p[1]) + e1[2] * p[2];
+    if (fabsf(det) < 1e-10f)
+        return false;
+    const float inv = 1.0f / det;
+    const float off[3] = {st[0] - a4.x, st[1] - a4.y, st[2] - a4.z};
+    const float u = ((off[0] * p[0] + off[1] * p[1]) + off[2] * p[2]) * inv;
+    if (u < -1e-6f || u > 1.0f + 1e-6f)
+        return false;
+    const float q[3] = {off[1] * e1[2] - off[2] * e1[1], off[2] * e1[0] - off[0] * e1[2],
+                        off[0] * e1[1] - off[1] * e1[0]};
+    const float v = ((dir[0] * q[0] + dir[1] * q[1]) + dir[2] * q[2]) * inv;
+    if (v < -1e-6f || u + v > 1.0f + 1e-6f)
+        return false;
+    const float t = ((e2[0] * q[0] + e2[1] * q[1]) + e2[2] * q[2]) * inv;
+    if (t <= 1e-6f || t > max_t)
+        return false;
+    hit_t = t;
+    return true;
+}
+
+// Closest hit of a segment no longer than `reach` through the candidate
+// lists: the candidates of the start cell hold every triangle whose bounding
+// box the segment can touch, in the traversal's leaf order, so the same
+// replacement rule (t <= best replaces) gives bvh_segment_hit's answer.
+// Returns false with `decided` unset when the lists cannot decide (no lists,
+// segment too long, start outside the grid).
+__device__ inline bool grid_segment_hit(const MeshView &m, V3 start, V3 end, bool &decided, uint32_t &hit,
+                                        float &hit_t)
+{
+    decided = false;
+    hit_t = 1.0f;
+    hit = 0;
+    if (m.cell_start == nullptr || !(m.reach > 0.0f))
+        return false;
+    const float st[3] = {start.x, start.y, start.z};
+    const float dir[3] = {end.x - start.x, end.y - start.y, end.z - start.z};
+    const float len2 = (dir[0] * dir[0] + dir[1] * dir[1]) + dir[2] * dir[2];
+    if (!(len2 <= m.reach * m.reach))
+        return false;
+    const int64_t cell = mesh_grid_cell(m, start);
+    if (cell < 0)
+        return false;
+    decided = true;
+    bool found = false;
+    const uint32_t end_i = m.cell_start[cell + 1];
+    for (uint32_t i = m.cell_start[cell]; i < end_i; i++) {
+        const float4 a = m.cell_entries[3 * (size_t)i];
+        const float4 e1 = m.cell_entries[3 * (size_t)i + 1];
+        const float4 e2 = m.cell_entries[3 * (size_t)i + 2];
+        float t;
+        if (grid_triangle_hit(a, e1, e2, st, dir, hit_t, t)) {
+            hit_t = t;
+            hit = __float_as_uint(a.w);
+            found = true;
         }
     }
     return found;

@@ -89,11 +89,20 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
             }
             stop = !L.mask_inside;
         } else if (!L.first) {
-            const float dx = L.point.x - L.previous.x, dy = L.point.y - L.previous.y, dz = L.point.z - L.previous.z;
-            if (!mesh_far(p.mesh, L.previous, dx * dx + dy * dy + dz * dz)) {
-                uint32_t tri;
-                float t;
-                if (bvh_segment_hit(p.mesh, L.previous, L.point, tri, t)) {
+            // The start cell's candidate list answers segments up to `reach`
+            // long with a few independent loads; longer segments, or starts
+            // off the grid, traverse the BVH.
+            bool decided;
+            uint32_t tri;
+            float t;
+            if (grid_segment_hit(p.mesh, L.previous, L.point, decided, tri, t)) {
+                hit = tri;
+                stop = true;
+            } else if (!decided) {
+                const float dx = L.point.x - L.previous.x, dy = L.point.y - L.previous.y,
+                            dz = L.point.z - L.previous.z;
+                if (!mesh_far(p.mesh, L.previous, dx * dx + dy * dy + dz * dz) &&
+                    bvh_segment_hit(p.mesh, L.previous, L.point, tri, t)) {
                     hit = tri;
                     stop = true;
                 }

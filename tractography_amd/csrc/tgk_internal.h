@@ -47,8 +47,9 @@ hipError_t launch_update(const AlgoParams &a, uint32_t rng_seed, const float *pa
                          int64_t n, float *out, hipStream_t stream);
 hipError_t launch_field_evaluate(const FieldView &f, const float *points, int64_t n, float *out, hipStream_t stream);
 hipError_t launch_contains(const MaskView &m, const float *points, int64_t n, uint8_t *out, hipStream_t stream);
-hipError_t launch_bvh_hit(const MeshView &m, const float *starts, const float *ends, int64_t n, uint8_t *found,
-                          uint32_t *hit, float *t, hipStream_t stream);
+// method 0: BVH traversal; 1: per-cell candidate lists where they decide.
+hipError_t launch_bvh_hit(const MeshView &m, int method, const float *starts, const float *ends, int64_t n,
+                          uint8_t *found, uint32_t *hit, float *t, hipStream_t stream);
 hipError_t launch_math(int fn, const float *x, const float *y, int64_t n, float *out, hipStream_t stream);
 hipError_t launch_domain_seed(const float4 *vertices, const uint4 *triangles, const float *cumulative,
                               uint32_t n_triangles, uint32_t rng_seed, int64_t n, float *out, hipStream_t stream);

@@ -268,17 +268,23 @@ __global__ void contains_kernel(MaskView m, const float *__restrict__ points, in
     out[g] = mask_contains(m, v3(points[3 * g], points[3 * g + 1], points[3 * g + 2])) ? 1 : 0;
 }
 
-__global__ void bvh_hit_kernel(MeshView m, const float *__restrict__ starts, const float *__restrict__ ends,
-                               int64_t n, uint8_t *__restrict__ found, uint32_t *__restrict__ hit,
-                               float *__restrict__ t)
+// method 0: BVH traversal; 1: the start cell's candidate list where it
+// decides, else the traversal.
+__global__ void bvh_hit_kernel(MeshView m, int method, const float *__restrict__ starts,
+                               const float *__restrict__ ends, int64_t n, uint8_t *__restrict__ found,
+                               uint32_t *__restrict__ hit, float *__restrict__ t)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n)
         return;
     uint32_t tri;
     float ht;
-    const bool f = bvh_segment_hit(m, v3(starts[3 * g], starts[3 * g + 1], starts[3 * g + 2]),
-                                   v3(ends[3 * g], ends[3 * g + 1], ends[3 * g + 2]), tri, ht);
+    const V3 a = v3(starts[3 * g], starts[3 * g + 1], starts[3 * g + 2]);
+    const V3 b = v3(ends[3 * g], ends[3 * g + 1], ends[3 * g + 2]);
+    bool decided = false;
+    bool f = method == 1 ? grid_segment_hit(m, a, b, decided, tri, ht) : false;
+    if (!decided)
+        f = bvh_segment_hit(m, a, b, tri, ht);
     found[g] = f ? 1 : 0;
     hit[g] = tri;
     t[g] = ht;
@@ -421,12 +427,12 @@ hipError_t launch_contains(const MaskView &m, const float *points, int64_t n, ui
     return hipGetLastError();
 }
 
-hipError_t launch_bvh_hit(const MeshView &m, const float *starts, const float *ends, int64_t n, uint8_t *found,
-                          uint32_t *hit, float *t, hipStream_t stream)
+hipError_t launch_bvh_hit(const MeshView &m, int method, const float *starts, const float *ends, int64_t n,
+                          uint8_t *found, uint32_t *hit, float *t, hipStream_t stream)
 {
     if (n == 0)
         return hipSuccess;
-    bvh_hit_kernel<<<blocks_for(n, 128), 128, 0, stream>>>(m, starts, ends, n, found, hit, t);
+    bvh_hit_kernel<<<blocks_for(n, 128), 128, 0, stream>>>(m, method, starts, ends, n, found, hit, t);
     return hipGetLastError();
 }
 

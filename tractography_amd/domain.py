@@ -340,8 +340,9 @@ class Mesh(BaseDomain):
             crossings[start : start + 256] = hit.sum(axis=1)
         return crossings % 2 == 1
 
-    def culling_grid(self):
-        """near (X, Y, Z) u8, its inverse affine, and the reach it certifies."""
+    def _grid_cells(self):
+        """Geometry of the culling grid and the cells each triangle's bounding
+        box reaches: (dims, origin, lo, hi) with per-triangle cell ranges."""
         data = self._ensure_boundary_data()
         tri = self.boundary.vertices[data["indices"][:, :3].astype(np.int64)]
         tmin, tmax = tri.min(axis=1), tri.max(axis=1)
@@ -351,8 +352,17 @@ class Mesh(BaseDomain):
         dims = np.ceil((tmax.max(axis=0) + margin + h - origin) / h).astype(np.int64) + 1
         lo = np.floor((tmin - margin - origin) / h + 0.5).astype(np.int64)
         hi = np.floor((tmax + margin - origin) / h + 0.5).astype(np.int64)
-        lo = np.clip(lo, 0, dims - 1)
-        hi = np.clip(hi, 0, dims - 1)
+        return dims, origin, np.clip(lo, 0, dims - 1), np.clip(hi, 0, dims - 1)
+
+    def _grid_inverse_affine(self, origin) -> np.ndarray:
+        affine = np.eye(4)
+        affine[:3, :3] *= self.cell_size
+        affine[:3, 3] = origin
+        return np.ascontiguousarray(np.linalg.inv(affine), dtype=np.float32)
+
+    def culling_grid(self):
+        """near (X, Y, Z) u8, its inverse affine, and the reach it certifies."""
+        dims, origin, lo, hi = self._grid_cells()
         near = np.zeros(tuple(dims), dtype=np.uint8)
         span = (hi - lo).max(axis=0)
         for ox in range(span[0] + 1):
@@ -361,11 +371,51 @@ class Mesh(BaseDomain):
                     cell = lo + np.array([ox, oy, oz])
                     ok = np.all(cell <= hi, axis=1)
                     near[cell[ok, 0], cell[ok, 1], cell[ok, 2]] = 1
-        affine = np.eye(4)
-        affine[:3, :3] *= h
-        affine[:3, 3] = origin
-        inverse = np.ascontiguousarray(np.linalg.inv(affine), dtype=np.float32)
-        return near, inverse, np.float32(self.reach)
+        return near, self._grid_inverse_affine(origin), np.float32(self.reach)
+
+    def segment_grid(self):
+        """Per-cell candidate triangles on the culling grid's cells: every
+        triangle whose bounding box comes within `reach` of the cell, in the
+        order a BVH traversal reaches them (leaf order), as CSR.
+
+        Returns cell_start (n_cells + 1,) u32 and entries (n, 12) f32 rows
+        (a, slot bits, b - a, 0, c - a, 0) in float32 arithmetic. A segment of
+        length <= reach starting in a cell can only hit that cell's
+        candidates; testing them in order with the traversal's replacement
+        rule gives the traversal's closest hit.
+        """
+        data = self._ensure_boundary_data()
+        dims, _, lo, hi = self._grid_cells()
+        n_tri = len(data["indices"])
+        position = np.empty(n_tri, dtype=np.int64)
+        position[data["bvh_triangles"].astype(np.int64)] = np.arange(n_tri)
+        cells, slots = [], []
+        span = (hi - lo).max(axis=0)
+        slot_ids = np.arange(n_tri, dtype=np.int64)
+        for ox in range(span[0] + 1):
+            for oy in range(span[1] + 1):
+                for oz in range(span[2] + 1):
+                    cell = lo + np.array([ox, oy, oz])
+                    ok = np.all(cell <= hi, axis=1)
+                    c = cell[ok]
+                    cells.append((c[:, 0] * dims[1] + c[:, 1]) * dims[2] + c[:, 2])
+                    slots.append(slot_ids[ok])
+        cells = np.concatenate(cells)
+        slots = np.concatenate(slots)
+        order = np.lexsort((position[slots], cells))
+        cells, slots = cells[order], slots[order]
+        n_cells = int(np.prod(dims))
+        cell_start = np.zeros(n_cells + 1, dtype=np.uint32)
+        np.cumsum(np.bincount(cells, minlength=n_cells), out=cell_start[1:])
+        vertices = data["vertices"][:, :3]
+        idx = data["indices"][slots, :3].astype(np.int64)
+        a = vertices[idx[:, 0]]
+        entries = np.zeros((len(slots), 12), dtype=np.float32)
+        entries[:, 0:3] = a
+        entries[:, 3] = slots.astype(np.uint32).view(np.float32)
+        entries[:, 4:7] = vertices[idx[:, 1]] - a
+        entries[:, 8:11] = vertices[idx[:, 2]] - a
+        return cell_start, np.ascontiguousarray(entries)
 
     def upload(self, runtime: HipRuntime, with_boundary: bool = False) -> DeviceHandle:
         key = id(runtime)
@@ -387,6 +437,11 @@ class Mesh(BaseDomain):
             )
         )
         device = DeviceHandle(runtime, handle, lib.tgk_domain_destroy)
+        cell_start, entries = self.segment_grid()
+        _native.check(
+            lib.tgk_domain_set_segment_grid(handle, _native.ptr(cell_start), len(cell_start) - 1,
+                                            _native.ptr(entries), len(entries))
+        )
         device.has_boundary = True
         self._device[key] = device
         return device
