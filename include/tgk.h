@@ -169,6 +169,17 @@ int tgk_field_set_fast_precision(tgk_field *field, int32_t precision);
 
 /* -------------------------------------------------------------- seeds --- */
 
+/* Replaces the kernel of tracking.connectome (tracking.py:388-602):
+ * n_seeds boundary-seeded tracks of at most n_steps steps in the mask
+ * `domain`, seeded on and ended against `boundary` (a mesh domain over the
+ * domain's boundary surface, with cumulative areas and BVH); seed g uses
+ * rng_init(rng_seed, g) for its boundary sample and its noise. rows, cols,
+ * weights (n_seeds * 9): boundary-vertex pairs and barycentric weight
+ * products, row -1 where a track contributes nothing. */
+int tgk_connectome(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
+                   const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
+                   float minimum_length, int32_t *rows, int32_t *cols, float *weights);
+
 /* Replaces domain.seed (domain.py:494-575): n boundary seeds (n, 8). */
 int tgk_domain_seed(tgk_runtime *rt, const tgk_domain *domain, uint32_t rng_seed, int64_t n, float *out);
 

@@ -62,6 +62,52 @@ static inline float tgo_randu(uint32_t state[2]) { return (float)(tgo_mwc64x(sta
 /* core.cl:227-229 */
 static inline uint32_t tgo_randi(uint32_t state[2], uint32_t max) { return (uint32_t)(tgo_randu(state) * (float)max); }
 
+/* domain.py:131-166 (domain_sample_boundary): area-weighted triangle by
+ * binary search of the cumulative areas, uniform point from sqrt(randu) and
+ * randu, inward normal as orientation (normalize(v) restated as
+ * v / sqrt(dot(v, v)) per component), the point moved 1e-4 inward; the
+ * triangle and the barycentric weights of the sample are returned too. */
+static void tgo_sample_boundary(const float *vertices, const uint32_t *triangles, const float *cumulative,
+                                uint32_t n_triangles, uint32_t rng[2], float point[3], float orientation[3],
+                                uint32_t *triangle, float bary[3])
+{
+    const float target = tgo_randu(rng);
+    uint32_t lower = 0, upper = n_triangles;
+    while (lower < upper) {
+        const uint32_t middle = lower + (upper - lower) / 2;
+        if (cumulative[middle] <= target)
+            lower = middle + 1;
+        else
+            upper = middle;
+    }
+    const uint32_t tri = lower < n_triangles - 1 ? lower : n_triangles - 1;
+    const uint32_t *ix = triangles + 4 * (size_t)tri;
+    float a[3], b[3], c[3];
+    for (int i = 0; i < 3; i++) {
+        a[i] = vertices[4 * (size_t)ix[0] + i];
+        b[i] = vertices[4 * (size_t)ix[1] + i];
+        c[i] = vertices[4 * (size_t)ix[2] + i];
+    }
+    const float root_u = tgk_sqrtf(tgo_randu(rng));
+    const float v = tgo_randu(rng);
+    const float wa = 1.0f - root_u;
+    const float wb = root_u * (1.0f - v);
+    const float wc = root_u * v;
+    bary[0] = wa;
+    bary[1] = wb;
+    bary[2] = wc;
+    const float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    const float nrm[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const float len = tgk_sqrtf((nrm[0] * nrm[0] + nrm[1] * nrm[1]) + nrm[2] * nrm[2]);
+    for (int i = 0; i < 3; i++) {
+        const float sampled = (wa * a[i] + wb * b[i]) + wc * c[i];
+        orientation[i] = -(nrm[i] / len);
+        point[i] = sampled + 1e-4f * orientation[i];
+    }
+    *triangle = tri;
+}
+
 /* ------------------------------------------------------ FP32 instance --- */
 
 static inline void tgo_sincos32(float x, float *s, float *c) { tgk_sincosf(x, s, c); }
@@ -267,49 +313,20 @@ void tgo_bvh_segment_hit(const tgo_scene *s, const float *starts, const float *e
 
 /* ---------------------------------------------------- boundary seeding --- */
 
-/* domain.py:131-166 (domain_sample_boundary) driven as in domain.py:530-556.
- * normalize(v) is restated as v / sqrt(dot(v, v)) per component. */
+/* domain.py:494-575: boundary seed g from rng_init(rng_seed, g). */
 void tgo_domain_seed(const float *vertices, const uint32_t *triangles, const float *cumulative, uint32_t n_triangles,
                      uint32_t rng_seed, int64_t n, float *out)
 {
     for (int64_t g = 0; g < n; g++) {
         uint32_t rng[2];
         tgo_rng_init(rng_seed, (uint32_t)g, rng);
-        const float target = tgo_randu(rng);
-        uint32_t lower = 0, upper = n_triangles;
-        while (lower < upper) {
-            const uint32_t middle = lower + (upper - lower) / 2;
-            if (cumulative[middle] <= target)
-                lower = middle + 1;
-            else
-                upper = middle;
-        }
-        const uint32_t tri = lower < n_triangles - 1 ? lower : n_triangles - 1;
-        const uint32_t *ix = triangles + 4 * (size_t)tri;
-        float a[3], b[3], c[3];
-        for (int i = 0; i < 3; i++) {
-            a[i] = vertices[4 * (size_t)ix[0] + i];
-            b[i] = vertices[4 * (size_t)ix[1] + i];
-            c[i] = vertices[4 * (size_t)ix[2] + i];
-        }
-        const float root_u = tgk_sqrtf(tgo_randu(rng));
-        const float v = tgo_randu(rng);
-        const float wa = 1.0f - root_u;
-        const float wb = root_u * (1.0f - v);
-        const float wc = root_u * v;
-        float sampled[3];
-        for (int i = 0; i < 3; i++)
-            sampled[i] = (wa * a[i] + wb * b[i]) + wc * c[i];
-        const float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-        const float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
-        const float nrm[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
-                              e1[0] * e2[1] - e1[1] * e2[0]};
-        const float len = tgk_sqrtf((nrm[0] * nrm[0] + nrm[1] * nrm[1]) + nrm[2] * nrm[2]);
+        float point[3], orientation[3], bary[3];
+        uint32_t tri;
+        tgo_sample_boundary(vertices, triangles, cumulative, n_triangles, rng, point, orientation, &tri, bary);
         float *row = out + 8 * g;
         for (int i = 0; i < 3; i++) {
-            const float o = -(nrm[i] / len);
-            row[4 + i] = o;
-            row[i] = sampled[i] + 1e-4f * o;
+            row[i] = point[i];
+            row[4 + i] = orientation[i];
         }
         row[3] = 1.0f;
         row[7] = 0.0f;

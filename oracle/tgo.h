@@ -109,6 +109,15 @@ void tgo64_track(const tgo_scene *scene, const tgo_algorithm *algorithm, const f
                  const uint32_t *seed_rng, const uint32_t *seed_stream, uint32_t n_points, float save_at,
                  uint32_t *lengths, uint32_t *steps, uint32_t *hits, double *points, double *margins,
                  int n_threads);
+/* tracking.py:457-558 (kernel connectome): scene = mask domain with the
+ * boundary's vertices, triangles and BVH; b* = the boundary surface for
+ * sampling. rows, cols, weights: 9 per seed (row -1: no contribution). */
+void tgo32_connectome(const tgo_scene *scene, const tgo_algorithm *algorithm, const float *bvertices,
+                      const uint32_t *btriangles, const float *bcumulative, uint32_t bn, uint32_t rng_seed, int64_t n,
+                      uint32_t n_steps, float min_length, int32_t *rows, int32_t *cols, float *weights, int n_threads);
+void tgo64_connectome(const tgo_scene *scene, const tgo_algorithm *algorithm, const float *bvertices,
+                      const uint32_t *btriangles, const float *bcumulative, uint32_t bn, uint32_t rng_seed, int64_t n,
+                      uint32_t n_steps, float min_length, int32_t *rows, int32_t *cols, float *weights, int n_threads);
 
 /* Boundary seeding (domain.py:131-166, 530-556): out (n, 8). */
 void tgo_domain_seed(const float *vertices, const uint32_t *triangles, const float *cumulative,

@@ -428,7 +428,7 @@ static inline int FN(bvh_bounds)(const tgo_scene *s, uint32_t node, const R star
 
 /* bvh.cl:48-80 (Moller-Trumbore) */
 static inline int FN(bvh_triangle)(const tgo_scene *s, uint32_t tri, const R start[3], const R dir[3], R max_t,
-                                   R *hit_t)
+                                   R *hit_t, R *hit_u, R *hit_v)
 {
     const uint32_t *ix = s->triangles + 4 * (size_t)tri;
     R a[3], b[3], c[3];
@@ -456,12 +456,15 @@ static inline int FN(bvh_triangle)(const tgo_scene *s, uint32_t tri, const R sta
     if (t <= K(1e-6) || t > max_t)
         return 0;
     *hit_t = t;
+    *hit_u = u;
+    *hit_v = v;
     return 1;
 }
 
 /* bvh.cl:83-131: closest hit with t in (1e-6, 1]; a later-visited triangle
  * at an equal t replaces the earlier one; left child visited first. */
-static inline int FN(bvh_hit)(const tgo_scene *s, VEC start, VEC end, uint32_t *hit, R *hit_t)
+/* bvh.cl:83-131; bary receives (1 - u - v, u, v) of the closest hit. */
+static inline int FN(bvh_hit_bary)(const tgo_scene *s, VEC start, VEC end, uint32_t *hit, R *hit_t, R bary[3])
 {
     const R st[3] = {start.x, start.y, start.z};
     const R dir[3] = {end.x - start.x, end.y - start.y, end.z - start.z};
@@ -479,10 +482,13 @@ static inline int FN(bvh_hit)(const tgo_scene *s, VEC start, VEC end, uint32_t *
         if (nd[3] > 0) {
             for (int i = 0; i < nd[3]; i++) {
                 const uint32_t tri = s->bvh_order[nd[2] + i];
-                R t;
-                if (FN(bvh_triangle)(s, tri, st, dir, *hit_t, &t)) {
+                R t, u, v;
+                if (FN(bvh_triangle)(s, tri, st, dir, *hit_t, &t, &u, &v)) {
                     *hit_t = t;
                     *hit = tri;
+                    bary[0] = K(1.0) - u - v;
+                    bary[1] = u;
+                    bary[2] = v;
                     found = 1;
                 }
             }
@@ -494,6 +500,12 @@ static inline int FN(bvh_hit)(const tgo_scene *s, VEC start, VEC end, uint32_t *
         }
     }
     return found;
+}
+
+static inline int FN(bvh_hit)(const tgo_scene *s, VEC start, VEC end, uint32_t *hit, R *hit_t)
+{
+    R bary[3];
+    return FN(bvh_hit_bary)(s, start, end, hit, hit_t, bary);
 }
 
 /* tracking.py:112-148 for one seed. Mesh domains: the loop's domain test is
@@ -509,7 +521,7 @@ static void FN(track_one)(const tgo_scene *s, const tgo_algorithm *a, const floa
     uint32_t rng[2];
     tgo_rng_init(rng_seed, stream_id, rng);
     const int K_ = a->n_parameters;
-    R params[128];
+    R params[160];
     const R step = (R)a->step_size;
     const R save = (R)save_at;
     R margin = K(0.5);
@@ -570,6 +582,73 @@ void FN(track)(const tgo_scene *s, const tgo_algorithm *a, const float *seeds, i
     for (int64_t g = 0; g < n; g++)
         FN(track_one)(s, a, seeds + 8 * g, seed_rng[g], seed_stream[g], n_points, save_at, lengths + g, steps + g,
                       hits + g, points + (size_t)3 * n_points * (size_t)g, TRACK_MARGINS(g));
+}
+
+/* tracking.py:457-558 (kernel connectome) for seeds 0..n-1: the boundary
+ * sample of rng_init(rng_seed, g), whose state then drives the noise; mask,
+ * field, update, orientation check (finite, length >= 0.5), move; then the
+ * last segment against the boundary BVH. rows/cols/weights: 9 per seed, row
+ * -1 when the track contributes nothing. */
+void FN(connectome)(const tgo_scene *s, const tgo_algorithm *a, const float *bvertices, const uint32_t *btriangles,
+                    const float *bcumulative, uint32_t bn, uint32_t rng_seed, int64_t n, uint32_t n_steps,
+                    float min_length, int32_t *rows, int32_t *cols, float *weights, int n_threads)
+{
+    const int K_ = a->n_parameters;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(n_threads > 0 ? n_threads : 1)
+    for (int64_t g = 0; g < n; g++) {
+        int32_t *row = rows + 9 * g;
+        int32_t *col = cols + 9 * g;
+        float *w = weights + 9 * g;
+        for (int i = 0; i < 9; i++) {
+            row[i] = -1;
+            col[i] = -1;
+            w[i] = 0.0f;
+        }
+        uint32_t rng[2];
+        tgo_rng_init(rng_seed, (uint32_t)g, rng);
+        float p0[3], o0[3], sbary[3];
+        uint32_t stri;
+        tgo_sample_boundary(bvertices, btriangles, bcumulative, bn, rng, p0, o0, &stri, sbary);
+        VEC point = FN(mkvec)((R)p0[0], (R)p0[1], (R)p0[2]);
+        VEC orientation = FN(mkvec)((R)o0[0], (R)o0[1], (R)o0[2]);
+        VEC previous = point;
+        const R step = (R)a->step_size;
+        R params[160];
+        uint32_t tracked = 0;
+        for (uint32_t k = 0; k < n_steps; k++) {
+            if (!FN(mask_contains)(s, point, 0))
+                break;
+            int64_t base;
+            if (!FN(field_index)(s, point, &base, 0))
+                break;
+            for (int i = 0; i < K_; i++)
+                params[i] = (R)s->field[base + i];
+            orientation = FN(update_one)(a, params, orientation, rng);
+            if (!(isfinite(orientation.x) && isfinite(orientation.y) && isfinite(orientation.z)) ||
+                SQRT((orientation.x * orientation.x + orientation.y * orientation.y) +
+                     orientation.z * orientation.z) < K(0.5))
+                break;
+            previous = point;
+            point.x += step * orientation.x;
+            point.y += step * orientation.y;
+            point.z += step * orientation.z;
+            tracked++;
+        }
+        if ((R)tracked * step < (R)min_length)
+            continue;
+        uint32_t htri;
+        R t, hbary[3];
+        if (!FN(bvh_hit_bary)(s, previous, point, &htri, &t, hbary))
+            continue;
+        const uint32_t *si = btriangles + 4 * (size_t)stri;
+        const uint32_t *ei = s->triangles + 4 * (size_t)htri;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                row[3 * i + j] = (int32_t)si[i];
+                col[3 * i + j] = (int32_t)ei[j];
+                w[3 * i + j] = (float)((R)sbary[i] * hbary[j]);
+            }
+    }
 }
 
 void FN(sh45)(const R *phi, const R *theta, int64_t n, R *ylm, R *dp, R *dt)

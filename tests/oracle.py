@@ -70,11 +70,23 @@ def algorithm_struct(algorithm) -> Algorithm:
 class SceneHolder:
     """A tgo_scene over a domain and field, keeping its arrays alive."""
 
-    def __init__(self, domain, field):
+    def __init__(self, domain, field, boundary=False):
         from tractography_amd import domain as domain_module
 
         self.keep = []
         s = Scene()
+        if boundary and not isinstance(domain, domain_module.Mesh):
+            # A mask domain whose boundary surface and BVH are also needed.
+            data = domain._ensure_boundary_data()
+            for name in ("vertices", "indices", "bvh_min", "bvh_max", "bvh_nodes", "bvh_triangles"):
+                self.keep.append(data[name])
+            s.vertices = data["vertices"].ctypes.data
+            s.triangles = data["indices"].ctypes.data
+            s.bvh_min = data["bvh_min"].ctypes.data
+            s.bvh_max = data["bvh_max"].ctypes.data
+            s.bvh_nodes = data["bvh_nodes"].ctypes.data
+            s.bvh_order = data["bvh_triangles"].ctypes.data
+            s.n_bvh_nodes = len(data["bvh_nodes"])
         if isinstance(domain, domain_module.Mesh):
             s.domain_kind = 1
             data = domain._ensure_boundary_data()
@@ -147,6 +159,22 @@ def track(domain, field, algorithm, seeds, rng_seeds, batch_size, save_at, max_l
                       _p(seed_stream), c_uint32(n_points), c_float(save), _p(lengths), _p(steps), _p(hits),
                       _p(points), _p(margins), c_int(threads))
     return {"lengths": lengths, "steps": steps, "hits": hits, "points": points, "margins": margins}
+
+
+def connectome(domain, field, algorithm, rng_seed, n_seeds, n_steps, min_length=0.0, *, precision=32,
+               threads=8):
+    """rows, cols, weights (9 per seed) of the connectome kernel."""
+    holder = SceneHolder(domain, field, boundary=True)
+    data = domain._ensure_boundary_data()
+    alg = algorithm_struct(algorithm)
+    rows = np.zeros(9 * n_seeds, np.int32)
+    cols = np.zeros(9 * n_seeds, np.int32)
+    weights = np.zeros(9 * n_seeds, np.float32)
+    fn = lib().tgo32_connectome if precision == 32 else lib().tgo64_connectome
+    fn(ctypes.byref(holder.scene), ctypes.byref(alg), _p(data["vertices"]), _p(data["indices"]),
+       _p(data["cumulative"]), c_uint32(len(data["indices"])), c_uint32(rng_seed), c_int64(n_seeds),
+       c_uint32(n_steps), c_float(min_length), _p(rows), _p(cols), _p(weights), c_int(threads))
+    return rows, cols, weights
 
 
 def ragged(result):

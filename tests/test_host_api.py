@@ -126,3 +126,32 @@ def test_streamlines_container():
     np.testing.assert_array_equal(s[1], np.arange(3, 12).reshape(3, 3))
     assert [len(x) for x in s] == [1, 3, 2]
     assert len(tg.Tractogram([np.zeros((2, 3))])) == 1
+
+
+def test_connectome_zero_seeds_returns_empty_csr_without_a_device():
+    # test/test_tracking.py:212-224
+    from scipy.sparse import csr_matrix
+
+    domain = tg.domain.Image(tg.Nifti1Image(np.ones((1, 1, 1), dtype=np.uint8), np.eye(4)))
+    field = tg.field.Image(tg.Nifti1Image(np.zeros((1, 1, 1, 15)), np.eye(4)))
+    algorithm = tg.algorithm.Transport(tg.local.RealSymmetricSphericalHarmonics(15))
+    matrix = tg.tracking.connectome(algorithm, domain, field, n_seeds=0)
+    assert isinstance(matrix, csr_matrix)
+    assert matrix.shape == (len(domain.boundary.vertices),) * 2
+    assert matrix.nnz == 0
+
+
+def test_connectome_argument_errors():
+    domain = tg.domain.Image(tg.Nifti1Image(np.ones((1, 1, 1), dtype=np.uint8), np.eye(4)))
+    field = tg.field.Image(tg.Nifti1Image(np.zeros((1, 1, 1, 15)), np.eye(4)))
+    algorithm = tg.algorithm.Transport(tg.local.RealSymmetricSphericalHarmonics(15))
+    with pytest.raises(ValueError, match="n_seeds"):
+        tg.tracking.connectome(algorithm, domain, field, n_seeds=-1)
+    with pytest.raises(ValueError, match="must not exceed"):
+        tg.tracking.connectome(algorithm, domain, field, n_seeds=1, maximum_streamline_length=1.0,
+                               minimum_streamline_length=2.0)
+    wrong = tg.algorithm.Transport(tg.local.RealSymmetricSphericalHarmonics(45))
+    with pytest.raises(ValueError, match="15 != 45|45 != 15"):
+        tg.tracking.connectome(wrong, domain, field, n_seeds=1)
+    with pytest.raises(TypeError):
+        tg.connectome(np.zeros((2, 2, 2, 15)), n_seeds=1)

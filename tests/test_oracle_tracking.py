@@ -209,3 +209,42 @@ def test_mesh_domain_straight_rays_stop_at_the_first_triangle():
         rr = np.linalg.norm(p, axis=1)
         r_in, r_out = radii(p / rr[:, None])
         assert np.all(rr > r_in - 0.11) and np.all(rr < r_out + 0.11)
+
+
+def test_connectome_restatement_known_answers():
+    # test/test_tracking.py:183-209: a one-voxel domain with an isotropic
+    # field and gamma 0: every track goes straight from a boundary sample to
+    # the opposite face, and each accepted track deposits total weight one.
+    domain = tg.domain.Image(tg.Nifti1Image(np.ones((1, 1, 1), dtype=np.uint8), np.eye(4)))
+    field_data = np.zeros((1, 1, 1, 15), dtype=np.float32)
+    field_data[..., 0] = 1.0
+    field = tg.field.Image(tg.Nifti1Image(field_data, np.eye(4)))
+    algorithm = tg.algorithm.Transport(tg.local.RealSymmetricSphericalHarmonics(15), step_size=0.1, gamma=0.0)
+    rows, cols, weights = oracle.connectome(domain, field, algorithm, 0, 64, 20)
+    per_seed = weights.reshape(64, 9).sum(axis=1)
+    accepted = rows.reshape(64, 9)[:, 0] >= 0
+    assert accepted.mean() > 0.9
+    np.testing.assert_allclose(per_seed[accepted], 1.0, atol=1e-6)
+    assert np.all(weights.reshape(64, 9)[~accepted] == 0)
+    # Straight inward rays through a unit cube end on the face opposite
+    # their start: the start and hit triangles never share a face plane.
+    vertices = domain.boundary.vertices
+    r = rows.reshape(64, 9)[accepted]
+    c = cols.reshape(64, 9)[accepted]
+    start_centre = vertices[r[:, [0, 3, 6]]].mean(axis=1)
+    end_centre = vertices[c[:, :3]].mean(axis=1)
+    assert np.all(np.abs(start_centre - end_centre).max(axis=1) > 0.5)
+
+
+def test_connectome_restatement_fp32_and_fp64_agree():
+    fod, wm, _ = fixtures.cross()
+    domain = tg.domain.Image(wm)
+    field = tg.field.Image(fod)
+    algorithm = tg.algorithm.Diffusion(tg.local.RealSymmetricSphericalHarmonics(45), step_size=0.25, gamma=0.5,
+                                       noise_variance=0.05, seed=3)
+    seed = algorithm.get_rng_seed()
+    r32 = oracle.connectome(domain, field, algorithm, seed, 2000, 48)
+    r64 = oracle.connectome(domain, field, algorithm, seed, 2000, 48, precision=64)
+    same = np.all(r32[0].reshape(-1, 9) == r64[0].reshape(-1, 9), axis=1)
+    assert same.mean() > 0.95
+    np.testing.assert_allclose(r32[2].reshape(-1, 9)[same], r64[2].reshape(-1, 9)[same], atol=1e-4)

@@ -1029,6 +1029,59 @@ int tgk_math(tgk_runtime *rt, int32_t fn, const float *x, const float *y, int64_
     return fetch(out, dout, (size_t)n, rt->stream);
 }
 
+int tgk_connectome(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
+                   const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
+                   float minimum_length, int32_t *rows, int32_t *cols, float *weights)
+{
+    if (!rt || !domain || !boundary || !field || n_seeds < 0 || (n_seeds > 0 && (!rows || !cols || !weights)))
+        return fail(TGK_ERR_INVALID, "bad arguments");
+    if (int r = check_algorithm(algorithm, field))
+        return r;
+    if (domain->kind != tgk::DOMAIN_MASK)
+        return fail(TGK_ERR_INVALID, "the connectome's domain must be a mask domain");
+    if (boundary->kind != tgk::DOMAIN_MESH || boundary->n_nodes == 0 || !boundary->cumulative.ptr ||
+        boundary->n_triangles == 0)
+        return fail(TGK_ERR_INVALID, "the boundary needs triangles, cumulative areas and a BVH");
+    if (!(algorithm->step_size > 0.0f) || !std::isfinite(algorithm->step_size))
+        return fail(TGK_ERR_INVALID, "step_size must be finite and positive");
+    if (!(minimum_length >= 0.0f) || !std::isfinite(minimum_length))
+        return fail(TGK_ERR_INVALID, "minimum_length must be finite and non-negative");
+    if (n_seeds == 0)
+        return TGK_OK;
+    if (int r = select_device(rt))
+        return r;
+    Staging st;
+    int32_t *drows, *dcols;
+    float *dweights;
+    if (int r = st.out((size_t)n_seeds * 9, &drows))
+        return r;
+    if (int r = st.out((size_t)n_seeds * 9, &dcols))
+        return r;
+    if (int r = st.out((size_t)n_seeds * 9, &dweights))
+        return r;
+    tgk::ConnectomeLaunch c;
+    memset(&c, 0, sizeof c);
+    c.mask = mask_view(domain);
+    c.field = field_view(field);
+    c.boundary = mesh_view(boundary);
+    c.cumulative = boundary->cumulative.as<const float>();
+    c.n_triangles = boundary->n_triangles;
+    c.algo = algo_params(algorithm);
+    c.rng_seed = rng_seed;
+    c.n_seeds = n_seeds;
+    c.n_steps = n_steps;
+    c.min_length = minimum_length;
+    c.rows = drows;
+    c.cols = dcols;
+    c.weights = dweights;
+    TGK_HIP(tgk::launch_connectome(c, rt->stream), "connectome kernel");
+    if (int r = fetch(rows, drows, (size_t)n_seeds * 9, rt->stream))
+        return r;
+    if (int r = fetch(cols, dcols, (size_t)n_seeds * 9, rt->stream))
+        return r;
+    return fetch(weights, dweights, (size_t)n_seeds * 9, rt->stream);
+}
+
 int tgk_domain_seed(tgk_runtime *rt, const tgk_domain *domain, uint32_t rng_seed, int64_t n, float *out)
 {
     if (!rt || !domain || n < 0 || (n > 0 && !out))

@@ -184,6 +184,53 @@ __device__ __forceinline__ int64_t field_voxel(const FieldView &f, V3 p)
     return ((int64_t)(uint32_t)rx * f.dims[1] + (uint32_t)ry) * f.dims[2] + (uint32_t)rz;
 }
 
+// ------------------------------------------------------- boundary -------
+// domain_sample_boundary (domain.py:131-166): an area-weighted triangle by
+// binary search of the cumulative areas, a uniform point on it from
+// sqrt(randu) and randu, the inward normal as orientation, the point moved
+// 1e-4 inward. normalize(v) is v / sqrt(dot(v, v)) per component.
+struct BoundarySample {
+    V3 point, orientation;
+    uint32_t triangle;
+    float bary[3];
+};
+
+__device__ __forceinline__ BoundarySample sample_boundary(const float4 *__restrict__ vertices,
+                                                          const uint4 *__restrict__ triangles,
+                                                          const float *__restrict__ cumulative, uint32_t n_triangles,
+                                                          uint2 &rng)
+{
+    const float target = randu(rng);
+    uint32_t lower = 0, upper = n_triangles;
+    while (lower < upper) {
+        const uint32_t middle = lower + (upper - lower) / 2;
+        if (cumulative[middle] <= target)
+            lower = middle + 1;
+        else
+            upper = middle;
+    }
+    BoundarySample s;
+    s.triangle = lower < n_triangles - 1 ? lower : n_triangles - 1;
+    const uint4 ix = triangles[s.triangle];
+    const float4 a = vertices[ix.x], b = vertices[ix.y], c = vertices[ix.z];
+    const float root_u = tgk_sqrtf(randu(rng));
+    const float v = randu(rng);
+    const float wa = 1.0f - root_u, wb = root_u * (1.0f - v), wc = root_u * v;
+    s.bary[0] = wa;
+    s.bary[1] = wb;
+    s.bary[2] = wc;
+    const float sx = (wa * a.x + wb * b.x) + wc * c.x;
+    const float sy = (wa * a.y + wb * b.y) + wc * c.y;
+    const float sz = (wa * a.z + wb * b.z) + wc * c.z;
+    const float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
+    const float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
+    const float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+    const float len = tgk_sqrtf((nx * nx + ny * ny) + nz * nz);
+    s.orientation = v3(-(nx / len), -(ny / len), -(nz / len));
+    s.point = v3(sx + 1e-4f * s.orientation.x, sy + 1e-4f * s.orientation.y, sz + 1e-4f * s.orientation.z);
+    return s;
+}
+
 // ------------------------------------------------------------ BVH -------
 // Closest triangle hit on the segment start -> end with t in (1e-6, 1]
 // (bvh.cl:16-131): slab test per node, Moller-Trumbore per triangle, stack of
@@ -222,7 +269,8 @@ __device__ __forceinline__ bool bvh_node_hit(const MeshView &m, uint32_t node, c
 }
 
 __device__ __forceinline__ bool bvh_triangle_hit(const MeshView &m, uint32_t tri, const float st[3],
-                                                 const float dir[3], float max_t, float &hit_t)
+                                                 const float dir[3], float max_t, float &hit_t, float &hit_u,
+                                                 float &hit_v)
 {
     const uint4 ix = m.triangles[tri];
     const float4 a4 = m.vertices[ix.x];
@@ -249,10 +297,15 @@ __device__ __forceinline__ bool bvh_triangle_hit(const MeshView &m, uint32_t tri
     if (t <= 1e-6f || t > max_t)
         return false;
     hit_t = t;
+    hit_u = u;
+    hit_v = v;
     return true;
 }
 
-__device__ inline bool bvh_segment_hit(const MeshView &m, V3 start, V3 end, uint32_t &hit, float &hit_t)
+// Closest hit with the barycentric coordinates (1 - u - v, u, v) of the hit
+// point on the hit triangle (bvh.cl:75-78).
+__device__ inline bool bvh_segment_hit_bary(const MeshView &m, V3 start, V3 end, uint32_t &hit, float &hit_t,
+                                            float bary[3])
 {
     const float st[3] = {start.x, start.y, start.z};
     const float dir[3] = {end.x - start.x, end.y - start.y, end.z - start.z};
@@ -270,10 +323,13 @@ __device__ inline bool bvh_segment_hit(const MeshView &m, V3 start, V3 end, uint
         if (nd.w > 0) {
             for (int i = 0; i < nd.w; i++) {
                 const uint32_t tri = m.bvh_order[nd.z + i];
-                float t;
-                if (bvh_triangle_hit(m, tri, st, dir, hit_t, t)) {
+                float t, u, v;
+                if (bvh_triangle_hit(m, tri, st, dir, hit_t, t, u, v)) {
                     hit_t = t;
                     hit = tri;
+                    bary[0] = 1.0f - u - v;
+                    bary[1] = u;
+                    bary[2] = v;
                     found = true;
                 }
             }
@@ -285,6 +341,12 @@ __device__ inline bool bvh_segment_hit(const MeshView &m, V3 start, V3 end, uint
         }
     }
     return found;
+}
+
+__device__ inline bool bvh_segment_hit(const MeshView &m, V3 start, V3 end, uint32_t &hit, float &hit_t)
+{
+    float bary[3];
+    return bvh_segment_hit_bary(m, start, end, hit, hit_t, bary);
 }
 
 // The culling-grid cell of p, or -1 outside the grid.

@@ -15,6 +15,25 @@ hipError_t launch_compact_point_major(const float *dense, int64_t n_seeds, uint3
                                       const uint32_t *lengths, const int64_t *offsets, float *out,
                                       hipStream_t stream);
 
+// Boundary connectome (tgk_plain.hip, plain arithmetic): a mask domain, its
+// boundary surface with BVH, 9 (row, col, weight) entries per seed.
+struct ConnectomeLaunch {
+    MaskView mask;
+    FieldView field;
+    MeshView boundary;      // boundary triangles and their BVH
+    const float *cumulative;
+    uint32_t n_triangles;
+    AlgoParams algo;
+    uint32_t rng_seed;
+    int64_t n_seeds;
+    uint32_t n_steps;
+    float min_length;
+    int32_t *rows;
+    int32_t *cols;
+    float *weights;
+};
+hipError_t launch_connectome(const ConnectomeLaunch &c, hipStream_t stream);
+
 // Fast path (tgk_fast.hip): persistent lanes with refill, seed-major output.
 struct FastLaunch {
     TrackLaunch track;

@@ -320,38 +320,77 @@ __global__ void domain_seed_kernel(const float4 *__restrict__ vertices, const ui
     if (g >= n)
         return;
     uint2 rng = rng_init(rng_seed, (uint32_t)g);
-    const float target = randu(rng);
-    uint32_t lower = 0, upper = n_triangles;
-    while (lower < upper) {
-        const uint32_t middle = lower + (upper - lower) / 2;
-        if (cumulative[middle] <= target)
-            lower = middle + 1;
-        else
-            upper = middle;
-    }
-    const uint32_t tri = lower < n_triangles - 1 ? lower : n_triangles - 1;
-    const uint4 ix = triangles[tri];
-    const float4 a = vertices[ix.x], b = vertices[ix.y], c = vertices[ix.z];
-    const float root_u = tgk_sqrtf(randu(rng));
-    const float v = randu(rng);
-    const float wa = 1.0f - root_u, wb = root_u * (1.0f - v), wc = root_u * v;
-    const float sx = (wa * a.x + wb * b.x) + wc * c.x;
-    const float sy = (wa * a.y + wb * b.y) + wc * c.y;
-    const float sz = (wa * a.z + wb * b.z) + wc * c.z;
-    const float e1x = b.x - a.x, e1y = b.y - a.y, e1z = b.z - a.z;
-    const float e2x = c.x - a.x, e2y = c.y - a.y, e2z = c.z - a.z;
-    const float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
-    const float len = tgk_sqrtf((nx * nx + ny * ny) + nz * nz);
-    const float ox = -(nx / len), oy = -(ny / len), oz = -(nz / len);
+    const BoundarySample s = sample_boundary(vertices, triangles, cumulative, n_triangles, rng);
     float *row = out + 8 * g;
-    row[0] = sx + 1e-4f * ox;
-    row[1] = sy + 1e-4f * oy;
-    row[2] = sz + 1e-4f * oz;
+    row[0] = s.point.x;
+    row[1] = s.point.y;
+    row[2] = s.point.z;
     row[3] = 1.0f;
-    row[4] = ox;
-    row[5] = oy;
-    row[6] = oz;
+    row[4] = s.orientation.x;
+    row[5] = s.orientation.y;
+    row[6] = s.orientation.z;
     row[7] = 0.0f;
+}
+
+// tracking.py:457-558 (kernel connectome): boundary seed g from
+// rng_init(rng_seed, g), whose state then drives the noise; the loop checks
+// the mask, then the field, updates, rejects a non-finite or short
+// orientation, and moves; afterwards the last segment is tested against the
+// boundary BVH and a hit deposits the outer product of the start and hit
+// barycentric weights on the two triangles' vertices (9 entries per seed,
+// row -1 when none).
+template <int MODEL, int ALGO, int KT>
+__global__ __launch_bounds__(256) void connectome_kernel(ConnectomeLaunch c)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= c.n_seeds)
+        return;
+    int32_t *rows = c.rows + 9 * g;
+    int32_t *cols = c.cols + 9 * g;
+    float *weights = c.weights + 9 * g;
+    for (int i = 0; i < 9; i++) {
+        rows[i] = -1;
+        cols[i] = -1;
+        weights[i] = 0.0f;
+    }
+    uint2 rng = rng_init(c.rng_seed, (uint32_t)g);
+    const BoundarySample s = sample_boundary(c.boundary.vertices, c.boundary.triangles, c.cumulative,
+                                             c.n_triangles, rng);
+    V3 point = s.point, o = s.orientation, previous = point;
+    const float step = c.algo.step_size;
+    const uint32_t stride = c.field.stride;
+    uint32_t tracked = 0;
+    for (uint32_t k = 0; k < c.n_steps; k++) {
+        if (!mask_contains(c.mask, point))
+            break;
+        const int64_t voxel = field_voxel(c.field, point);
+        if (voxel < 0)
+            break;
+        o = plain::update<MODEL, ALGO, KT>(c.algo, c.field.data + voxel * stride, o, rng);
+        if (!(__builtin_isfinite(o.x) && __builtin_isfinite(o.y) && __builtin_isfinite(o.z)) || length3(o) < 0.5f)
+            break;
+        previous = point;
+        point.x += step * o.x;
+        point.y += step * o.y;
+        point.z += step * o.z;
+        tracked++;
+    }
+    if ((float)tracked * step < c.min_length)
+        return;
+    uint32_t hit;
+    float t, bary[3];
+    if (!bvh_segment_hit_bary(c.boundary, previous, point, hit, t, bary))
+        return;
+    const uint4 si = c.boundary.triangles[s.triangle];
+    const uint4 ei = c.boundary.triangles[hit];
+    const uint32_t sv[3] = {si.x, si.y, si.z};
+    const uint32_t ev[3] = {ei.x, ei.y, ei.z};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            rows[3 * i + j] = (int32_t)sv[i];
+            cols[3 * i + j] = (int32_t)ev[j];
+            weights[3 * i + j] = s.bary[i] * bary[j];
+        }
 }
 
 inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
@@ -442,6 +481,28 @@ hipError_t launch_math(int fn, const float *x, const float *y, int64_t n, float 
         return hipSuccess;
     math_kernel<<<blocks_for(n, 256), 256, 0, stream>>>(fn, x, y, n, out);
     return hipGetLastError();
+}
+
+template <int MODEL, int ALGO>
+hipError_t launch_connectome_kt(const ConnectomeLaunch &c, hipStream_t stream)
+{
+    const unsigned blocks = blocks_for(c.n_seeds, 256);
+    if (MODEL == MODEL_SH && c.algo.n_parameters == 45)
+        connectome_kernel<MODEL, ALGO, 45><<<blocks, 256, 0, stream>>>(c);
+    else
+        connectome_kernel<MODEL, ALGO, 0><<<blocks, 256, 0, stream>>>(c);
+    return hipGetLastError();
+}
+
+hipError_t launch_connectome(const ConnectomeLaunch &c, hipStream_t stream)
+{
+    if (c.n_seeds == 0)
+        return hipSuccess;
+    if (c.algo.model == MODEL_DTI)
+        return c.algo.algorithm == ALGO_DIFFUSION ? launch_connectome_kt<MODEL_DTI, ALGO_DIFFUSION>(c, stream)
+                                                  : launch_connectome_kt<MODEL_DTI, ALGO_TRANSPORT>(c, stream);
+    return c.algo.algorithm == ALGO_DIFFUSION ? launch_connectome_kt<MODEL_SH, ALGO_DIFFUSION>(c, stream)
+                                              : launch_connectome_kt<MODEL_SH, ALGO_TRANSPORT>(c, stream);
 }
 
 hipError_t launch_domain_seed(const float4 *vertices, const uint4 *triangles, const float *cumulative,

@@ -140,3 +140,78 @@ def tractogram(
         runtime,
     )
     return Tractogram(result.streamlines(), affine_to_rasmm=np.eye(4))
+
+
+def connectome(
+    algorithm: BaseAlgorithm,
+    domain: BaseDomain,
+    field: BaseField,
+    n_seeds: int,
+    maximum_streamline_length: float = 300.0,
+    minimum_streamline_length: float = 0.0,
+    runtime: HipRuntime | None = None,
+):
+    """Asymmetric, piecewise-linear boundary connectome (tracking.py:388-602).
+
+    Each track starts at a uniform sample of the domain's boundary surface
+    (directed inward) and runs for at most floor(maximum / step_size) steps
+    while the domain contains its point and the field is defined. Its last
+    segment is then tested against the boundary; a hit deposits the outer
+    product of the start and hit barycentric weights onto the vertices of
+    the two triangles, so an accepted track adds total weight one. Tracks
+    shorter than ``minimum_streamline_length`` or without a hit add nothing.
+
+    Returns a float32 (n_vertices, n_vertices) scipy CSR matrix over
+    ``domain.boundary.vertices``.
+    """
+    from scipy.sparse import coo_matrix, csr_matrix
+
+    if algorithm.local_model.n_parameters != field.n_parameters:
+        raise ValueError(
+            "Incompatible local model and field, they do not have the "
+            "same number of parameters: "
+            f"{algorithm.local_model.n_parameters} != {field.n_parameters}"
+        )
+    if not isinstance(n_seeds, (int, np.integer)) or n_seeds < 0:
+        raise ValueError("n_seeds must be a non-negative integer.")
+    if not np.isfinite(maximum_streamline_length) or maximum_streamline_length < 0:
+        raise ValueError("maximum_streamline_length must be finite and non-negative.")
+    if not np.isfinite(minimum_streamline_length) or minimum_streamline_length < 0:
+        raise ValueError("minimum_streamline_length must be finite and non-negative.")
+    if minimum_streamline_length > maximum_streamline_length:
+        raise ValueError("minimum_streamline_length must not exceed maximum_streamline_length.")
+
+    n_vertices = len(domain.boundary.vertices)
+    if n_seeds == 0 or n_vertices == 0:
+        return csr_matrix((n_vertices, n_vertices), dtype=np.float32)
+    step_size = float(algorithm.step_size)
+    if not np.isfinite(step_size) or step_size <= 0:
+        raise ValueError("algorithm.step_size must be finite and positive.")
+    n_steps = int(np.floor(maximum_streamline_length / step_size))
+    if n_steps == 0:
+        return csr_matrix((n_vertices, n_vertices), dtype=np.float32)
+
+    from .domain import _mesh_handle_for
+
+    if runtime is None:
+        runtime = HipRuntime()
+    domain_handle = domain.upload(runtime)
+    boundary_handle = _mesh_handle_for(domain, runtime)
+    field_handle = field.upload(runtime)
+    rows = np.empty(9 * n_seeds, dtype=np.int32)
+    cols = np.empty(9 * n_seeds, dtype=np.int32)
+    weights = np.empty(9 * n_seeds, dtype=np.float32)
+    _native.check(
+        _native.lib().tgk_connectome(
+            runtime.handle, domain_handle.handle, boundary_handle.handle, field_handle.handle,
+            algorithm.descriptor(), int(algorithm.get_rng_seed()), int(n_seeds), n_steps,
+            float(np.float32(minimum_streamline_length)), _native.ptr(rows), _native.ptr(cols), _native.ptr(weights),
+        )
+    )
+    valid = rows >= 0
+    if not np.any(valid):
+        return csr_matrix((n_vertices, n_vertices), dtype=np.float32)
+    matrix = coo_matrix((weights[valid], (rows[valid], cols[valid])), shape=(n_vertices, n_vertices),
+                        dtype=np.float32).tocsr()
+    matrix.sum_duplicates()
+    return matrix
