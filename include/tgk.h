@@ -180,6 +180,33 @@ int tgk_connectome(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *
                    const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
                    float minimum_length, int32_t *rows, int32_t *cols, float *weights);
 
+/* Replaces the kernel of tracking.tod (tracking.py:181-385): n_seeds
+ * boundary-seeded tracks of at most n_steps steps in the mask `domain`,
+ * seeded on `boundary` (any domain handle that carries the boundary
+ * triangles and cumulative areas, e.g. the mask domain itself); seed g uses
+ * rng_init(rng_seed, g). At every step inside the domain and the field, the
+ * 45-term SH basis of the current orientation is added to the output cell
+ * floor(output_inverse_affine * p + 0.5) when that cell is on the
+ * (X, Y, Z) = output_dims grid. coefficients: (X, Y, Z, 45) float32, C order;
+ * with normalize != 0 each cell is scaled as utils.normalize_odf
+ * (utils.py:6-22) does. The sums are float atomics, so their rounding depends
+ * on the order in which tracks reach a cell, as the reference's do. */
+int tgk_tod(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
+            const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
+            const uint32_t output_dims[3], const float output_inverse_affine[16], int32_t normalize,
+            float *coefficients);
+/* The same on the runtime stream, into device memory the caller owns
+ * (X * Y * Z * 45 floats, cleared by the call). No allocation, so it may be
+ * recorded into a graph. */
+int tgk_tod_device(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
+                   const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
+                   const uint32_t output_dims[3], const float output_inverse_affine[16], int32_t normalize,
+                   float *d_coefficients);
+/* Replaces utils.normalize_odf (utils.py:6-22) on n_cells rows of 45 SH
+ * coefficients, in place: row / sqrt(4 pi) / row[0] (float64 arithmetic,
+ * one rounding to float32), rows with row[0] == 0 set to 0. */
+int tgk_normalize_odf(tgk_runtime *rt, int64_t n_cells, float *coefficients);
+
 /* Replaces domain.seed (domain.py:494-575): n boundary seeds (n, 8). */
 int tgk_domain_seed(tgk_runtime *rt, const tgk_domain *domain, uint32_t rng_seed, int64_t n, float *out);
 

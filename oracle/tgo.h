@@ -119,6 +119,25 @@ void tgo64_connectome(const tgo_scene *scene, const tgo_algorithm *algorithm, co
                       const uint32_t *btriangles, const float *bcumulative, uint32_t bn, uint32_t rng_seed, int64_t n,
                       uint32_t n_steps, float min_length, int32_t *rows, int32_t *cols, float *weights, int n_threads);
 
+/* tracking.py:262-357 (kernel tod), serially in seed order: scene = mask
+ * domain and field; b* = the boundary surface for sampling; raw (X, Y, Z, 45)
+ * zero on entry. abs_sum (X, Y, Z, 45) and flushes (X, Y, Z), optional, zero
+ * on entry: per cell the sum of |flushed values| and the flush count. */
+void tgo32_tod(const tgo_scene *scene, const tgo_algorithm *algorithm, const float *bvertices,
+               const uint32_t *btriangles, const float *bcumulative, uint32_t bn, uint32_t rng_seed, int64_t n,
+               uint32_t n_steps, const uint32_t out_dims[3], const float out_inverse_affine[16], float *raw,
+               double *abs_sum, uint32_t *flushes);
+void tgo64_tod(const tgo_scene *scene, const tgo_algorithm *algorithm, const float *bvertices,
+               const uint32_t *btriangles, const float *bcumulative, uint32_t bn, uint32_t rng_seed, int64_t n,
+               uint32_t n_steps, const uint32_t out_dims[3], const float out_inverse_affine[16], double *raw,
+               double *abs_sum, uint32_t *flushes);
+
+/* One orientation update from an explicit MWC64X state, advanced in place. */
+void tgo32_update_state(const tgo_algorithm *algorithm, const float *params, const float orientation[3],
+                        uint32_t state[2], float out[3]);
+void tgo64_update_state(const tgo_algorithm *algorithm, const double *params, const double orientation[3],
+                        uint32_t state[2], double out[3]);
+
 /* Boundary seeding (domain.py:131-166, 530-556): out (n, 8). */
 void tgo_domain_seed(const float *vertices, const uint32_t *triangles, const float *cumulative,
                      uint32_t n_triangles, uint32_t rng_seed, int64_t n, float *out);

@@ -651,6 +651,98 @@ void FN(connectome)(const tgo_scene *s, const tgo_algorithm *a, const float *bve
     }
 }
 
+/* One flush of a voxel run into the sums (the reference's atomic add,
+ * core.cl:80-99, here in seed order). */
+static inline void FN(tod_flush)(int64_t cell, const R acc[45], R *raw, double *abs_sum, uint32_t *flushes)
+{
+    for (int i = 0; i < 45; i++) {
+        raw[45 * cell + i] += acc[i];
+        if (abs_sum)
+            abs_sum[45 * cell + i] += fabs((double)acc[i]);
+    }
+    if (flushes)
+        flushes[cell] += 1;
+}
+
+/* tracking.py:262-357 (kernel tod) for seeds 0..n-1, serially in seed order:
+ * the boundary sample of rng_init(rng_seed, g), whose state then drives the
+ * noise; per step mask, field, then the SH basis (spharm.cl:10-207) of
+ * cart2sph(orientation) on the output cell floor(out_inverse_affine p + 0.5)
+ * (summed over a run of steps in one cell, flushed on leaving it, on leaving
+ * the grid and at the end), then update, orientation check, move. raw
+ * (X, Y, Z, 45) receives the sums and must start at zero; abs_sum (same
+ * shape, float64) and flushes (X, Y, Z), when given, receive per cell the
+ * sum of |flushed value| and the number of flushes, which bound how far a
+ * sum of the same flushes in another order can fall from raw. */
+void FN(tod)(const tgo_scene *s, const tgo_algorithm *a, const float *bvertices, const uint32_t *btriangles,
+             const float *bcumulative, uint32_t bn, uint32_t rng_seed, int64_t n, uint32_t n_steps,
+             const uint32_t out_dims[3], const float out_inverse_affine[16], R *raw, double *abs_sum,
+             uint32_t *flushes)
+{
+    const int K_ = a->n_parameters;
+    for (int64_t g = 0; g < n; g++) {
+        uint32_t rng[2];
+        tgo_rng_init(rng_seed, (uint32_t)g, rng);
+        float p0[3], o0[3], sbary[3];
+        uint32_t stri;
+        tgo_sample_boundary(bvertices, btriangles, bcumulative, bn, rng, p0, o0, &stri, sbary);
+        VEC point = FN(mkvec)((R)p0[0], (R)p0[1], (R)p0[2]);
+        VEC orientation = FN(mkvec)((R)o0[0], (R)o0[1], (R)o0[2]);
+        const R step = (R)a->step_size;
+        R params[160];
+        R acc[45], ylm[45], dp[45], dt[45];
+        int64_t previous = -1;
+        for (uint32_t k = 0; k < n_steps; k++) {
+            if (!FN(mask_contains)(s, point, 0))
+                break;
+            int64_t base;
+            if (!FN(field_index)(s, point, &base, 0))
+                break;
+            R v[3];
+            FN(to_voxel)(out_inverse_affine, point, v);
+            int inside = 1;
+            int64_t idx[3];
+            for (int i = 0; i < 3; i++) {
+                const R r = FLOOR(v[i] + K(0.5));
+                if (!(r >= K(0.0) && r < (R)out_dims[i]))
+                    inside = 0;
+                idx[i] = inside ? (int64_t)r : 0;
+            }
+            if (inside) {
+                const int64_t cell = (idx[0] * out_dims[1] + idx[1]) * out_dims[2] + idx[2];
+                R az, col;
+                FN(cart2sph_one)(orientation, &az, &col);
+                FN(sh45_one)(az, col, ylm, dp, dt);
+                if (cell == previous) {
+                    for (int i = 0; i < 45; i++)
+                        acc[i] += ylm[i];
+                } else {
+                    if (previous >= 0)
+                        FN(tod_flush)(previous, acc, raw, abs_sum, flushes);
+                    for (int i = 0; i < 45; i++)
+                        acc[i] = ylm[i];
+                    previous = cell;
+                }
+            } else if (previous >= 0) {
+                FN(tod_flush)(previous, acc, raw, abs_sum, flushes);
+                previous = -1;
+            }
+            for (int i = 0; i < K_; i++)
+                params[i] = (R)s->field[base + i];
+            orientation = FN(update_one)(a, params, orientation, rng);
+            if (!(isfinite(orientation.x) && isfinite(orientation.y) && isfinite(orientation.z)) ||
+                SQRT((orientation.x * orientation.x + orientation.y * orientation.y) + orientation.z * orientation.z) <
+                    K(0.5))
+                break;
+            point.x += step * orientation.x;
+            point.y += step * orientation.y;
+            point.z += step * orientation.z;
+        }
+        if (previous >= 0)
+            FN(tod_flush)(previous, acc, raw, abs_sum, flushes);
+    }
+}
+
 void FN(sh45)(const R *phi, const R *theta, int64_t n, R *ylm, R *dp, R *dt)
 {
     for (int64_t i = 0; i < n; i++)
@@ -680,6 +772,17 @@ void FN(update)(const tgo_algorithm *a, uint32_t rng_seed, const R *params, cons
         out[3 * i + 1] = r.y;
         out[3 * i + 2] = r.z;
     }
+}
+
+/* One update of one orientation from an explicit noise state, which is
+ * advanced in place (the state a track carries from step to step). */
+void FN(update_state)(const tgo_algorithm *a, const R *params, const R orientation[3], uint32_t state[2], R out[3])
+{
+    const VEC o = FN(mkvec)(orientation[0], orientation[1], orientation[2]);
+    const VEC r = FN(update_one)(a, params, o, state);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
 }
 
 #undef VEC

@@ -14,6 +14,16 @@ if str(TESTS) not in sys.path:
     sys.path.insert(0, str(TESTS))
 
 
+# PyTorch ships its own HIP runtime library. Imported after libtgk.so has
+# initialised the HIP runtime of /opt/rocm, torch finds no device ("No HIP
+# GPUs are available"); imported first, both work. Tests that hand torch
+# buffers to the library therefore import torch before any runtime exists.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built libtgk.so")
 

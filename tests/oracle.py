@@ -177,6 +177,32 @@ def connectome(domain, field, algorithm, rng_seed, n_seeds, n_steps, min_length=
     return rows, cols, weights
 
 
+def tod(domain, field, algorithm, rng_seed, n_seeds, n_steps, out_dims, out_inverse_affine, *, precision=32):
+    """Raw TOD sums (X, Y, Z, 45) in seed order, with the per-cell sum of
+    |flushed values| (X, Y, Z, 45) and flush count (X, Y, Z)."""
+    holder = SceneHolder(domain, field)
+    data = domain._ensure_boundary_data()
+    alg = algorithm_struct(algorithm)
+    dims = np.ascontiguousarray(out_dims, dtype=np.uint32)
+    inv = np.ascontiguousarray(out_inverse_affine, dtype=np.float32).reshape(16)
+    shape = tuple(int(d) for d in dims)
+    raw = np.zeros(shape + (45,), np.float32 if precision == 32 else np.float64)
+    abs_sum = np.zeros(shape + (45,), np.float64)
+    flushes = np.zeros(shape, np.uint32)
+    fn = lib().tgo32_tod if precision == 32 else lib().tgo64_tod
+    fn(ctypes.byref(holder.scene), ctypes.byref(alg), _p(data["vertices"]), _p(data["indices"]),
+       _p(data["cumulative"]), c_uint32(len(data["indices"])), c_uint32(rng_seed), c_int64(n_seeds),
+       c_uint32(n_steps), _p(dims), _p(inv), _p(raw), _p(abs_sum), _p(flushes))
+    return raw, abs_sum, flushes
+
+
+def normalize_odf(odf):
+    """utils.py:6-22, in numpy as the reference writes it."""
+    normalized = np.zeros_like(odf)
+    np.divide(odf / np.sqrt(4 * np.pi), odf[..., :1], out=normalized, where=odf[..., :1] != 0)
+    return normalized
+
+
 def ragged(result):
     """Dense oracle points -> (total, 3) ragged array in seed order."""
     pts = result["points"]
@@ -254,6 +280,17 @@ def update(algorithm, rng_seed: int, params, orientations, precision=32):
     fn = lib().tgo32_update if precision == 32 else lib().tgo64_update
     fn(ctypes.byref(alg), c_uint32(rng_seed), _p(params), _p(o), c_int64(len(o)), _p(out))
     return out
+
+
+def update_state(algorithm, state, params, orientation):
+    """One FP32 update from an explicit noise state: (orientation, state)."""
+    params = np.ascontiguousarray(params, dtype=np.float32).ravel()
+    o = np.ascontiguousarray(np.asarray(orientation, dtype=np.float32).ravel()[:3])
+    state = np.array(state, dtype=np.uint32)
+    out = np.zeros(3, np.float32)
+    alg = algorithm_struct(algorithm)
+    lib().tgo32_update_state(ctypes.byref(alg), _p(params), _p(o), _p(state), _p(out))
+    return out, state
 
 
 def field_evaluate(field, points):

@@ -155,3 +155,54 @@ def test_connectome_argument_errors():
         tg.tracking.connectome(wrong, domain, field, n_seeds=1)
     with pytest.raises(TypeError):
         tg.connectome(np.zeros((2, 2, 2, 15)), n_seeds=1)
+
+
+def test_tod_zero_seeds_returns_empty_image_without_a_device():
+    # test/test_tracking.py:164-180
+    description = tg.Nifti1Image(np.zeros((2, 1, 1), dtype=np.float32), np.eye(4))
+    domain = tg.domain.Image(tg.Nifti1Image(np.ones((1, 1, 1), dtype=np.uint8), np.eye(4)))
+    field = tg.field.Image(tg.Nifti1Image(np.ones((1, 1, 1, 1), dtype=np.float32), np.eye(4)))
+    algorithm = tg.algorithm.Transport(tg.local.RealSymmetricSphericalHarmonics(1))
+    result = tg.tracking.tod(algorithm, domain, field, description, n_seeds=0)
+    assert result.shape == (2, 1, 1, 45)
+    assert result.get_data_dtype() == np.float32
+    np.testing.assert_array_equal(result.get_fdata(), 0.0)
+    # Zero steps (maximum length below one step) also return without a device.
+    result = tg.tracking.tod(algorithm, domain, field, description, n_seeds=5, maximum_streamline_length=0.0)
+    np.testing.assert_array_equal(result.get_fdata(), 0.0)
+
+
+def test_tod_argument_errors():
+    description = tg.Nifti1Image(np.zeros((2, 1, 1), dtype=np.float32), np.eye(4))
+    domain = tg.domain.Image(tg.Nifti1Image(np.ones((1, 1, 1), dtype=np.uint8), np.eye(4)))
+    field = tg.field.Image(tg.Nifti1Image(np.ones((1, 1, 1, 15), dtype=np.float32), np.eye(4)))
+    algorithm = tg.algorithm.Transport(tg.local.RealSymmetricSphericalHarmonics(15))
+    with pytest.raises(ValueError, match="n_seeds"):
+        tg.tracking.tod(algorithm, domain, field, description, n_seeds=-1)
+    with pytest.raises(ValueError, match="n_seeds"):
+        tg.tracking.tod(algorithm, domain, field, description, n_seeds=1.5)
+    with pytest.raises(ValueError, match="maximum_streamline_length"):
+        tg.tracking.tod(algorithm, domain, field, description, n_seeds=1, maximum_streamline_length=np.inf)
+    with pytest.raises(ValueError, match="3D"):
+        tg.tracking.tod(algorithm, domain, field, tg.Nifti1Image(np.zeros((2, 2)), np.eye(4)), n_seeds=1)
+    wrong = tg.algorithm.Transport(tg.local.RealSymmetricSphericalHarmonics(45))
+    with pytest.raises(ValueError, match="15 != 45|45 != 15"):
+        tg.tracking.tod(wrong, domain, field, description, n_seeds=1)
+    # The public entry point (__init__.py:257-336).
+    with pytest.raises(ValueError, match="n_seeds"):
+        tg.tod(np.zeros((2, 2, 2, 15)), n_seeds=-1)
+    with pytest.raises(TypeError):
+        tg.tod(np.zeros((2, 2, 2, 15)), n_seeds=1)
+    fod = tg.Nifti1Image(np.ones((2, 2, 2, 45), dtype=np.float32), np.eye(4))
+    with pytest.raises(ValueError, match="four-dimensional"):
+        tg.tod(tg.Nifti1Image(np.ones((2, 2, 45)), np.eye(4)), n_seeds=1)
+    with pytest.raises(ValueError, match="only DTI"):
+        tg.tod(tg.Nifti1Image(np.ones((2, 2, 2, 15)), np.eye(4)), n_seeds=1)
+    with pytest.raises(ValueError, match="three-dimensional"):
+        tg.tod(fod, n_seeds=1, domain=tg.Nifti1Image(np.ones((2, 2)), np.eye(4)))
+    with pytest.raises(TypeError, match="domain"):
+        tg.tod(fod, n_seeds=1, domain="mask")
+    # An all-zero domain image gives the zero TOD on the field's grid.
+    empty = tg.tod(fod, n_seeds=3, domain=tg.Nifti1Image(np.zeros((2, 2, 2), dtype=np.uint8), np.eye(4)))
+    assert empty.shape == (2, 2, 2, 45)
+    np.testing.assert_array_equal(empty.get_fdata(), 0.0)

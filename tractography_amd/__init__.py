@@ -149,6 +149,100 @@ def tractogram(
     return Tractogram(streamlines, affine_to_rasmm=np.eye(4))
 
 
+def tod(
+    field,
+    n_seeds: int = 1000000,
+    domain=None,
+    algorithm: BaseAlgorithm | None = None,
+    *,
+    maximum_streamline_length: float = 300.0,
+    runtime: HipRuntime | None = None,
+):
+    """Track-orientation distribution of a field (__init__.py:217-366).
+
+    ``field`` is a 4-D image (local model from its coefficient count) or a
+    ``BaseField``; ``algorithm`` defaults to transport. Seeds are sampled on
+    the boundary of ``domain`` (a domain object or a 3-D image; by default
+    the nonzero voxels of an image field). The TOD is accumulated on the
+    field's image grid, or on an image domain's grid for other fields.
+    Returns an image of 45 normalised SH coefficients per voxel.
+    """
+    from .image import float32_image
+
+    if not isinstance(n_seeds, (int, np.integer)) or n_seeds < 0:
+        raise ValueError("n_seeds must be a non-negative integer.")
+    if not np.isfinite(maximum_streamline_length) or maximum_streamline_length < 0:
+        raise ValueError("maximum_streamline_length must be finite and non-negative.")
+    if is_image(field):
+        if field.ndim != 4:
+            raise ValueError("The tracking image must be four-dimensional.")
+        tracking_field = field_module.Image(field)
+        domain_image = Nifti1Image(
+            np.any(np.asanyarray(field.dataobj) != 0, axis=-1).astype(np.uint8), field.affine
+        )
+        local_model = local.from_shape(field.shape)
+        grid_shape, grid_affine, grid_like = tuple(field.shape[:3]), field.affine, field
+    elif isinstance(field, field_module.BaseField):
+        tracking_field = field
+        local_model = local.from_shape((field.n_parameters,)) if algorithm is None else None
+        if isinstance(field, field_module.Image):
+            affine = np.linalg.inv(field.inverse_affine)
+            domain_image = Nifti1Image(np.any(field.values != 0, axis=-1).astype(np.uint8), affine)
+            grid_shape, grid_affine, grid_like = tuple(int(s) for s in field.dims[:3]), affine, None
+        else:
+            domain_image = grid_shape = grid_affine = grid_like = None
+    else:
+        raise TypeError("field must be a Nifti1Image or a BaseField instance.")
+
+    if algorithm is None:
+        assert local_model is not None
+        tracking_algorithm = algorithm_module.Transport(local_model)
+    else:
+        _validate_algorithm(algorithm)
+        tracking_algorithm = algorithm
+    if tracking_algorithm.local_model.n_parameters != tracking_field.n_parameters:
+        raise ValueError(
+            "Algorithm local model parameter count does not match the field: "
+            f"{tracking_algorithm.local_model.n_parameters} != "
+            f"{tracking_field.n_parameters}"
+        )
+
+    if domain is not None and not (is_image(domain) or isinstance(domain, domain_module.BaseDomain)):
+        raise TypeError("domain must be a BaseDomain, Nifti1Image, or None.")
+    tracking_domain = None
+    if is_image(domain):
+        if domain.ndim != 3:
+            raise ValueError("Domain image must be three-dimensional.")
+        domain_image = domain
+        if grid_shape is None:
+            grid_shape, grid_affine, grid_like = tuple(domain.shape), domain.affine, domain
+    elif domain is not None:
+        tracking_domain = domain
+        if grid_shape is None and isinstance(domain, domain_module.Image):
+            grid_shape = tuple(int(s) for s in domain.dims[:3])
+            grid_affine = np.linalg.inv(domain.inverse_affine)
+    elif domain_image is not None:
+        domain = domain_image
+    else:
+        raise ValueError("A domain is required when the field is not an image field.")
+
+    if grid_shape is None or grid_affine is None:
+        raise ValueError("A field image or image-based domain is required to define the TOD grid.")
+    image_grid = float32_image(np.zeros(grid_shape, dtype=np.float32), grid_affine, like=grid_like)
+
+    if is_image(domain):
+        if not np.any(np.asanyarray(domain.dataobj) != 0):
+            return float32_image(np.zeros(tuple(grid_shape) + (45,), dtype=np.float32), grid_affine, like=grid_like)
+        tracking_domain = domain_module.Image(domain)
+    elif tracking_domain is None:
+        raise ValueError("Could not infer a tracking domain from the field.")
+
+    return tracking.tod(
+        tracking_algorithm, tracking_domain, tracking_field, image_grid, n_seeds,
+        maximum_streamline_length=maximum_streamline_length, runtime=runtime,
+    )
+
+
 def connectome(
     field,
     n_seeds: int = 1000000,
@@ -284,6 +378,7 @@ __all__ = [
     "field",
     "local",
     "seeds",
+    "tod",
     "tracking",
     "tractogram",
     "utils",

@@ -1082,6 +1082,125 @@ int tgk_connectome(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *
     return fetch(weights, dweights, (size_t)n_seeds * 9, rt->stream);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Checks shared by the TOD entry points; fills the launch on success.
+int tod_launch(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
+               const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
+               const uint32_t output_dims[3], const float output_inverse_affine[16], tgk::TodLaunch *t,
+               int64_t *n_values)
+{
+    if (!rt || !domain || !boundary || !field || n_seeds < 0 || !output_dims || !output_inverse_affine)
+        return fail(TGK_ERR_INVALID, "bad arguments");
+    if (int r = check_algorithm(algorithm, field))
+        return r;
+    if (domain->kind != tgk::DOMAIN_MASK)
+        return fail(TGK_ERR_INVALID, "the TOD's domain must be a mask domain");
+    if (boundary->n_triangles == 0 || !boundary->cumulative.ptr)
+        return fail(TGK_ERR_INVALID, "the boundary needs triangles and cumulative areas");
+    if (!(algorithm->step_size > 0.0f) || !std::isfinite(algorithm->step_size))
+        return fail(TGK_ERR_INVALID, "step_size must be finite and positive");
+    for (int i = 0; i < 12; i++)
+        if (!std::isfinite(output_inverse_affine[i]))
+            return fail(TGK_ERR_INVALID, "output inverse affine must be finite");
+    const uint64_t cells = (uint64_t)output_dims[0] * output_dims[1] * output_dims[2];
+    if (cells > (uint64_t)INT64_MAX / (45 * sizeof(float)))
+        return fail(TGK_ERR_INVALID, "output grid too large");
+    memset(t, 0, sizeof *t);
+    t->mask = mask_view(domain);
+    t->field = field_view(field);
+    t->vertices = boundary->vertices.as<const float4>();
+    t->triangles = boundary->triangles.as<const uint4>();
+    t->cumulative = boundary->cumulative.as<const float>();
+    t->n_triangles = boundary->n_triangles;
+    t->algo = algo_params(algorithm);
+    t->rng_seed = rng_seed;
+    t->n_seeds = n_seeds;
+    t->n_steps = n_steps;
+    for (int i = 0; i < 3; i++)
+        t->grid.dims[i] = output_dims[i];
+    copy_rows3(output_inverse_affine, t->grid.inv);
+    *n_values = (int64_t)cells * 45;
+    return TGK_OK;
+}
+
+// Zero, deposit, normalise: kernels only, on the runtime stream.
+int enqueue_tod(tgk_runtime *rt, tgk::TodLaunch &t, int64_t n_values, int32_t normalize, float *d_coefficients)
+{
+    t.coefficients = d_coefficients;
+    TGK_HIP(tgk::launch_zero(d_coefficients, n_values, rt->stream), "TOD clear");
+    TGK_HIP(tgk::launch_tod(t, rt->stream), "TOD kernel");
+    if (normalize)
+        TGK_HIP(tgk::launch_normalize_odf(d_coefficients, n_values / 45, rt->stream), "TOD normalisation");
+    return TGK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tgk_tod(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
+            const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
+            const uint32_t output_dims[3], const float output_inverse_affine[16], int32_t normalize,
+            float *coefficients)
+{
+    tgk::TodLaunch t;
+    int64_t n_values = 0;
+    if (int r = tod_launch(rt, domain, boundary, field, algorithm, rng_seed, n_seeds, n_steps, output_dims,
+                           output_inverse_affine, &t, &n_values))
+        return r;
+    if (n_values > 0 && !coefficients)
+        return fail(TGK_ERR_INVALID, "coefficients is NULL");
+    if (n_values == 0)
+        return TGK_OK;
+    if (int r = select_device(rt))
+        return r;
+    Staging st;
+    float *dout;
+    if (int r = st.out((size_t)n_values, &dout))
+        return r;
+    if (int r = enqueue_tod(rt, t, n_values, normalize, dout))
+        return r;
+    return fetch(coefficients, dout, (size_t)n_values, rt->stream);
+}
+
+int tgk_tod_device(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
+                   const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
+                   const uint32_t output_dims[3], const float output_inverse_affine[16], int32_t normalize,
+                   float *d_coefficients)
+{
+    tgk::TodLaunch t;
+    int64_t n_values = 0;
+    if (int r = tod_launch(rt, domain, boundary, field, algorithm, rng_seed, n_seeds, n_steps, output_dims,
+                           output_inverse_affine, &t, &n_values))
+        return r;
+    if (n_values > 0 && !d_coefficients)
+        return fail(TGK_ERR_INVALID, "coefficients is NULL");
+    if (n_values == 0)
+        return TGK_OK;
+    if (int r = select_device(rt))
+        return r;
+    return enqueue_tod(rt, t, n_values, normalize, d_coefficients);
+}
+
+int tgk_normalize_odf(tgk_runtime *rt, int64_t n_cells, float *coefficients)
+{
+    if (!rt || n_cells < 0 || (n_cells > 0 && !coefficients))
+        return fail(TGK_ERR_INVALID, "bad arguments");
+    if (n_cells == 0)
+        return TGK_OK;
+    if (int r = select_device(rt))
+        return r;
+    Staging st;
+    float *dev;
+    if (int r = st.in(coefficients, (size_t)n_cells * 45, rt->stream, &dev))
+        return r;
+    TGK_HIP(tgk::launch_normalize_odf(dev, n_cells, rt->stream), "ODF normalisation");
+    return fetch(coefficients, dev, (size_t)n_cells * 45, rt->stream);
+}
+
 int tgk_domain_seed(tgk_runtime *rt, const tgk_domain *domain, uint32_t rng_seed, int64_t n, float *out)
 {
     if (!rt || !domain || n < 0 || (n > 0 && !out))

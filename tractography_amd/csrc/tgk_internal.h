@@ -34,6 +34,28 @@ struct ConnectomeLaunch {
 };
 hipError_t launch_connectome(const ConnectomeLaunch &c, hipStream_t stream);
 
+// Track orientation distribution (tgk_plain.hip): boundary-seeded tracks
+// deposit the SH basis of their orientation on an output grid.
+struct TodLaunch {
+    MaskView mask;
+    FieldView field;
+    const float4 *vertices;  // boundary triangles for seeding
+    const uint4 *triangles;
+    const float *cumulative;
+    uint32_t n_triangles;
+    AlgoParams algo;
+    uint32_t rng_seed;
+    int64_t n_seeds;
+    uint32_t n_steps;
+    MaskView grid;           // output grid: dims and inverse affine (mask unused)
+    float *coefficients;     // (X, Y, Z, 45) sums, zeroed before the launch
+};
+hipError_t launch_tod(const TodLaunch &t, hipStream_t stream);
+// Fill n floats with +0 (a kernel, so recorded graphs hold no memset node).
+hipError_t launch_zero(float *out, int64_t n, hipStream_t stream);
+// reference utils.py:6-22 (normalize_odf) on n_cells rows of 45, in place.
+hipError_t launch_normalize_odf(float *coefficients, int64_t n_cells, hipStream_t stream);
+
 // Fast path (tgk_fast.hip): persistent lanes with refill, seed-major output.
 struct FastLaunch {
     TrackLaunch track;

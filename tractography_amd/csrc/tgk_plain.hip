@@ -393,6 +393,114 @@ __global__ __launch_bounds__(256) void connectome_kernel(ConnectomeLaunch c)
         }
 }
 
+// Output-grid cell of p (tracking.py:290-300): floor(inverse_affine p + 0.5)
+// per axis, C-order linear index, -1 off the grid.
+__device__ __forceinline__ int64_t grid_cell(const MaskView &grid, V3 p)
+{
+    const float rx = floorf(affine_row(grid.inv, 0, p) + 0.5f);
+    const float ry = floorf(affine_row(grid.inv, 1, p) + 0.5f);
+    const float rz = floorf(affine_row(grid.inv, 2, p) + 0.5f);
+    if (!((rx >= 0.0f) & (rx < (float)grid.dims[0]) & (ry >= 0.0f) & (ry < (float)grid.dims[1]) & (rz >= 0.0f) &
+          (rz < (float)grid.dims[2])))
+        return -1;
+    return ((int64_t)(uint32_t)rx * grid.dims[1] + (uint32_t)ry) * grid.dims[2] + (uint32_t)rz;
+}
+
+// One flush of a voxel run: 45 hardware float atomics (global_atomic_add_f32,
+// no return). The reference's compare-and-swap loop (core.cl:80-99) adds the
+// same values; either way the order of adds from different tracks is the
+// device's.
+__device__ __forceinline__ void deposit_run(float *__restrict__ dst, const float acc[45])
+{
+#pragma unroll
+    for (int i = 0; i < 45; i++)
+        unsafeAtomicAdd(dst + i, acc[i]);
+}
+
+// tracking.py:262-357 (kernel tod): boundary seed g from rng_init(rng_seed,
+// g), whose state then drives the noise; each step checks the mask and the
+// field, then deposits the SH basis of the current orientation
+// (spharm.cl:10-207 at cart2sph(orientation)) on the output cell of the
+// point, then updates, rejects a non-finite or short orientation, and moves.
+// Consecutive steps in one cell are summed in registers (acc = basis on
+// entering a cell, acc += basis after) and flushed with atomics when the
+// track leaves the cell, leaves the grid, or ends.
+template <int MODEL, int ALGO, int KT>
+__global__ __launch_bounds__(256) void tod_kernel(TodLaunch t)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= t.n_seeds)
+        return;
+    uint2 rng = rng_init(t.rng_seed, (uint32_t)g);
+    const BoundarySample s = sample_boundary(t.vertices, t.triangles, t.cumulative, t.n_triangles, rng);
+    V3 point = s.point, o = s.orientation;
+    const float step = t.algo.step_size;
+    const uint32_t stride = t.field.stride;
+    float acc[45];
+#pragma unroll
+    for (int i = 0; i < 45; i++)
+        acc[i] = 0.0f;
+    int64_t previous = -1;
+    for (uint32_t k = 0; k < t.n_steps; k++) {
+        if (!mask_contains(t.mask, point))
+            break;
+        const int64_t voxel = field_voxel(t.field, point);
+        if (voxel < 0)
+            break;
+        const int64_t cell = grid_cell(t.grid, point);
+        if (cell >= 0) {
+            const bool same = cell == previous;
+            if (!same && previous >= 0)
+                deposit_run(t.coefficients + previous * 45, acc);
+            float az, col;
+            plain::cart2sph(o, az, col);
+            // One basis evaluation for both cases: a select keeps the
+            // reference's `= ylm` (not `0 + ylm`, which differs for -0) on
+            // entering a cell.
+            plain::sh45_terms(az, col, [&](int i, float y, float, float) { acc[i] = same ? acc[i] + y : y; });
+            previous = cell;
+        } else if (previous >= 0) {
+            deposit_run(t.coefficients + previous * 45, acc);
+            previous = -1;
+        }
+        o = plain::update<MODEL, ALGO, KT>(t.algo, t.field.data + voxel * stride, o, rng);
+        if (!(__builtin_isfinite(o.x) && __builtin_isfinite(o.y) && __builtin_isfinite(o.z)) || length3(o) < 0.5f)
+            break;
+        point.x += step * o.x;
+        point.y += step * o.y;
+        point.z += step * o.z;
+    }
+    if (previous >= 0)
+        deposit_run(t.coefficients + previous * 45, acc);
+}
+
+__global__ __launch_bounds__(256) void zero_kernel(float *__restrict__ out, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = 0.0f;
+}
+
+// utils.py:6-22 (normalize_odf): numpy forms odf / sqrt(4 pi) in float64
+// (sqrt(4 pi) is a float64 scalar), divides by the float32 c0 promoted to
+// float64 and rounds once to float32 on the store; cells with c0 == 0 become
+// 0. A NaN c0 compares unequal to 0 and propagates, as in numpy.
+__global__ __launch_bounds__(256) void normalize_odf_kernel(float *__restrict__ c, int64_t n_cells)
+{
+    const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= n_cells)
+        return;
+    float *row = c + cell * 45;
+    const float c0 = row[0];
+    if (c0 != 0.0f) {
+        const double sqrt_4pi = 0x1.c5bf891b4ef6ap+1;  // float64(sqrt(4 pi)) = 3.5449077018110318
+        for (int i = 0; i < 45; i++)
+            row[i] = (float)(((double)row[i] / sqrt_4pi) / (double)c0);
+    } else {
+        for (int i = 0; i < 45; i++)
+            row[i] = 0.0f;
+    }
+}
+
 inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
 }  // namespace
@@ -503,6 +611,45 @@ hipError_t launch_connectome(const ConnectomeLaunch &c, hipStream_t stream)
                                                   : launch_connectome_kt<MODEL_DTI, ALGO_TRANSPORT>(c, stream);
     return c.algo.algorithm == ALGO_DIFFUSION ? launch_connectome_kt<MODEL_SH, ALGO_DIFFUSION>(c, stream)
                                               : launch_connectome_kt<MODEL_SH, ALGO_TRANSPORT>(c, stream);
+}
+
+template <int MODEL, int ALGO>
+hipError_t launch_tod_kt(const TodLaunch &t, hipStream_t stream)
+{
+    const unsigned blocks = blocks_for(t.n_seeds, 256);
+    if (MODEL == MODEL_SH && t.algo.n_parameters == 45)
+        tod_kernel<MODEL, ALGO, 45><<<blocks, 256, 0, stream>>>(t);
+    else
+        tod_kernel<MODEL, ALGO, 0><<<blocks, 256, 0, stream>>>(t);
+    return hipGetLastError();
+}
+
+hipError_t launch_tod(const TodLaunch &t, hipStream_t stream)
+{
+    if (t.n_seeds == 0 || t.n_steps == 0)
+        return hipSuccess;
+    if (t.algo.model == MODEL_DTI)
+        return t.algo.algorithm == ALGO_DIFFUSION ? launch_tod_kt<MODEL_DTI, ALGO_DIFFUSION>(t, stream)
+                                                  : launch_tod_kt<MODEL_DTI, ALGO_TRANSPORT>(t, stream);
+    return t.algo.algorithm == ALGO_DIFFUSION ? launch_tod_kt<MODEL_SH, ALGO_DIFFUSION>(t, stream)
+                                              : launch_tod_kt<MODEL_SH, ALGO_TRANSPORT>(t, stream);
+}
+
+hipError_t launch_zero(float *out, int64_t n, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+    zero_kernel<<<(unsigned)blocks, 256, 0, stream>>>(out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_normalize_odf(float *coefficients, int64_t n_cells, hipStream_t stream)
+{
+    if (n_cells == 0)
+        return hipSuccess;
+    normalize_odf_kernel<<<blocks_for(n_cells, 256), 256, 0, stream>>>(coefficients, n_cells);
+    return hipGetLastError();
 }
 
 hipError_t launch_domain_seed(const float4 *vertices, const uint4 *triangles, const float *cumulative,

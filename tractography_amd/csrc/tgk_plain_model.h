@@ -57,18 +57,18 @@ __device__ __forceinline__ float randn(uint2 &s)
     return tgk_sqrtf(-2.0f * tgk_logf(n)) * tgk_cosf((2.0f * 3.14159265359f) * randu(s));
 }
 
-// Contraction of up to 45 SH coefficients with the l <= 8 basis and its two
-// angular derivatives (spharm.cl:216-413, local.py:91-120). The basis is
-// generated band by band: for each (l, m > 0) the reference's prefactor of
-// the value (A), of the azimuth derivative over sin(theta) (D) and of the
-// colatitude derivative (T) is formed once and multiplied by sin(m phi) for
-// index l(l+1)/2 - m and by cos(m phi) for index l(l+1)/2 + m; the D term of
-// the cos partner is -(D sin(m phi)), equal bit for bit to the reference's
-// negated-constant form. Terms are accumulated in index order, so sums round
-// exactly as the reference's `for (i < K)` loop. Coefficients past K arrive
-// as +0 and leave the sums unchanged.
-__device__ __forceinline__ void sh45_contract(const float c[45], float phi, float theta, float &v, float &th,
-                                              float &ph)
+// The l <= 8 real SH basis (45 terms) and its two angular derivatives
+// (spharm.cl:216-413; the values alone are spharm.cl:10-207, the same
+// expressions), handed to visit(i, Y_i, dY_i/dphi / sin(theta), dY_i/dtheta)
+// in index order. The basis is generated band by band: for each (l, m > 0)
+// the reference's prefactor of the value (A), of the azimuth derivative over
+// sin(theta) (D) and of the colatitude derivative (T) is formed once and
+// multiplied by sin(m phi) for index l(l+1)/2 - m and by cos(m phi) for index
+// l(l+1)/2 + m; the D term of the cos partner is -(D sin(m phi)), equal bit
+// for bit to the reference's negated-constant form. A visitor that ignores
+// the derivatives lets the compiler drop them.
+template <class Visit>
+__device__ __forceinline__ void sh45_terms(float phi, float theta, Visit &&visit)
 {
     float st, ct, sp, cp, s2p, c2p;
     tgk_sincosf(theta, &st, &ct);
@@ -87,14 +87,7 @@ __device__ __forceinline__ void sh45_contract(const float c[45], float phi, floa
     const float ct6 = ct5 * ct, ct7 = ct6 * ct, ct8 = ct7 * ct;
     const float stct = st * ct;
 
-    float sv = 0.0f, sth = 0.0f, sph = 0.0f;
-#define TGK_ACC(i, Y, DP, DT)          \
-    {                                  \
-        const float ci_ = c[i];        \
-        sv += ci_ * (Y);               \
-        sth += ci_ * (DT);             \
-        sph += ci_ * (DP);             \
-    }
+#define TGK_ACC(i, Y, DP, DT) visit(i, (Y), (DP), (DT));
 #define TGK_PAIR(ineg, ipos, A, D, T, S, C) \
     TGK_ACC(ineg, (A) * (S), (D) * (C), (T) * (S))
 #define TGK_PAIR_POS(ipos, A, D, T, S, C) TGK_ACC(ipos, (A) * (C), -((D) * (S)), (T) * (C))
@@ -220,6 +213,21 @@ __device__ __forceinline__ void sh45_contract(const float c[45], float phi, floa
 #undef TGK_PAIR_POS
 #undef TGK_PAIR
 #undef TGK_ACC
+}
+
+// Model value and angular derivatives: terms accumulated in index order, so
+// sums round exactly as the reference's `for (i < K)` loop. Coefficients past
+// K arrive as +0 and leave the sums unchanged.
+__device__ __forceinline__ void sh45_contract(const float c[45], float phi, float theta, float &v, float &th,
+                                              float &ph)
+{
+    float sv = 0.0f, sth = 0.0f, sph = 0.0f;
+    sh45_terms(phi, theta, [&](int i, float y, float dp, float dt) {
+        const float ci = c[i];
+        sv += ci * y;
+        sth += ci * dt;
+        sph += ci * dp;
+    });
     v = sv;
     th = sth;
     ph = sph;

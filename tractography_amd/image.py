@@ -43,6 +43,19 @@ class Nifti1Image:
         return self.dataobj.dtype
 
 
+def float32_image(data, affine, like=None):
+    """An image of float32 ``data`` on ``affine``: a NiBabel image with a copy
+    of ``like``'s header (data type set to float32) when ``like`` carries a
+    NiBabel header, else the stand-in."""
+    data = np.asarray(data, dtype=np.float32)
+    header = getattr(like, "header", None)
+    if _NibabelImage is not None and header is not None and hasattr(header, "set_data_dtype"):
+        header = header.copy()
+        header.set_data_dtype(np.float32)
+        return _NibabelImage(data, affine, header=header)
+    return Nifti1Image(data, affine)
+
+
 def is_image(value) -> bool:
     """True for NiBabel images and for the stand-in above."""
     if isinstance(value, Nifti1Image):

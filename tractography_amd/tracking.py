@@ -142,6 +142,76 @@ def tractogram(
     return Tractogram(result.streamlines(), affine_to_rasmm=np.eye(4))
 
 
+def tod(
+    algorithm: BaseAlgorithm,
+    domain: BaseDomain,
+    field: BaseField,
+    image,
+    n_seeds: int,
+    maximum_streamline_length: float = 300.0,
+    runtime: HipRuntime | None = None,
+):
+    """Track orientation distribution as SH coefficients (tracking.py:181-385).
+
+    Each track starts at a uniform sample of the domain's boundary surface
+    (directed inward) and runs for at most floor(maximum / step_size) steps
+    while the domain contains its point and the field is defined. At every
+    such step the 45-term SH basis of its orientation is added to the
+    ``image`` grid cell of its point (when on the grid). Each nonempty cell
+    is then normalised to unit integral (utils.normalize_odf).
+
+    Returns an image of shape ``image.shape + (45,)`` on ``image``'s affine.
+
+    Raises:
+        ValueError: incompatible model and field, invalid seed count or
+            maximum length, a non-3D output image, or a non-positive step.
+
+    Zero seeds or zero steps return the zero image without creating a
+    runtime.
+    """
+    from .image import float32_image
+    from .utils import normalize_odf
+
+    if algorithm.local_model.n_parameters != field.n_parameters:
+        raise ValueError(
+            "Incompatible local model and field, they do not have the "
+            "same number of parameters: "
+            f"{algorithm.local_model.n_parameters} != {field.n_parameters}"
+        )
+    if not isinstance(n_seeds, (int, np.integer)) or n_seeds < 0:
+        raise ValueError("n_seeds must be a non-negative integer.")
+    if not np.isfinite(maximum_streamline_length) or maximum_streamline_length < 0:
+        raise ValueError("maximum_streamline_length must be finite and non-negative.")
+    if len(image.shape) != 3:
+        raise ValueError("The output image description must be 3D.")
+
+    output_shape = tuple(int(size) for size in image.shape)
+    affine = image.affine if image.affine is not None else np.eye(4)
+    inverse_affine = np.ascontiguousarray(np.linalg.inv(affine), dtype=np.float32)
+    step_size = float(algorithm.step_size)
+    if not np.isfinite(step_size) or step_size <= 0:
+        raise ValueError("algorithm.step_size must be finite and positive.")
+    n_steps = int(np.floor(maximum_streamline_length / step_size))
+    if n_seeds == 0 or n_steps == 0:
+        raw = np.zeros(output_shape + (45,), dtype=np.float32)
+        return float32_image(normalize_odf(raw), affine, like=image)
+
+    if runtime is None:
+        runtime = HipRuntime()
+    domain_handle = domain.upload(runtime, with_boundary=True)
+    field_handle = field.upload(runtime)
+    out = np.empty(output_shape + (45,), dtype=np.float32)
+    dims = np.asarray(output_shape, dtype=np.uint32)
+    _native.check(
+        _native.lib().tgk_tod(
+            runtime.handle, domain_handle.handle, domain_handle.handle, field_handle.handle, algorithm.descriptor(),
+            int(algorithm.get_rng_seed()), int(n_seeds), min(n_steps, 2**32 - 1), _native.ptr(dims),
+            _native.ptr(inverse_affine), 1, _native.ptr(out),
+        )
+    )
+    return float32_image(out, affine, like=image)
+
+
 def connectome(
     algorithm: BaseAlgorithm,
     domain: BaseDomain,
