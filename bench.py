@@ -15,6 +15,8 @@ of all saved points into one ragged array.
 Other workloads (--workload): sphere32 (32^3 SH45 Transport, 10k seeds), dti
 (145x174x145 tensors, Transport, 1M seeds), shell (the SH45 field bounded by
 two displaced ellipsoid meshes of ~200k triangles each, Diffusion, 2M seeds).
+--product tod / connectome times that product instead (tracks from boundary
+seeds of the workload's mask, plain FP32 path) and prints its own line.
 The module also holds the synthetic workload generators used by the tests.
 """
 
@@ -677,6 +679,106 @@ def host_leg(cfg, gpu_results: dict, seconds: float = 15.0, n_accuracy: int = 40
     return baseline, accuracy
 
 
+Y0_F32 = float(np.float32(0.28209479177387814))  # the constant l = 0 basis value, as deposited
+
+
+def product_leg(args, tg, torch, cfg, workload: str):
+    """Time one TOD or connectome pass over boundary seeds of the workload's
+    mask domain (n_seeds tracks of at most max_length / step steps) on the
+    plain FP32 path, and the FP32 host restatement (one core) on a bounded
+    sample of the same tracks. Prints one JSON line."""
+    from tractography_amd import _native
+    from tractography_amd.domain import _mesh_handle_for
+
+    lib = _native.lib()
+    runtime = tg.HipRuntime()
+    field = tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4)))
+    domain = tg.domain.Image(tg.Nifti1Image(cfg["mask"], np.eye(4)))
+    algorithm = make_algorithm(tg, cfg)
+    rng_seed = int(algorithm.get_rng_seed())
+    n = int(args.seeds or len(cfg["seeds"]))
+    n_steps = int(np.floor(cfg["max_length"] / cfg["step_size"]))
+    field_handle = field.upload(runtime)
+    domain_handle = domain.upload(runtime, with_boundary=True)
+    dev = torch.device("cuda", runtime.device)
+    stream = torch.cuda.ExternalStream(runtime.stream(), device=dev)
+    dims = np.ascontiguousarray(cfg["mask"].shape, dtype=np.uint32)
+    inverse = np.ascontiguousarray(np.eye(4), dtype=np.float32)
+    if args.product == "tod":
+        out = torch.empty(int(np.prod(dims)) * 45, dtype=torch.float32, device=dev)
+
+        def launch():
+            _native.check(lib.tgk_tod_device(runtime.handle, domain_handle.handle, domain_handle.handle,
+                                             field_handle.handle, algorithm.descriptor(), rng_seed, n, n_steps,
+                                             _native.ptr(dims), _native.ptr(inverse), 0, out.data_ptr()))
+    else:
+        boundary = _mesh_handle_for(domain, runtime)
+        # The host entry point copies its 9 entries per seed back; the timed
+        # region is the whole call.
+        rows_h = np.empty(9 * n, np.int32)
+        cols_h = np.empty(9 * n, np.int32)
+        weights_h = np.empty(9 * n, np.float32)
+
+        def launch():
+            _native.check(lib.tgk_connectome(runtime.handle, domain_handle.handle, boundary.handle,
+                                             field_handle.handle, algorithm.descriptor(), rng_seed, n, n_steps,
+                                             0.0, _native.ptr(rows_h), _native.ptr(cols_h),
+                                             _native.ptr(weights_h)))
+    for _ in range(args.warmup):
+        launch()
+    runtime.finish()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        t0 = time.perf_counter()
+        launch()
+        stop.record(stream)
+        runtime.finish()
+        torch.cuda.synchronize()
+        times.append(((time.perf_counter() - t0) * 1e3, start.elapsed_time(stop)))
+    wall = float(np.mean([t[0] for t in times]))
+    device_ms = float(np.mean([t[1] for t in times]))
+    line = {"metric": f"{args.product} tracks/s on 1 MI355X (plain FP32 path)", "unit": "tracks/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded numpy generators)",
+            "config": {"workload": workload, "product": args.product, "boundary_seeds": n, "max_steps": n_steps}}
+    if args.product == "tod":
+        raw = out.view(-1, 45)[:, 0].to(torch.float64)
+        deposits = float(raw.sum().item()) / Y0_F32
+        line["value"] = n / (device_ms * 1e-3)
+        line["deposited_steps"] = deposits
+        line["steps_per_s"] = deposits / (device_ms * 1e-3)
+        line["kernel_ms"] = device_ms
+    else:
+        line["value"] = n / (wall * 1e-3)
+        line["entries_set"] = int((rows_h >= 0).sum())
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle  # noqa: E402  (the host restatement)
+
+        sample = 64
+        while True:
+            t0 = time.perf_counter()
+            if args.product == "tod":
+                r, _, _ = oracle.tod(domain, field, algorithm, rng_seed, sample, n_steps, dims, inverse)
+                work = float(r[..., 0].astype(np.float64).sum()) / Y0_F32
+            else:
+                oracle.connectome(domain, field, algorithm, rng_seed, sample, n_steps, threads=1)
+                work = sample
+            dt = time.perf_counter() - t0
+            if dt > 5.0 or sample >= n:
+                break
+            sample = min(n, int(sample * min(16.0, max(2.0, 8.0 / max(dt, 1e-3)))))
+        line["cpu_baseline"] = {"value": sample / dt, "unit": "tracks/s", "cores": 1, "kind": "port",
+                                "sample": f"first {sample} boundary seeds in {dt:.1f} s (FP32 host restatement)"}
+        if args.product == "tod":
+            line["cpu_baseline"]["steps_per_s"] = work / dt
+    print(json.dumps(line), flush=True)
+
+
 WORKLOADS = {
     "sphere32": (sphere32_transport, 10_000,
                  "32^3 SH45 (lmax 8) Transport gamma 1, {n} seeds, step 0.1, save 0.25, max 300"),
@@ -714,6 +816,8 @@ def main():
                              "(reference dispatch for hardware byte counters)")
     parser.add_argument("--no-traffic", action="store_true",
                         help="skip the two hardware-counter runs that measure the tracking kernel's HBM bytes")
+    parser.add_argument("--product", default="tractogram", choices=["tractogram", "tod", "connectome"],
+                        help="tod / connectome: time that product over boundary seeds of the workload's mask")
     args = parser.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (seeds shard with no "
@@ -722,7 +826,8 @@ def main():
     # Counter runs are child processes started before this process touches
     # the GPU.
     traffic = None
-    if not args.no_traffic and args.calibrate_copy == 0 and args.workload != "sh91" and "fast" in args.paths:
+    if (not args.no_traffic and args.calibrate_copy == 0 and args.workload != "sh91" and "fast" in args.paths
+            and args.product == "tractogram"):
         traffic = traffic_leg(args)
 
     import torch
@@ -736,6 +841,11 @@ def main():
     n_seeds = cfg["n_seeds"] if cfg.get("seeds") is None else len(cfg["seeds"])
     workload = f"{args.workload}: {description.format(n=n_seeds)}"
     print(f"[bench] workload generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    if args.product != "tractogram":
+        if cfg.get("mask") is None:
+            raise SystemExit(f"--product {args.product} needs a workload with a mask domain")
+        product_leg(args, tg, torch, cfg, workload)
+        return
     if cfg.get("chunks") is not None:
         run_graph_workload(args, cfg, workload)
         return

@@ -84,3 +84,21 @@ def test_connectome_cross_dataset_connects_expected_regions(runtime):
     for region, partner in enumerate(expected_partners):
         competing_weights = np.delete(undirected[region], [region, partner])
         assert undirected[region, partner] > competing_weights.max()
+
+
+def test_connectome_persistent_lanes_refill(runtime, oracle_threads):
+    # Many more seeds than the grid has lanes, with short tracks: lanes claim
+    # new seeds many times over, and every seed's entries still equal the
+    # restatement's.
+    fod, wm, _ = fixtures.cross(shape=(14, 14, 3), n_coefficients=45)
+    domain = tg.domain.Image(wm)
+    field = tg.field.Image(fod)
+    algorithm = tg.algorithm.Diffusion(tg.local.RealSymmetricSphericalHarmonics(45), step_size=0.25, gamma=1.0,
+                                       noise_variance=0.1, seed=21)
+    seed = algorithm.get_rng_seed()
+    n = 600_000
+    got = _device_connectome(runtime, domain, field, algorithm, seed, n, 24, 0.0)
+    ref = oracle.connectome(domain, field, algorithm, seed, n, 24, 0.0, threads=oracle_threads)
+    print(f"[connectome refill] {(got[0] >= 0).mean():.3f} of entries set", flush=True)
+    for a, b in zip(got, ref):
+        np.testing.assert_array_equal(a, b)

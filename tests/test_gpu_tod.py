@@ -210,3 +210,15 @@ def test_tod_from_image_field_uses_defaults(runtime):
     fod, _, _ = fixtures.cross()
     result = tg.tod(tg.field.Image(fod), n_seeds=16, maximum_streamline_length=2.0, runtime=runtime)
     assert result.shape == fod.shape[:3] + (45,)
+
+
+def test_tod_persistent_lanes_refill(runtime):
+    # Many more seeds than the grid has lanes, with short tracks.
+    domain, field, algorithm, shape = _scene("sh45-diffusion")
+    seed = algorithm.get_rng_seed()
+    dims = np.array(shape)
+    inverse = np.eye(4, dtype=np.float32)
+    got = _device_tod(runtime, domain, field, algorithm, seed, 400_000, 12, dims, inverse)
+    raw, abs_sum, flushes = oracle.tod(domain, field, algorithm, seed, 400_000, 12, dims, inverse)
+    assert flushes.sum() > 400_000
+    assert_same_sums(got, raw, abs_sum, flushes, "refill")

@@ -269,6 +269,9 @@ int tgk_runtime_create(int32_t device, tgk_runtime **out)
     rt->device = device;
     TGK_HIP(hipSetDevice(device), "hipSetDevice");
     TGK_HIP(hipStreamCreateWithFlags(&rt->stream, hipStreamNonBlocking), "hipStreamCreate");
+    // The seed counter of the persistent kernels, allocated up front so that
+    // no launch allocates (e.g. while a graph is being recorded).
+    TGK_HIP(rt->queue.allocate(256), "queue");
     *out = rt.release();
     return TGK_OK;
 }
@@ -534,6 +537,11 @@ int tgk_domain_destroy(tgk_domain *domain)
 
 namespace {
 
+// Seeds per call: seed indices are 32-bit (the reference's get_global_id,
+// rng_init's stream id) and the persistent kernels claim them from a 32-bit
+// counter that may run past n by one claim per wavefront.
+constexpr int64_t MAX_SEEDS = INT32_MAX;
+
 int check_track_args(const tgk_domain *domain, const tgk_field *field, const tgk_algorithm *algorithm, int32_t path,
                      int64_t n_seeds, int64_t batch_size, uint32_t n_points, float save_at)
 {
@@ -545,6 +553,8 @@ int check_track_args(const tgk_domain *domain, const tgk_field *field, const tgk
         return r;
     if (n_seeds < 0)
         return fail(TGK_ERR_INVALID, "negative seed count");
+    if (n_seeds > MAX_SEEDS)
+        return fail(TGK_ERR_INVALID, "%lld seeds: at most %lld per call", (long long)n_seeds, (long long)MAX_SEEDS);
     if (batch_size <= 0)
         return fail(TGK_ERR_INVALID, "batch_size must be positive");
     if (n_points == 0)
@@ -1035,6 +1045,8 @@ int tgk_connectome(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *
 {
     if (!rt || !domain || !boundary || !field || n_seeds < 0 || (n_seeds > 0 && (!rows || !cols || !weights)))
         return fail(TGK_ERR_INVALID, "bad arguments");
+    if (n_seeds > MAX_SEEDS)
+        return fail(TGK_ERR_INVALID, "%lld seeds: at most %lld per call", (long long)n_seeds, (long long)MAX_SEEDS);
     if (int r = check_algorithm(algorithm, field))
         return r;
     if (domain->kind != tgk::DOMAIN_MASK)
@@ -1074,6 +1086,7 @@ int tgk_connectome(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *
     c.rows = drows;
     c.cols = dcols;
     c.weights = dweights;
+    c.queue = rt->queue.as<uint32_t>();
     TGK_HIP(tgk::launch_connectome(c, rt->stream), "connectome kernel");
     if (int r = fetch(rows, drows, (size_t)n_seeds * 9, rt->stream))
         return r;
@@ -1094,6 +1107,8 @@ int tod_launch(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boun
 {
     if (!rt || !domain || !boundary || !field || n_seeds < 0 || !output_dims || !output_inverse_affine)
         return fail(TGK_ERR_INVALID, "bad arguments");
+    if (n_seeds > MAX_SEEDS)
+        return fail(TGK_ERR_INVALID, "%lld seeds: at most %lld per call", (long long)n_seeds, (long long)MAX_SEEDS);
     if (int r = check_algorithm(algorithm, field))
         return r;
     if (domain->kind != tgk::DOMAIN_MASK)
@@ -1122,6 +1137,7 @@ int tod_launch(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boun
     for (int i = 0; i < 3; i++)
         t->grid.dims[i] = output_dims[i];
     copy_rows3(output_inverse_affine, t->grid.inv);
+    t->queue = rt->queue.as<uint32_t>();
     *n_values = (int64_t)cells * 45;
     return TGK_OK;
 }

@@ -31,6 +31,7 @@ struct ConnectomeLaunch {
     int32_t *rows;
     int32_t *cols;
     float *weights;
+    uint32_t *queue;         // seed counter of the persistent lanes (zeroed by the launch)
 };
 hipError_t launch_connectome(const ConnectomeLaunch &c, hipStream_t stream);
 
@@ -49,6 +50,7 @@ struct TodLaunch {
     uint32_t n_steps;
     MaskView grid;           // output grid: dims and inverse affine (mask unused)
     float *coefficients;     // (X, Y, Z, 45) sums, zeroed before the launch
+    uint32_t *queue;         // seed counter of the persistent lanes (zeroed by the launch)
 };
 hipError_t launch_tod(const TodLaunch &t, hipStream_t stream);
 // Fill n floats with +0 (a kernel, so recorded graphs hold no memset node).

@@ -332,6 +332,29 @@ __global__ void domain_seed_kernel(const float4 *__restrict__ vertices, const ui
     row[7] = 0.0f;
 }
 
+// Claims seeds for the idle lanes of the wavefront from a device queue, one
+// atomic per refill; returns the lane's new seed index, or -1 when it gets
+// none. Sets `drained` (wavefront-uniform) once the queue has passed n.
+__device__ __forceinline__ int64_t claim_seed(bool idle, uint32_t *queue, int64_t n, bool &drained)
+{
+    const uint64_t need = __ballot(idle);
+    if (need == 0 || drained)
+        return -1;
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)need) - 1;
+    const uint32_t count = (uint32_t)__popcll(need);
+    uint32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(queue, count);
+    base = __shfl(base, leader);
+    if ((uint64_t)base + count >= (uint64_t)n)
+        drained = true;
+    if (!idle)
+        return -1;
+    const uint64_t g = (uint64_t)base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
+    return g < (uint64_t)n ? (int64_t)g : -1;
+}
+
 // tracking.py:457-558 (kernel connectome): boundary seed g from
 // rng_init(rng_seed, g), whose state then drives the noise; the loop checks
 // the mask, then the field, updates, rejects a non-finite or short
@@ -339,58 +362,88 @@ __global__ void domain_seed_kernel(const float4 *__restrict__ vertices, const ui
 // boundary BVH and a hit deposits the outer product of the start and hit
 // barycentric weights on the two triangles' vertices (9 entries per seed,
 // row -1 when none).
+//
+// Persistent lanes: a grid that fills the GPU once; a lane whose track ends
+// claims the next seed, so tracks of 1 to n_steps steps do not leave lanes
+// idle until the longest of their wavefront ends. Every track is computed
+// exactly as in a one-lane-per-seed launch; only which lane runs it changes.
 template <int MODEL, int ALGO, int KT>
 __global__ __launch_bounds__(256) void connectome_kernel(ConnectomeLaunch c)
 {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= c.n_seeds)
-        return;
-    int32_t *rows = c.rows + 9 * g;
-    int32_t *cols = c.cols + 9 * g;
-    float *weights = c.weights + 9 * g;
-    for (int i = 0; i < 9; i++) {
-        rows[i] = -1;
-        cols[i] = -1;
-        weights[i] = 0.0f;
-    }
-    uint2 rng = rng_init(c.rng_seed, (uint32_t)g);
-    const BoundarySample s = sample_boundary(c.boundary.vertices, c.boundary.triangles, c.cumulative,
-                                             c.n_triangles, rng);
-    V3 point = s.point, o = s.orientation, previous = point;
     const float step = c.algo.step_size;
     const uint32_t stride = c.field.stride;
-    uint32_t tracked = 0;
-    for (uint32_t k = 0; k < c.n_steps; k++) {
-        if (!mask_contains(c.mask, point))
-            break;
-        const int64_t voxel = field_voxel(c.field, point);
-        if (voxel < 0)
-            break;
-        o = plain::update<MODEL, ALGO, KT>(c.algo, c.field.data + voxel * stride, o, rng);
-        if (!(__builtin_isfinite(o.x) && __builtin_isfinite(o.y) && __builtin_isfinite(o.z)) || length3(o) < 0.5f)
-            break;
-        previous = point;
-        point.x += step * o.x;
-        point.y += step * o.y;
-        point.z += step * o.z;
-        tracked++;
-    }
-    if ((float)tracked * step < c.min_length)
-        return;
-    uint32_t hit;
-    float t, bary[3];
-    if (!bvh_segment_hit_bary(c.boundary, previous, point, hit, t, bary))
-        return;
-    const uint4 si = c.boundary.triangles[s.triangle];
-    const uint4 ei = c.boundary.triangles[hit];
-    const uint32_t sv[3] = {si.x, si.y, si.z};
-    const uint32_t ev[3] = {ei.x, ei.y, ei.z};
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-            rows[3 * i + j] = (int32_t)sv[i];
-            cols[3 * i + j] = (int32_t)ev[j];
-            weights[3 * i + j] = s.bary[i] * bary[j];
+    int64_t g = -1;
+    bool drained = false;
+    uint2 rng = make_uint2(0u, 0u);
+    V3 point = v3(0.0f, 0.0f, 0.0f), o = point, previous = point;
+    uint32_t tracked = 0, start_triangle = 0;
+    float start_bary[3] = {0.0f, 0.0f, 0.0f};
+    while (true) {
+        const int64_t claimed = claim_seed(g < 0, c.queue, c.n_seeds, drained);
+        if (claimed >= 0) {
+            g = claimed;
+            rng = rng_init(c.rng_seed, (uint32_t)g);
+            const BoundarySample s = sample_boundary(c.boundary.vertices, c.boundary.triangles, c.cumulative,
+                                                     c.n_triangles, rng);
+            point = s.point;
+            o = s.orientation;
+            previous = point;
+            tracked = 0;
+            start_triangle = s.triangle;
+            for (int i = 0; i < 3; i++)
+                start_bary[i] = s.bary[i];
         }
+        if (__ballot(g >= 0) == 0)
+            break;
+        if (g < 0)
+            continue;
+        bool stop = tracked >= c.n_steps || !mask_contains(c.mask, point);
+        int64_t voxel = -1;
+        if (!stop) {
+            voxel = field_voxel(c.field, point);
+            stop = voxel < 0;
+        }
+        if (!stop) {
+            o = plain::update<MODEL, ALGO, KT>(c.algo, c.field.data + voxel * stride, o, rng);
+            stop = !(__builtin_isfinite(o.x) && __builtin_isfinite(o.y) && __builtin_isfinite(o.z)) ||
+                   length3(o) < 0.5f;
+        }
+        if (!stop) {
+            previous = point;
+            point.x += step * o.x;
+            point.y += step * o.y;
+            point.z += step * o.z;
+            tracked++;
+            continue;
+        }
+        // The track has ended: its 9 entries.
+        int32_t *rows = c.rows + 9 * g;
+        int32_t *cols = c.cols + 9 * g;
+        float *weights = c.weights + 9 * g;
+        uint32_t hit = 0;
+        float t, bary[3];
+        const bool accepted = !((float)tracked * step < c.min_length) &&
+                              bvh_segment_hit_bary(c.boundary, previous, point, hit, t, bary);
+        if (accepted) {
+            const uint4 si = c.boundary.triangles[start_triangle];
+            const uint4 ei = c.boundary.triangles[hit];
+            const uint32_t sv[3] = {si.x, si.y, si.z};
+            const uint32_t ev[3] = {ei.x, ei.y, ei.z};
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    rows[3 * i + j] = (int32_t)sv[i];
+                    cols[3 * i + j] = (int32_t)ev[j];
+                    weights[3 * i + j] = start_bary[i] * bary[j];
+                }
+        } else {
+            for (int i = 0; i < 9; i++) {
+                rows[i] = -1;
+                cols[i] = -1;
+                weights[i] = 0.0f;
+            }
+        }
+        g = -1;
+    }
 }
 
 // Output-grid cell of p (tracking.py:290-300): floor(inverse_affine p + 0.5)
@@ -406,15 +459,48 @@ __device__ __forceinline__ int64_t grid_cell(const MaskView &grid, V3 p)
     return ((int64_t)(uint32_t)rx * grid.dims[1] + (uint32_t)ry) * grid.dims[2] + (uint32_t)rz;
 }
 
-// One flush of a voxel run: 45 hardware float atomics (global_atomic_add_f32,
-// no return). The reference's compare-and-swap loop (core.cl:80-99) adds the
-// same values; either way the order of adds from different tracks is the
+// Run sums staged in LDS for one wavefront: up to one record (45 sums and
+// the cell) per lane. Stride 45 (odd) keeps the lanes' writes on distinct
+// banks; the drain reads the records as one linear array.
+struct RunStage {
+    float vals[64 * 45];
+    int64_t cell[64];
+};
+
+// Flushes the runs of the lanes with `need` (wavefront-converged call): each
+// such lane writes its record to the stage, then the whole wavefront adds
+// the records to the grid with hardware float atomics (global_atomic_add_f32,
+// no return), 64 consecutive (record, coefficient) pairs per instruction, so
+// one record's 45 adds touch two or three cache lines in one or two
+// instructions instead of 45 instructions of scattered lanes. The reference
+// adds the same values one at a time (compare-and-swap loop, core.cl:80-99);
+// either way the order in which runs of different tracks land is the
 // device's.
-__device__ __forceinline__ void deposit_run(float *__restrict__ dst, const float acc[45])
+__device__ __forceinline__ void flush_runs(bool need, int64_t cell, const float acc[45], float *__restrict__ grid,
+                                           RunStage &stage)
 {
+    const uint64_t mask = __ballot(need);
+    if (mask == 0)
+        return;
+    const int lane = __lane_id();
+    if (need) {
+        const int slot = __popcll(mask & ((1ull << lane) - 1ull));
 #pragma unroll
-    for (int i = 0; i < 45; i++)
-        unsafeAtomicAdd(dst + i, acc[i]);
+        for (int i = 0; i < 45; i++)
+            stage.vals[slot * 45 + i] = acc[i];
+        stage.cell[slot] = cell;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int total = __popcll(mask) * 45;
+    for (int p = lane; p < total; p += 64) {
+        const int r = p / 45;
+        unsafeAtomicAdd(grid + stage.cell[r] * 45 + (p - r * 45), stage.vals[p]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // tracking.py:262-357 (kernel tod): boundary seed g from rng_init(rng_seed,
@@ -423,55 +509,96 @@ __device__ __forceinline__ void deposit_run(float *__restrict__ dst, const float
 // (spharm.cl:10-207 at cart2sph(orientation)) on the output cell of the
 // point, then updates, rejects a non-finite or short orientation, and moves.
 // Consecutive steps in one cell are summed in registers (acc = basis on
-// entering a cell, acc += basis after) and flushed with atomics when the
-// track leaves the cell, leaves the grid, or ends.
+// entering a cell, acc += basis after) and flushed when the track leaves the
+// cell, leaves the grid, or ends. For the SH model the deposited basis is the
+// one the update's model evaluation computes at the same orientation (same
+// angles, same values), so it is not evaluated twice. Persistent lanes with
+// seed claiming, as the connectome kernel; each iteration has two flush
+// points (leaving a cell before the update, ending after it) where the
+// wavefront is converged.
 template <int MODEL, int ALGO, int KT>
 __global__ __launch_bounds__(256) void tod_kernel(TodLaunch t)
 {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= t.n_seeds)
-        return;
-    uint2 rng = rng_init(t.rng_seed, (uint32_t)g);
-    const BoundarySample s = sample_boundary(t.vertices, t.triangles, t.cumulative, t.n_triangles, rng);
-    V3 point = s.point, o = s.orientation;
+    __shared__ RunStage stages[4];
+    RunStage &stage = stages[threadIdx.x >> 6];
     const float step = t.algo.step_size;
     const uint32_t stride = t.field.stride;
+    int64_t g = -1;
+    bool drained = false;
+    uint2 rng = make_uint2(0u, 0u);
+    V3 point = v3(0.0f, 0.0f, 0.0f), o = point;
+    uint32_t k = 0;
+    int64_t previous = -1;
     float acc[45];
 #pragma unroll
     for (int i = 0; i < 45; i++)
         acc[i] = 0.0f;
-    int64_t previous = -1;
-    for (uint32_t k = 0; k < t.n_steps; k++) {
-        if (!mask_contains(t.mask, point))
-            break;
-        const int64_t voxel = field_voxel(t.field, point);
-        if (voxel < 0)
-            break;
-        const int64_t cell = grid_cell(t.grid, point);
-        if (cell >= 0) {
-            const bool same = cell == previous;
-            if (!same && previous >= 0)
-                deposit_run(t.coefficients + previous * 45, acc);
-            float az, col;
-            plain::cart2sph(o, az, col);
-            // One basis evaluation for both cases: a select keeps the
-            // reference's `= ylm` (not `0 + ylm`, which differs for -0) on
-            // entering a cell.
-            plain::sh45_terms(az, col, [&](int i, float y, float, float) { acc[i] = same ? acc[i] + y : y; });
-            previous = cell;
-        } else if (previous >= 0) {
-            deposit_run(t.coefficients + previous * 45, acc);
+    while (true) {
+        const int64_t claimed = claim_seed(g < 0, t.queue, t.n_seeds, drained);
+        if (claimed >= 0) {
+            g = claimed;
+            rng = rng_init(t.rng_seed, (uint32_t)g);
+            const BoundarySample s = sample_boundary(t.vertices, t.triangles, t.cumulative, t.n_triangles, rng);
+            point = s.point;
+            o = s.orientation;
+            k = 0;
             previous = -1;
         }
-        o = plain::update<MODEL, ALGO, KT>(t.algo, t.field.data + voxel * stride, o, rng);
-        if (!(__builtin_isfinite(o.x) && __builtin_isfinite(o.y) && __builtin_isfinite(o.z)) || length3(o) < 0.5f)
+        if (__ballot(g >= 0) == 0)
             break;
-        point.x += step * o.x;
-        point.y += step * o.y;
-        point.z += step * o.z;
+        const bool live = g >= 0;
+        bool stop = false;
+        int64_t voxel = -1, cell = -1;
+        if (live) {
+            stop = k >= t.n_steps || !mask_contains(t.mask, point);
+            if (!stop) {
+                voxel = field_voxel(t.field, point);
+                stop = voxel < 0;
+            }
+            if (!stop)
+                cell = grid_cell(t.grid, point);  // -1 off the grid
+        }
+        // Leaving a cell (for another or for no cell) ends its run.
+        flush_runs(live && !stop && previous >= 0 && cell != previous, previous, acc, t.coefficients, stage);
+        if (live && !stop) {
+            const bool in_grid = cell >= 0;
+            const bool same = cell == previous;
+            previous = cell;  // -1: no run
+            // A select keeps the reference's `= ylm` on entering a cell (not
+            // `0 + ylm`, which differs for -0).
+            auto add = [&](int i, float y) {
+                if (in_grid)
+                    acc[i] = same ? acc[i] + y : y;
+            };
+            if constexpr (MODEL != MODEL_SH) {
+                if (in_grid) {
+                    float az, col;
+                    plain::cart2sph(o, az, col);
+                    plain::sh45_terms(az, col, [&](int i, float y, float, float) { add(i, y); });
+                }
+            }
+            o = plain::update<MODEL, ALGO, KT>(t.algo, t.field.data + voxel * stride, o, rng, add);
+            stop = !(__builtin_isfinite(o.x) && __builtin_isfinite(o.y) && __builtin_isfinite(o.z)) ||
+                   length3(o) < 0.5f;
+            if (!stop) {
+                point.x += step * o.x;
+                point.y += step * o.y;
+                point.z += step * o.z;
+                k++;
+            }
+        }
+        // The end of a track ends its run.
+        const bool ended = live && stop;
+        flush_runs(ended && previous >= 0, previous, acc, t.coefficients, stage);
+        if (ended)
+            g = -1;
     }
-    if (previous >= 0)
-        deposit_run(t.coefficients + previous * 45, acc);
+}
+
+__global__ void zero_u32_kernel(uint32_t *counter)
+{
+    if (threadIdx.x == 0)
+        *counter = 0u;
 }
 
 __global__ __launch_bounds__(256) void zero_kernel(float *__restrict__ out, int64_t n)
@@ -591,15 +718,41 @@ hipError_t launch_math(int fn, const float *x, const float *y, int64_t n, float 
     return hipGetLastError();
 }
 
+// Blocks of `kernel` (256 threads) that fill the current device once; never
+// more than the blocks `n` seeds need.
+template <class Kernel> hipError_t persistent_blocks(Kernel kernel, int64_t n, unsigned *blocks)
+{
+    int device = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(kernel), 256, 0);
+    if (e != hipSuccess)
+        return e;
+    const int64_t fill = (int64_t)(cus > 0 ? cus : 1) * (per_cu > 0 ? per_cu : 1);
+    *blocks = (unsigned)std::min<int64_t>(fill, (n + 255) / 256);
+    return hipSuccess;
+}
+
+template <class Kernel, class Launch>
+hipError_t launch_persistent(Kernel kernel, const Launch &args, uint32_t *queue, int64_t n, hipStream_t stream)
+{
+    unsigned blocks = 0;
+    const hipError_t e = persistent_blocks(kernel, n, &blocks);
+    if (e != hipSuccess)
+        return e;
+    zero_u32_kernel<<<1, 64, 0, stream>>>(queue);
+    kernel<<<blocks, 256, 0, stream>>>(args);
+    return hipGetLastError();
+}
+
 template <int MODEL, int ALGO>
 hipError_t launch_connectome_kt(const ConnectomeLaunch &c, hipStream_t stream)
 {
-    const unsigned blocks = blocks_for(c.n_seeds, 256);
     if (MODEL == MODEL_SH && c.algo.n_parameters == 45)
-        connectome_kernel<MODEL, ALGO, 45><<<blocks, 256, 0, stream>>>(c);
-    else
-        connectome_kernel<MODEL, ALGO, 0><<<blocks, 256, 0, stream>>>(c);
-    return hipGetLastError();
+        return launch_persistent(connectome_kernel<MODEL, ALGO, 45>, c, c.queue, c.n_seeds, stream);
+    return launch_persistent(connectome_kernel<MODEL, ALGO, 0>, c, c.queue, c.n_seeds, stream);
 }
 
 hipError_t launch_connectome(const ConnectomeLaunch &c, hipStream_t stream)
@@ -616,12 +769,9 @@ hipError_t launch_connectome(const ConnectomeLaunch &c, hipStream_t stream)
 template <int MODEL, int ALGO>
 hipError_t launch_tod_kt(const TodLaunch &t, hipStream_t stream)
 {
-    const unsigned blocks = blocks_for(t.n_seeds, 256);
     if (MODEL == MODEL_SH && t.algo.n_parameters == 45)
-        tod_kernel<MODEL, ALGO, 45><<<blocks, 256, 0, stream>>>(t);
-    else
-        tod_kernel<MODEL, ALGO, 0><<<blocks, 256, 0, stream>>>(t);
-    return hipGetLastError();
+        return launch_persistent(tod_kernel<MODEL, ALGO, 45>, t, t.queue, t.n_seeds, stream);
+    return launch_persistent(tod_kernel<MODEL, ALGO, 0>, t, t.queue, t.n_seeds, stream);
 }
 
 hipError_t launch_tod(const TodLaunch &t, hipStream_t stream)

@@ -215,11 +215,18 @@ __device__ __forceinline__ void sh45_terms(float phi, float theta, Visit &&visit
 #undef TGK_ACC
 }
 
+// No use for the basis values beyond the contraction.
+struct NoTerms {
+    __device__ __forceinline__ void operator()(int, float) const {}
+};
+
 // Model value and angular derivatives: terms accumulated in index order, so
 // sums round exactly as the reference's `for (i < K)` loop. Coefficients past
-// K arrive as +0 and leave the sums unchanged.
+// K arrive as +0 and leave the sums unchanged. `terms(i, Y_i)` sees every
+// basis value on the way (all 45, whatever K).
+template <class Terms = NoTerms>
 __device__ __forceinline__ void sh45_contract(const float c[45], float phi, float theta, float &v, float &th,
-                                              float &ph)
+                                              float &ph, Terms &&terms = Terms{})
 {
     float sv = 0.0f, sth = 0.0f, sph = 0.0f;
     sh45_terms(phi, theta, [&](int i, float y, float dp, float dt) {
@@ -227,6 +234,7 @@ __device__ __forceinline__ void sh45_contract(const float c[45], float phi, floa
         sv += ci * y;
         sth += ci * dt;
         sph += ci * dp;
+        terms(i, y);
     });
     v = sv;
     th = sth;
@@ -267,15 +275,15 @@ __device__ __forceinline__ void load_sh_coefficients(const float *__restrict__ r
 }
 
 // local.py:91-120 (SH with softplus, K <= 45)
-template <int KT>
-__device__ __forceinline__ ModelValue sh_model(const float *__restrict__ row, int K, V3 o)
+template <int KT, class Terms = NoTerms>
+__device__ __forceinline__ ModelValue sh_model(const float *__restrict__ row, int K, V3 o, Terms &&terms = Terms{})
 {
     float c[45];
     load_sh_coefficients<KT>(row, K, c);
     float az, col;
     cart2sph(o, az, col);
     float v, th, ph;
-    sh45_contract(c, az, col, v, th, ph);
+    sh45_contract(c, az, col, v, th, ph, terms);
     const float d = dsoftplus(v);
     ModelValue r;
     r.value = softplus(v);
@@ -306,21 +314,26 @@ __device__ __forceinline__ ModelValue dti_model(const float *__restrict__ row, V
     return r;
 }
 
-template <int MODEL, int KT>
-__device__ __forceinline__ ModelValue model_evaluate(const float *__restrict__ row, int K, V3 o)
+// `terms` sees the SH basis values at o (SH model only; the DTI model has
+// none).
+template <int MODEL, int KT, class Terms = NoTerms>
+__device__ __forceinline__ ModelValue model_evaluate(const float *__restrict__ row, int K, V3 o,
+                                                     Terms &&terms = Terms{})
 {
     if constexpr (MODEL == MODEL_DTI)
         return dti_model(row, o);
     else
-        return sh_model<KT>(row, K, o);
+        return sh_model<KT>(row, K, o, terms);
 }
 
 // algorithm.py:83-112 (Transport) and 198-228 (Diffusion); the left randn of
-// the noise expression is drawn first.
-template <int MODEL, int ALGO, int KT>
-__device__ __forceinline__ V3 update(const AlgoParams &a, const float *__restrict__ row, V3 o, uint2 &rng)
+// the noise expression is drawn first. For the SH model, `terms(i, Y_i)`
+// receives the basis at cart2sph(o) that the model evaluation computes.
+template <int MODEL, int ALGO, int KT, class Terms = NoTerms>
+__device__ __forceinline__ V3 update(const AlgoParams &a, const float *__restrict__ row, V3 o, uint2 &rng,
+                                     Terms &&terms = Terms{})
 {
-    const ModelValue mv = model_evaluate<MODEL, KT>(row, a.n_parameters, o);
+    const ModelValue mv = model_evaluate<MODEL, KT>(row, a.n_parameters, o, terms);
     float az, col, sp, cp, st, ct;
     cart2sph(o, az, col);
     tgk_sincosf(az, &sp, &cp);
