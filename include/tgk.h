@@ -38,7 +38,7 @@ enum {
 };
 
 enum { TGK_MODEL_SH = 0, TGK_MODEL_DTI = 1 };
-enum { TGK_ALGO_TRANSPORT = 0, TGK_ALGO_DIFFUSION = 1 };
+enum { TGK_ALGO_TRANSPORT = 0, TGK_ALGO_DIFFUSION = 1, TGK_ALGO_PROBABILISTIC = 2, TGK_ALGO_DETERMINISTIC = 3 };
 
 /* Tracking paths. PLAIN is the reference's integrator, bit-exact with the
  * FP32 host restatement. FAST is the persistent, lane-refilling integrator
@@ -59,15 +59,22 @@ typedef struct tgk_domain tgk_domain;
 typedef struct tgk_tracks tgk_tracks;
 
 /* Orientation-update algorithm and its local model: the kernel arguments of
- * Transport (algorithm.py:115-117) and Diffusion (algorithm.py:231-237) plus
- * the model selection of local.from_shape (local.py:265-282). */
+ * Transport (algorithm.py:115-117), Diffusion (algorithm.py:231-237),
+ * Probabilistic (algorithm.py:378-380) and Deterministic
+ * (algorithm.py:619-626) plus the model selection of local.from_shape
+ * (local.py:265-282). Fields an algorithm does not use are ignored. */
 typedef struct {
-    int32_t model;        /* TGK_MODEL_* */
-    int32_t algorithm;    /* TGK_ALGO_* */
-    int32_t n_parameters; /* K: 1..45 for SH, 6 for DTI */
+    int32_t model;          /* TGK_MODEL_* */
+    int32_t algorithm;      /* TGK_ALGO_* */
+    int32_t n_parameters;   /* K: 1..45 for SH, 6 for DTI */
     float step_size;
-    float gamma;
-    float noise_variance; /* Diffusion only */
+    float gamma;            /* Transport, Diffusion */
+    float noise_variance;   /* Diffusion */
+    int32_t n_samples;      /* Probabilistic: candidates per update, >= 1 */
+    float max_angle;        /* Probabilistic: cap radius in [0, pi] */
+    int32_t max_iterations; /* Deterministic: >= 0 */
+    float tolerance;        /* Deterministic: gradient-norm threshold */
+    float max_step_size;    /* Deterministic: largest update angle */
 } tgk_algorithm;
 
 const char *tgk_last_error(void);

@@ -22,7 +22,7 @@ extern "C" {
 #endif
 
 enum { TGO_MODEL_SH = 0, TGO_MODEL_DTI = 1 };
-enum { TGO_ALGO_TRANSPORT = 0, TGO_ALGO_DIFFUSION = 1 };
+enum { TGO_ALGO_TRANSPORT = 0, TGO_ALGO_DIFFUSION = 1, TGO_ALGO_PROBABILISTIC = 2, TGO_ALGO_DETERMINISTIC = 3 };
 enum { TGO_DOMAIN_MASK = 0, TGO_DOMAIN_MESH = 1 };
 
 typedef struct {
@@ -32,6 +32,11 @@ typedef struct {
     float step_size;
     float gamma;
     float noise_variance;
+    int32_t n_samples;      /* Probabilistic */
+    float max_angle;        /* Probabilistic */
+    int32_t max_iterations; /* Deterministic */
+    float tolerance;        /* Deterministic */
+    float max_step_size;    /* Deterministic */
 } tgo_algorithm;
 
 typedef struct {

@@ -311,8 +311,156 @@ static inline MV FN(model_evaluate)(int model, int n_parameters, const R *params
 }
 
 /* algorithm.py:83-112 (Transport) and 198-228 (Diffusion). */
+static inline VEC FN(cross)(VEC a, VEC b)
+{
+    return FN(mkvec)(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline R FN(dot)(VEC a, VEC b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+/* The OpenCL normalize builtin, taken as v / length(v). */
+static inline VEC FN(normalize)(VEC v)
+{
+    const R n = FN(length)(v);
+    return FN(mkvec)(v.x / n, v.y / n, v.z / n);
+}
+
+/* algorithm.py:340-343: a direction uniform on the cap of cos >= cos_max
+ * around o (two uniform draws: the cosine, then the azimuth). */
+static inline VEC FN(cap_candidate)(VEC o, VEC t1, VEC t2, R cos_max, uint32_t rng[2])
+{
+    const R cos_angle = K(1.0) - (R)tgo_randu(rng) * (K(1.0) - cos_max);
+    const R sin_angle = SQRT(FMAX(K(0.0), K(1.0) - cos_angle * cos_angle));
+    const R azimuth = (2 * PI_R) * (R)tgo_randu(rng);
+    R sa, ca;
+    SINCOS(azimuth, &sa, &ca);
+    return FN(mkvec)(cos_angle * o.x + sin_angle * (ca * t1.x + sa * t2.x),
+                     cos_angle * o.y + sin_angle * (ca * t1.y + sa * t2.y),
+                     cos_angle * o.z + sin_angle * (ca * t1.z + sa * t2.z));
+}
+
+/* algorithm.py:315-376 (Probabilistic): weights max(value, 0) (0 if not
+ * finite) of n_samples cap candidates, target = randu * total, then the
+ * noise state replayed and the first candidate whose running weight exceeds
+ * the target returned; the normalised input when the total is not positive
+ * or no candidate exceeds the target. */
+static VEC FN(probabilistic_one)(const tgo_algorithm *a, const R *params, VEC o_in, uint32_t rng[2])
+{
+    const R norm = FN(length)(o_in);
+    const VEC o = norm <= K(1e-8) ? FN(mkvec)(K(0.0), K(0.0), K(1.0))
+                                  : FN(mkvec)(o_in.x / norm, o_in.y / norm, o_in.z / norm);
+    const VEC helper = FABS(o.z) < K(0.9) ? FN(mkvec)(K(0.0), K(0.0), K(1.0)) : FN(mkvec)(K(0.0), K(1.0), K(0.0));
+    const VEC t1 = FN(normalize)(FN(cross)(helper, o));
+    const VEC t2 = FN(cross)(o, t1);
+    const R cos_max = COS_OF((R)a->max_angle);
+    const uint32_t initial[2] = {rng[0], rng[1]};
+    R total = K(0.0);
+    for (int i = 0; i < a->n_samples; i++) {
+        const R v = FN(model_evaluate)(a->model, a->n_parameters, params,
+                                       FN(cap_candidate)(o, t1, t2, cos_max, rng)).value;
+        total += isfinite(v) ? FMAX(v, K(0.0)) : K(0.0);
+    }
+    if (total <= K(0.0))
+        return o;
+    const R target = (R)tgo_randu(rng) * total;
+    rng[0] = initial[0];
+    rng[1] = initial[1];
+    R cumulative = K(0.0);
+    for (int i = 0; i < a->n_samples; i++) {
+        const VEC c = FN(cap_candidate)(o, t1, t2, cos_max, rng);
+        const R v = FN(model_evaluate)(a->model, a->n_parameters, params, c).value;
+        cumulative += isfinite(v) ? FMAX(v, K(0.0)) : K(0.0);
+        if (cumulative > target)
+            return c;
+    }
+    return o;
+}
+
+/* algorithm.py:500-513: model gradient on the sphere (dphi already carries
+ * the 1 / sin(theta)). */
+static inline VEC FN(sphere_gradient)(const tgo_algorithm *a, const R *params, VEC o)
+{
+    const MV m = FN(model_evaluate)(a->model, a->n_parameters, params, o);
+    R az, col, sp, cp, st, ct;
+    FN(cart2sph_one)(o, &az, &col);
+    SINCOS(az, &sp, &cp);
+    SINCOS(col, &st, &ct);
+    return FN(mkvec)(m.dtheta * (ct * cp) + m.dphi * (-sp), m.dtheta * (ct * sp) + m.dphi * cp,
+                     m.dtheta * (-st) + m.dphi * K(0.0));
+}
+
+/* algorithm.py:535-598 (Deterministic): Newton ascent with a central-
+ * difference Hessian (h = 0.01), 0.1 gradient steps off negative-definite
+ * regions, steps capped at max_step_size, 8-halving line search on
+ * value > f0 + 1e-7. */
+static VEC FN(deterministic_one)(const tgo_algorithm *a, const R *params, VEC o_in)
+{
+    const R h = K(1e-2);
+    VEC o = FN(normalize)(o_in);
+    for (int iteration = 0; iteration < a->max_iterations; iteration++) {
+        R az, col, sp, cp, st, ct;
+        FN(cart2sph_one)(o, &az, &col);
+        SINCOS(az, &sp, &cp);
+        SINCOS(col, &st, &ct);
+        const VEC e1 = FN(mkvec)(ct * cp, ct * sp, -st);
+        const VEC e2 = FN(mkvec)(-sp, cp, K(0.0));
+        const MV current = FN(model_evaluate)(a->model, a->n_parameters, params, o);
+        const R f0 = current.value, g1 = current.dtheta, g2 = current.dphi;
+        if (SQRT(g1 * g1 + g2 * g2) <= (R)a->tolerance)
+            break;
+        const VEC step1 = FN(mkvec)(h * e1.x, h * e1.y, h * e1.z);
+        const VEC step2 = FN(mkvec)(h * e2.x, h * e2.y, h * e2.z);
+        const VEC gp1 = FN(sphere_gradient)(a, params, FN(exps2)(o, step1, K(1.0)));
+        const VEC gm1 = FN(sphere_gradient)(a, params, FN(exps2)(o, FN(mkvec)(-step1.x, -step1.y, -step1.z), K(1.0)));
+        const VEC gp2 = FN(sphere_gradient)(a, params, FN(exps2)(o, step2, K(1.0)));
+        const VEC gm2 = FN(sphere_gradient)(a, params, FN(exps2)(o, FN(mkvec)(-step2.x, -step2.y, -step2.z), K(1.0)));
+        const VEC dg1 = FN(mkvec)(gp1.x - gm1.x, gp1.y - gm1.y, gp1.z - gm1.z);
+        const VEC dg2 = FN(mkvec)(gp2.x - gm2.x, gp2.y - gm2.y, gp2.z - gm2.z);
+        const R h11 = FN(dot)(dg1, e1) / (K(2.0) * h);
+        const R h22 = FN(dot)(dg2, e2) / (K(2.0) * h);
+        const R h12a = FN(dot)(dg2, e1) / (K(2.0) * h);
+        const R h12b = FN(dot)(dg1, e2) / (K(2.0) * h);
+        const R h12 = K(0.5) * (h12a + h12b);
+        const R determinant = h11 * h22 - h12 * h12;
+        R d1, d2;
+        if (h11 < K(-1e-7) && h22 < K(-1e-7) && determinant > K(1e-12)) {
+            d1 = (-h22 * g1 + h12 * g2) / determinant;
+            d2 = (h12 * g1 - h11 * g2) / determinant;
+        } else {
+            const R gradient_norm = SQRT(g1 * g1 + g2 * g2);
+            d1 = K(0.1) * g1 / gradient_norm;
+            d2 = K(0.1) * g2 / gradient_norm;
+        }
+        const R step_norm = SQRT(d1 * d1 + d2 * d2);
+        if (step_norm > (R)a->max_step_size) {
+            const R scale = (R)a->max_step_size / step_norm;
+            d1 *= scale;
+            d2 *= scale;
+        }
+        if (SQRT(d1 * d1 + d2 * d2) <= K(1e-7))
+            break;
+        const VEC tangent = FN(mkvec)(d1 * e1.x + d2 * e2.x, d1 * e1.y + d2 * e2.y, d1 * e1.z + d2 * e2.z);
+        int accepted = 0;
+        R scale = K(1.0);
+        for (int search = 0; search < 8; search++) {
+            const VEC c = FN(exps2)(o, FN(mkvec)(scale * tangent.x, scale * tangent.y, scale * tangent.z), K(1.0));
+            if (FN(model_evaluate)(a->model, a->n_parameters, params, c).value > f0 + K(1e-7)) {
+                accepted = 1;
+                o = c;
+                break;
+            }
+            scale *= K(0.5);
+        }
+        if (!accepted)
+            break;
+    }
+    return o;
+}
+
 static inline VEC FN(update_one)(const tgo_algorithm *a, const R *params, VEC o, uint32_t rng[2])
 {
+    if (a->algorithm == TGO_ALGO_PROBABILISTIC)
+        return FN(probabilistic_one)(a, params, o, rng);
+    if (a->algorithm == TGO_ALGO_DETERMINISTIC)
+        return FN(deterministic_one)(a, params, o);
     const MV mv = FN(model_evaluate)(a->model, a->n_parameters, params, o);
     R az, col, sp, cp, st, ct;
     FN(cart2sph_one)(o, &az, &col);

@@ -24,6 +24,11 @@ class Algorithm(ctypes.Structure):
         ("step_size", c_float),
         ("gamma", c_float),
         ("noise_variance", c_float),
+        ("n_samples", c_int32),
+        ("max_angle", c_float),
+        ("max_iterations", c_int32),
+        ("tolerance", c_float),
+        ("max_step_size", c_float),
     ]
 
 
@@ -64,7 +69,8 @@ def _p(a):
 
 def algorithm_struct(algorithm) -> Algorithm:
     d = algorithm.descriptor()
-    return Algorithm(d.model, d.algorithm, d.n_parameters, d.step_size, d.gamma, d.noise_variance)
+    return Algorithm(d.model, d.algorithm, d.n_parameters, d.step_size, d.gamma, d.noise_variance, d.n_samples,
+                     d.max_angle, d.max_iterations, d.tolerance, d.max_step_size)
 
 
 class SceneHolder:

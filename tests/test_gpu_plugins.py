@@ -1,0 +1,152 @@
+"""GPU parity of the Probabilistic and Deterministic algorithms
+(algorithm.py:272-677) on the plain path: one-op updates, tracking, the
+connectome and the TOD bit-exact with the FP32 restatement, and the
+reference's tests of both algorithms."""
+
+import numpy as np
+import pytest
+
+import bench
+import fixtures
+import oracle
+import tractography_amd as tg
+from tractography_amd import _native
+
+from test_gpu_plain import _compare_tracks
+from test_gpu_tod import _device_tod, assert_same_sums
+from test_gpu_connectome import _device_connectome
+
+pytestmark = pytest.mark.gpu
+
+
+def _algorithm(name, model, seed=4, step_size=0.1):
+    if name == "probabilistic":
+        return tg.algorithm.Probabilistic(model, n_samples=24, max_angle=0.5, seed=seed, step_size=step_size)
+    return tg.algorithm.Deterministic(model, max_iterations=8, tolerance=1e-5, max_step_size=0.4,
+                                      step_size=step_size)
+
+
+@pytest.mark.parametrize("name", ["probabilistic", "deterministic"])
+@pytest.mark.parametrize("K", [45, 15, 6])
+def test_update_orientation_bit_exact(runtime, name, K):
+    rng = np.random.default_rng(K + len(name))
+    model = tg.local.DiffusionTensorImaging() if K == 6 else tg.local.RealSymmetricSphericalHarmonics(K)
+    params = rng.normal(size=(4000, K)).astype(np.float32)
+    if K != 6:
+        params[:, 0] = np.abs(params[:, 0]) + 1
+    o = rng.standard_normal((4000, 3)).astype(np.float32)
+    algorithm = _algorithm(name, model)
+    twin = _algorithm(name, model)
+    got = tg.algorithm.update_orientation(algorithm, params, o, runtime)
+    seed = twin.get_rng_seed()
+    np.testing.assert_array_equal(got, oracle.update(algorithm, seed, params, o, 32))
+
+
+@pytest.mark.parametrize("name", ["probabilistic", "deterministic"])
+def test_plain_tracking_bit_exact(runtime, name, oracle_threads):
+    cfg = bench.sphere32_transport(seed=0, n_seeds=1500)
+    field = tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4)))
+    domain = tg.domain.Image(tg.Nifti1Image(cfg["mask"], np.eye(4)))
+    algorithm = _algorithm(name, tg.local.RealSymmetricSphericalHarmonics(45), seed=6)
+    rng = [algorithm.get_rng_seed() for _ in range(3)]
+    got = tg.tracking.track(algorithm, domain, field, cfg["seeds"], rng, 500, 0.25, 20.0, runtime)
+    ref = oracle.track(domain, field, algorithm, cfg["seeds"], rng, 500, 0.25, 20.0, threads=oracle_threads)
+    _compare_tracks(got, ref)
+    print(f"[plugins {name}] {int(got.steps.sum())} steps", flush=True)
+    assert got.steps.sum() > 50_000
+
+
+@pytest.mark.parametrize("name", ["probabilistic", "deterministic"])
+def test_plain_tracking_bit_exact_dti_and_generic_k(runtime, name, oracle_threads):
+    tensor, wm, _ = fixtures.tensor_cross()
+    np.random.seed(2)
+    seeds = tg.seeds.from_mask(wm, 1500)
+    algorithm = _algorithm(name, tg.local.DiffusionTensorImaging(), seed=7, step_size=0.05)
+    rng = [algorithm.get_rng_seed() for _ in range(2)]
+    field, domain = tg.field.Image(tensor), tg.domain.Image(wm)
+    got = tg.tracking.track(algorithm, domain, field, seeds, rng, 1000, 0.25, 10.0, runtime)
+    _compare_tracks(got, oracle.track(domain, field, algorithm, seeds, rng, 1000, 0.25, 10.0,
+                                      threads=oracle_threads))
+    fod, wm, _ = fixtures.cross(shape=(14, 14, 3), n_coefficients=15)
+    seeds = tg.seeds.from_mask(wm, 1500)
+    algorithm = _algorithm(name, tg.local.RealSymmetricSphericalHarmonics(15), seed=8, step_size=0.1)
+    rng = [algorithm.get_rng_seed() for _ in range(2)]
+    field, domain = tg.field.Image(fod), tg.domain.Image(wm)
+    got = tg.tracking.track(algorithm, domain, field, seeds, rng, 1000, 0.5, 15.0, runtime)
+    _compare_tracks(got, oracle.track(domain, field, algorithm, seeds, rng, 1000, 0.5, 15.0,
+                                      threads=oracle_threads))
+
+
+@pytest.mark.parametrize("name", ["probabilistic", "deterministic"])
+def test_connectome_and_tod_with_plugins(runtime, name, oracle_threads):
+    fod, wm, _ = fixtures.cross(shape=(14, 14, 3), n_coefficients=45)
+    domain, field = tg.domain.Image(wm), tg.field.Image(fod)
+    algorithm = _algorithm(name, tg.local.RealSymmetricSphericalHarmonics(45), seed=9, step_size=0.2)
+    seed = algorithm.get_rng_seed()
+    got = _device_connectome(runtime, domain, field, algorithm, seed, 3000, 100, 0.0)
+    ref = oracle.connectome(domain, field, algorithm, seed, 3000, 100, 0.0, threads=oracle_threads)
+    for a, b in zip(got, ref):
+        np.testing.assert_array_equal(a, b)
+    dims = np.array(wm.shape)
+    inverse = np.eye(4, dtype=np.float32)
+    got = _device_tod(runtime, domain, field, algorithm, seed, 1500, 100, dims, inverse)
+    raw, abs_sum, flushes = oracle.tod(domain, field, algorithm, seed, 1500, 100, dims, inverse)
+    assert flushes.sum() > 1000
+    assert_same_sums(got, raw, abs_sum, flushes, name)
+
+
+def test_fast_path_refuses_the_plugins(runtime):
+    fod, wm, _ = fixtures.cross()
+    fast = tg.HipRuntime(path="fast")
+    np.random.seed(0)
+    seeds = tg.seeds.from_mask(wm, 10)
+    for algorithm in (_algorithm("probabilistic", tg.local.RealSymmetricSphericalHarmonics(45)),
+                      _algorithm("deterministic", tg.local.RealSymmetricSphericalHarmonics(45))):
+        with pytest.raises(RuntimeError, match="plain path"):
+            tg.tractogram(fod, seeds, algorithm=algorithm, runtime=fast, maximum_streamline_length=2.0)
+
+
+def test_probabilistic_exact_zero_angle_limit(runtime):
+    # test/test_algorithms/test_probabilistic.py:12-27
+    algorithm = tg.algorithm.Probabilistic(tg.local.RealSymmetricSphericalHarmonics(45), max_angle=0.0, seed=0)
+    initial = np.asarray([tg.core.sph2cart(0.5, 0.12, 1.0)])
+    coefficients = fixtures.u0u3_coefficients()[np.newaxis, :]
+    result = tg.algorithm.update_orientation(algorithm, coefficients, initial, runtime)
+    np.testing.assert_allclose(result, initial, atol=1e-6)
+
+
+def test_deterministic_exact_cross_term_maximum(runtime):
+    # test/test_algorithms/test_deterministic.py:12-29
+    algorithm = tg.algorithm.Deterministic(tg.local.RealSymmetricSphericalHarmonics(45), max_iterations=30,
+                                           tolerance=1e-6, max_step_size=0.5)
+    initial = np.asarray([tg.core.sph2cart(0.5, 0.12, 1.0)])
+    coefficients = fixtures.u0u3_coefficients()[np.newaxis, :]
+    result = tg.algorithm.update_orientation(algorithm, coefficients, initial, runtime)
+    np.testing.assert_allclose(result[0], np.asarray([1.0, 0.0, 1.0]) / np.sqrt(2.0), atol=1e-3)
+
+
+def _run_tractogram(field, seeds, algorithm_type, runtime):
+    # test/test_algorithms/__init__.py:6-52 (make_algorithm's defaults)
+    local_model = tg.local.from_shape(field.shape)
+    options = {
+        tg.algorithm.Deterministic: {"step_size": 0.25, "max_step_size": np.deg2rad(30.0)},
+        tg.algorithm.Probabilistic: {"step_size": 0.25, "max_angle": np.deg2rad(20.0)},
+    }[algorithm_type]
+    return tg.tractogram(field, seeds, algorithm=algorithm_type(local_model, **options), save_at=0.25,
+                         maximum_streamline_length=300.0, batch_size=50000, runtime=runtime)
+
+
+@pytest.mark.parametrize("algorithm_type", [tg.algorithm.Probabilistic, tg.algorithm.Deterministic])
+def test_reference_tractograms(runtime, algorithm_type):
+    # test/test_algorithms/test_probabilistic.py:30-49,
+    # test/test_algorithms/test_deterministic.py:32-51
+    fod = fixtures.uniform_isotropic()
+    nii = tg.Nifti1Image(fod, np.eye(4))
+    seeds = tg.seeds.from_fod(nii, 1000, runtime=runtime)
+    result = _run_tractogram(nii, seeds, algorithm_type, runtime)
+    assert len(result) == 1000
+    fod, wm, _ = fixtures.cross()
+    np.random.seed(0)
+    seeds = tg.seeds.from_mask(wm, 1000)
+    result = _run_tractogram(fod, seeds, algorithm_type, runtime)
+    assert len(result) == 1000

@@ -24,6 +24,8 @@ MODEL_SH = 0
 MODEL_DTI = 1
 ALGO_TRANSPORT = 0
 ALGO_DIFFUSION = 1
+ALGO_PROBABILISTIC = 2
+ALGO_DETERMINISTIC = 3
 PATH_PLAIN = 0
 PATH_FAST = 1
 PRECISIONS = {"fp32": 0, "bf16x2": 1, "fp16": 2, "bf16": 3}
@@ -41,6 +43,11 @@ class tgk_algorithm(ctypes.Structure):
         ("step_size", c_float),
         ("gamma", c_float),
         ("noise_variance", c_float),
+        ("n_samples", c_int32),
+        ("max_angle", c_float),
+        ("max_iterations", c_int32),
+        ("tolerance", c_float),
+        ("max_step_size", c_float),
     ]
 
 
@@ -160,5 +167,7 @@ def ptr(array: np.ndarray | None) -> int | None:
 
 
 def algorithm_struct(model: int, algorithm: int, n_parameters: int, step_size: float, gamma: float,
-                     noise_variance: float) -> tgk_algorithm:
-    return tgk_algorithm(model, algorithm, n_parameters, step_size, gamma, noise_variance)
+                     noise_variance: float, n_samples: int = 0, max_angle: float = 0.0, max_iterations: int = 0,
+                     tolerance: float = 0.0, max_step_size: float = 0.0) -> tgk_algorithm:
+    return tgk_algorithm(model, algorithm, n_parameters, step_size, gamma, noise_variance, n_samples, max_angle,
+                         max_iterations, tolerance, max_step_size)

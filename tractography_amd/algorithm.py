@@ -1,9 +1,11 @@
 """Orientation-update algorithms (algorithm.py of the reference).
 
-``Transport`` (tangent drift along the gradient of the local model) and
-``Diffusion`` (drift plus tangent-plane Gaussian noise) keep the reference's
-constructors and NumPy twins; their device implementations are the HIP
-kernels, selected by the algorithm descriptor.
+``Transport`` (tangent drift along the gradient of the local model),
+``Diffusion`` (drift plus tangent-plane Gaussian noise), ``Probabilistic``
+(a candidate drawn from a spherical cap with probability proportional to the
+model value) and ``Deterministic`` (Newton ascent of the model on the sphere)
+keep the reference's constructors and NumPy twins; their device
+implementations are the HIP kernels, selected by the algorithm descriptor.
 """
 
 from __future__ import annotations
@@ -58,6 +60,11 @@ class BaseAlgorithm(ABC):
             float(np.float32(self.step_size)),
             float(np.float32(getattr(self, "gamma", 1.0))),
             float(np.float32(getattr(self, "noise_variance", 0.0))),
+            int(getattr(self, "n_samples", 0)),
+            float(np.float32(getattr(self, "max_angle", 0.0))),
+            int(getattr(self, "max_iterations", 0)),
+            float(np.float32(getattr(self, "tolerance", 0.0))),
+            float(np.float32(getattr(self, "max_step_size", 0.0))),
         )
 
 
@@ -143,6 +150,162 @@ class Diffusion(BaseAlgorithm):
         scale_noise = np.sqrt(self.noise_variance * self.gamma * self.step_size)
         tangent = scale_drift * drift + scale_noise * noise
         return exps2(orientations, tangent, 1.0)
+
+
+class Probabilistic(BaseAlgorithm):
+    """Sample a new orientation near the current one (algorithm.py:272-436).
+
+    ``n_samples`` candidate directions are drawn uniformly from the spherical
+    cap of radius ``max_angle`` around each orientation, and one is chosen
+    with probability proportional to its non-negative local-model value.
+
+    Args:
+        local_model: Model whose non-negative value is the sampling density.
+        n_samples: Candidates per update.
+        max_angle: Cap radius in radians, in [0, pi].
+        seed: Seed of the NumPy generator that draws the device RNG seeds
+            (and the NumPy twin's samples).
+        step_size: Tracking step; it does not affect sampling.
+    """
+
+    algorithm_id = _native.ALGO_PROBABILISTIC
+
+    def __init__(
+        self,
+        local_model: BaseLocalModel,
+        n_samples: int = 64,
+        max_angle: float = np.pi / 12,
+        seed: int | None = None,
+        step_size: float = 0.25,
+    ):
+        super().__init__(local_model, step_size=step_size)
+        if n_samples <= 0:
+            raise ValueError("n_samples must be positive")
+        if not np.isfinite(max_angle) or not 0.0 <= max_angle <= np.pi:
+            raise ValueError("max_angle must be finite and between 0 and pi")
+        self.n_samples = int(n_samples)
+        self.max_angle = float(max_angle)
+        self._rng = np.random.default_rng(seed)
+
+    def get_rng_seed(self) -> int:
+        """One draw per tracking batch (algorithm.py:382-384)."""
+        return int(self._rng.integers(0, np.iinfo(np.uint32).max, dtype=np.uint32))
+
+    def update_orientation(self, local_models: np.ndarray, orientations: Orientations):
+        """NumPy twin (algorithm.py:386-436): its own NumPy random stream."""
+        n_orientations = orientations.shape[0]
+        result_dtype = np.result_type(orientations, np.float32)
+        if n_orientations == 0:
+            return np.empty((0, 3), dtype=result_dtype)
+        units = np.asarray(orientations, dtype=result_dtype).copy()
+        norms = np.linalg.norm(units, axis=1)
+        invalid = norms <= 1e-8
+        units[~invalid] /= norms[~invalid, np.newaxis]
+        units[invalid] = (0.0, 0.0, 1.0)
+        helpers = np.zeros_like(units)
+        use_z = np.abs(units[:, 2]) < 0.9
+        helpers[use_z, 2] = 1.0
+        helpers[~use_z, 1] = 1.0
+        tangent1 = np.cross(helpers, units)
+        tangent1 /= np.linalg.norm(tangent1, axis=1, keepdims=True)
+        tangent2 = np.cross(units, tangent1)
+        draws = self._rng.random((n_orientations, self.n_samples, 2))
+        cos_max = np.cos(self.max_angle)
+        cos_angles = 1.0 - draws[:, :, 0] * (1.0 - cos_max)
+        sin_angles = np.sqrt(np.maximum(0.0, 1.0 - cos_angles**2))
+        azimuths = 2.0 * np.pi * draws[:, :, 1]
+        candidates = cos_angles[:, :, None] * units[:, None, :] + sin_angles[:, :, None] * (
+            np.cos(azimuths)[:, :, None] * tangent1[:, None, :] + np.sin(azimuths)[:, :, None] * tangent2[:, None, :]
+        )
+        repeated = np.repeat(local_models, self.n_samples, axis=0)
+        values = self.local_model.evaluate(repeated, candidates.reshape(-1, 3))[:, 0].reshape(
+            n_orientations, self.n_samples
+        )
+        weights = np.where(np.isfinite(values), np.maximum(values, 0.0), 0.0)
+        totals = weights.sum(axis=1)
+        result = units.copy()
+        valid = totals > 0.0
+        if np.any(valid):
+            targets = self._rng.random(np.count_nonzero(valid)) * totals[valid]
+            cumulative = np.cumsum(weights[valid], axis=1)
+            selected = np.minimum(np.sum(cumulative <= targets[:, None], axis=1), self.n_samples - 1)
+            result[valid] = candidates[valid, selected]
+        return result.astype(result_dtype, copy=False)
+
+
+class Deterministic(BaseAlgorithm):
+    """Maximise the local model near each direction (algorithm.py:439-677).
+
+    The device update is Newton ascent in the tangent frame: gradient from
+    the model, Hessian from central differences of nearby gradients, a fixed
+    small gradient step where that Hessian is not negative definite, steps
+    capped at ``max_step_size`` and a backtracking line search. The NumPy
+    twin maximises with SciPy's SLSQP under a unit-norm constraint, as the
+    reference's does.
+
+    Args:
+        local_model: Model whose value is maximised.
+        max_iterations: Iterations per update.
+        tolerance: Gradient-norm threshold (device) / SLSQP ftol (twin).
+        max_step_size: Largest device update angle, in radians.
+        step_size: Tracking step; it does not enter the update.
+    """
+
+    algorithm_id = _native.ALGO_DETERMINISTIC
+
+    def __init__(
+        self,
+        local_model: BaseLocalModel,
+        max_iterations: int = 10,
+        tolerance: float = 1e-5,
+        max_step_size: float = 0.5,
+        step_size: float = 0.1,
+    ):
+        super().__init__(local_model, step_size=step_size)
+        self.max_iterations = max_iterations
+        self.tolerance = tolerance
+        self.max_step_size = max_step_size
+
+    def update_orientation(self, local_models: np.ndarray, orientations: Orientations):
+        """NumPy twin (algorithm.py:628-677): SciPy SLSQP per orientation."""
+        from scipy.optimize import minimize
+
+        result = np.array(orientations, dtype=np.result_type(orientations, np.float32), copy=True)
+        for i, initial in enumerate(result):
+            initial = np.asarray(initial, dtype=np.float64)
+            initial /= np.linalg.norm(initial)
+            parameters = local_models[i : i + 1]
+
+            def objective(orientation, parameters=parameters):
+                norm = np.linalg.norm(orientation)
+                direction = orientation / norm
+                value = self.local_model.evaluate(parameters, direction[None, :])[0]
+                phi, theta, _ = cart2sph(*direction)
+                st, ct = np.sin(theta), np.cos(theta)
+                sp, cp = np.sin(phi), np.cos(phi)
+                e1 = np.array([ct * cp, ct * sp, -st])
+                e2 = np.array([-sp, cp, 0.0])
+                gradient = value[1] * e1 + value[2] * e2
+                gradient = (gradient - direction * np.dot(direction, gradient)) / norm
+                return -value[0], -gradient
+
+            if self.max_iterations <= 0:
+                result[i] = initial
+                continue
+            optimisation = minimize(
+                objective,
+                initial,
+                method="SLSQP",
+                jac=True,
+                constraints={
+                    "type": "eq",
+                    "fun": lambda orientation: np.dot(orientation, orientation) - 1.0,
+                    "jac": lambda orientation: 2.0 * orientation,
+                },
+                options={"ftol": self.tolerance, "maxiter": self.max_iterations},
+            )
+            result[i] = optimisation.x / np.linalg.norm(optimisation.x)
+        return result
 
 
 def exps2(p: np.ndarray, x: np.ndarray, t: float) -> np.ndarray:

@@ -188,8 +188,16 @@ int check_algorithm(const tgk_algorithm *a, const tgk_field *f, int max_sh = 45)
         return fail(TGK_ERR_INVALID, "algorithm is NULL");
     if (a->model != TGK_MODEL_SH && a->model != TGK_MODEL_DTI)
         return fail(TGK_ERR_INVALID, "unknown model %d", a->model);
-    if (a->algorithm != TGK_ALGO_TRANSPORT && a->algorithm != TGK_ALGO_DIFFUSION)
+    if (a->algorithm < TGK_ALGO_TRANSPORT || a->algorithm > TGK_ALGO_DETERMINISTIC)
         return fail(TGK_ERR_INVALID, "unknown algorithm %d", a->algorithm);
+    if (a->algorithm == TGK_ALGO_PROBABILISTIC) {
+        if (a->n_samples < 1)
+            return fail(TGK_ERR_INVALID, "n_samples must be positive");
+        if (!std::isfinite(a->max_angle) || !(a->max_angle >= 0.0f) || !(a->max_angle <= 3.14159265358979f))
+            return fail(TGK_ERR_INVALID, "max_angle must be finite and between 0 and pi");
+    }
+    if (a->algorithm == TGK_ALGO_DETERMINISTIC && a->max_iterations < 0)
+        return fail(TGK_ERR_INVALID, "max_iterations must be non-negative");
     if (a->model == TGK_MODEL_DTI && a->n_parameters != 6)
         return fail(TGK_ERR_INVALID, "DTI needs 6 parameters, got %d", a->n_parameters);
     if (a->model == TGK_MODEL_SH && (a->n_parameters < 1 || a->n_parameters > max_sh))
@@ -209,6 +217,11 @@ tgk::AlgoParams algo_params(const tgk_algorithm *a)
     p.step_size = a->step_size;
     p.gamma = a->gamma;
     p.noise_variance = a->noise_variance;
+    p.n_samples = a->n_samples;
+    p.max_angle = a->max_angle;
+    p.max_iterations = a->max_iterations;
+    p.tolerance = a->tolerance;
+    p.max_step_size = a->max_step_size;
     return p;
 }
 
@@ -551,6 +564,9 @@ int check_track_args(const tgk_domain *domain, const tgk_field *field, const tgk
         return fail(TGK_ERR_INVALID, "unknown path %d", path);
     if (int r = check_algorithm(algorithm, field, path == TGK_PATH_FAST ? 91 : 45))
         return r;
+    if (path == TGK_PATH_FAST && algorithm->algorithm >= TGK_ALGO_PROBABILISTIC)
+        return fail(TGK_ERR_UNSUPPORTED, "the fast path runs Transport and Diffusion; Probabilistic and "
+                                         "Deterministic run on the plain path");
     if (n_seeds < 0)
         return fail(TGK_ERR_INVALID, "negative seed count");
     if (n_seeds > MAX_SEEDS)

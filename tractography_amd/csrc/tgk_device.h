@@ -15,7 +15,7 @@
 namespace tgk {
 
 enum Model : int32_t { MODEL_SH = 0, MODEL_DTI = 1 };
-enum Algo : int32_t { ALGO_TRANSPORT = 0, ALGO_DIFFUSION = 1 };
+enum Algo : int32_t { ALGO_TRANSPORT = 0, ALGO_DIFFUSION = 1, ALGO_PROBABILISTIC = 2, ALGO_DETERMINISTIC = 3 };
 enum DomainKind : int32_t { DOMAIN_MASK = 0, DOMAIN_MESH = 1 };
 
 constexpr uint32_t NO_HIT = 0xffffffffu;
@@ -66,6 +66,11 @@ struct AlgoParams {
     float step_size;
     float gamma;
     float noise_variance;
+    int32_t n_samples;       // Probabilistic: candidates per update
+    float max_angle;         // Probabilistic: cap radius (rad)
+    int32_t max_iterations;  // Deterministic: Newton iterations
+    float tolerance;         // Deterministic: gradient-norm threshold
+    float max_step_size;     // Deterministic: largest update angle (rad)
 };
 
 struct TrackLaunch {

@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void track_plain_kernel(TrackLaunch p)
     V3 point = v3(s0.x, s0.y, s0.z);
     V3 o = v3(s1.x, s1.y, s1.z);
     uint2 rng = make_uint2(0u, 0u);
-    if constexpr (ALGO == ALGO_DIFFUSION)
+    if constexpr (ALGO == ALGO_DIFFUSION || ALGO == ALGO_PROBABILISTIC)
         rng = seed_stream(p, p.first_seed + g);
 
     store_point_major(p.points, p.n_seeds, 0, g, point);
@@ -110,9 +110,12 @@ hipError_t launch_plain_domain(const TrackLaunch &p, hipStream_t stream)
 template <int MODEL>
 hipError_t launch_plain_algo(const TrackLaunch &p, hipStream_t stream)
 {
-    if (p.algo.algorithm == ALGO_DIFFUSION)
-        return launch_plain_domain<MODEL, ALGO_DIFFUSION>(p, stream);
-    return launch_plain_domain<MODEL, ALGO_TRANSPORT>(p, stream);
+    switch (p.algo.algorithm) {
+    case ALGO_DIFFUSION: return launch_plain_domain<MODEL, ALGO_DIFFUSION>(p, stream);
+    case ALGO_PROBABILISTIC: return launch_plain_domain<MODEL, ALGO_PROBABILISTIC>(p, stream);
+    case ALGO_DETERMINISTIC: return launch_plain_domain<MODEL, ALGO_DETERMINISTIC>(p, stream);
+    default: return launch_plain_domain<MODEL, ALGO_TRANSPORT>(p, stream);
+    }
 }
 
 // Point-major dense [n][seed][3] -> ragged points in seed order. A block owns
@@ -570,7 +573,10 @@ __global__ __launch_bounds__(256) void tod_kernel(TodLaunch t)
                 if (in_grid)
                     acc[i] = same ? acc[i] + y : y;
             };
-            if constexpr (MODEL != MODEL_SH) {
+            // The update's own model evaluation at o supplies the basis for
+            // SH under Transport and Diffusion; otherwise evaluate it here.
+            constexpr bool fused = MODEL == MODEL_SH && (ALGO == ALGO_TRANSPORT || ALGO == ALGO_DIFFUSION);
+            if constexpr (!fused) {
                 if (in_grid) {
                     float az, col;
                     plain::cart2sph(o, az, col);
@@ -665,24 +671,37 @@ hipError_t launch_evaluate(int model, int K, const float *params, const float *o
     return hipGetLastError();
 }
 
+template <int MODEL>
+hipError_t launch_update_model(const AlgoParams &a, uint32_t rng_seed, const float *params, const float *orientations,
+                               int64_t n, float *out, hipStream_t stream)
+{
+    const unsigned blocks = blocks_for(n, 128);
+    switch (a.algorithm) {
+    case ALGO_DIFFUSION:
+        update_kernel<MODEL, ALGO_DIFFUSION><<<blocks, 128, 0, stream>>>(a, rng_seed, params, orientations, n, out);
+        break;
+    case ALGO_PROBABILISTIC:
+        update_kernel<MODEL, ALGO_PROBABILISTIC><<<blocks, 128, 0, stream>>>(a, rng_seed, params, orientations, n,
+                                                                            out);
+        break;
+    case ALGO_DETERMINISTIC:
+        update_kernel<MODEL, ALGO_DETERMINISTIC><<<blocks, 128, 0, stream>>>(a, rng_seed, params, orientations, n,
+                                                                            out);
+        break;
+    default:
+        update_kernel<MODEL, ALGO_TRANSPORT><<<blocks, 128, 0, stream>>>(a, rng_seed, params, orientations, n, out);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_update(const AlgoParams &a, uint32_t rng_seed, const float *params, const float *orientations,
                          int64_t n, float *out, hipStream_t stream)
 {
     if (n == 0)
         return hipSuccess;
-    const unsigned blocks = blocks_for(n, 128);
-    if (a.model == MODEL_DTI) {
-        if (a.algorithm == ALGO_DIFFUSION)
-            update_kernel<MODEL_DTI, ALGO_DIFFUSION><<<blocks, 128, 0, stream>>>(a, rng_seed, params, orientations, n, out);
-        else
-            update_kernel<MODEL_DTI, ALGO_TRANSPORT><<<blocks, 128, 0, stream>>>(a, rng_seed, params, orientations, n, out);
-    } else {
-        if (a.algorithm == ALGO_DIFFUSION)
-            update_kernel<MODEL_SH, ALGO_DIFFUSION><<<blocks, 128, 0, stream>>>(a, rng_seed, params, orientations, n, out);
-        else
-            update_kernel<MODEL_SH, ALGO_TRANSPORT><<<blocks, 128, 0, stream>>>(a, rng_seed, params, orientations, n, out);
-    }
-    return hipGetLastError();
+    if (a.model == MODEL_DTI)
+        return launch_update_model<MODEL_DTI>(a, rng_seed, params, orientations, n, out, stream);
+    return launch_update_model<MODEL_SH>(a, rng_seed, params, orientations, n, out, stream);
 }
 
 hipError_t launch_field_evaluate(const FieldView &f, const float *points, int64_t n, float *out, hipStream_t stream)
@@ -755,15 +774,24 @@ hipError_t launch_connectome_kt(const ConnectomeLaunch &c, hipStream_t stream)
     return launch_persistent(connectome_kernel<MODEL, ALGO, 0>, c, c.queue, c.n_seeds, stream);
 }
 
+template <int MODEL>
+hipError_t launch_connectome_model(const ConnectomeLaunch &c, hipStream_t stream)
+{
+    switch (c.algo.algorithm) {
+    case ALGO_DIFFUSION: return launch_connectome_kt<MODEL, ALGO_DIFFUSION>(c, stream);
+    case ALGO_PROBABILISTIC: return launch_connectome_kt<MODEL, ALGO_PROBABILISTIC>(c, stream);
+    case ALGO_DETERMINISTIC: return launch_connectome_kt<MODEL, ALGO_DETERMINISTIC>(c, stream);
+    default: return launch_connectome_kt<MODEL, ALGO_TRANSPORT>(c, stream);
+    }
+}
+
 hipError_t launch_connectome(const ConnectomeLaunch &c, hipStream_t stream)
 {
     if (c.n_seeds == 0)
         return hipSuccess;
     if (c.algo.model == MODEL_DTI)
-        return c.algo.algorithm == ALGO_DIFFUSION ? launch_connectome_kt<MODEL_DTI, ALGO_DIFFUSION>(c, stream)
-                                                  : launch_connectome_kt<MODEL_DTI, ALGO_TRANSPORT>(c, stream);
-    return c.algo.algorithm == ALGO_DIFFUSION ? launch_connectome_kt<MODEL_SH, ALGO_DIFFUSION>(c, stream)
-                                              : launch_connectome_kt<MODEL_SH, ALGO_TRANSPORT>(c, stream);
+        return launch_connectome_model<MODEL_DTI>(c, stream);
+    return launch_connectome_model<MODEL_SH>(c, stream);
 }
 
 template <int MODEL, int ALGO>
@@ -774,15 +802,24 @@ hipError_t launch_tod_kt(const TodLaunch &t, hipStream_t stream)
     return launch_persistent(tod_kernel<MODEL, ALGO, 0>, t, t.queue, t.n_seeds, stream);
 }
 
+template <int MODEL>
+hipError_t launch_tod_model(const TodLaunch &t, hipStream_t stream)
+{
+    switch (t.algo.algorithm) {
+    case ALGO_DIFFUSION: return launch_tod_kt<MODEL, ALGO_DIFFUSION>(t, stream);
+    case ALGO_PROBABILISTIC: return launch_tod_kt<MODEL, ALGO_PROBABILISTIC>(t, stream);
+    case ALGO_DETERMINISTIC: return launch_tod_kt<MODEL, ALGO_DETERMINISTIC>(t, stream);
+    default: return launch_tod_kt<MODEL, ALGO_TRANSPORT>(t, stream);
+    }
+}
+
 hipError_t launch_tod(const TodLaunch &t, hipStream_t stream)
 {
     if (t.n_seeds == 0 || t.n_steps == 0)
         return hipSuccess;
     if (t.algo.model == MODEL_DTI)
-        return t.algo.algorithm == ALGO_DIFFUSION ? launch_tod_kt<MODEL_DTI, ALGO_DIFFUSION>(t, stream)
-                                                  : launch_tod_kt<MODEL_DTI, ALGO_TRANSPORT>(t, stream);
-    return t.algo.algorithm == ALGO_DIFFUSION ? launch_tod_kt<MODEL_SH, ALGO_DIFFUSION>(t, stream)
-                                              : launch_tod_kt<MODEL_SH, ALGO_TRANSPORT>(t, stream);
+        return launch_tod_model<MODEL_DTI>(t, stream);
+    return launch_tod_model<MODEL_SH>(t, stream);
 }
 
 hipError_t launch_zero(float *out, int64_t n, hipStream_t stream)
