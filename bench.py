@@ -416,6 +416,11 @@ def make_algorithm(tg, cfg, seed: int = 0):
         model = tg.local.DiffusionTensorImaging()
     else:
         model = tg.local.RealSymmetricSphericalHarmonics(cfg["fod"].shape[-1])
+    if cfg["algorithm"] == "probabilistic":
+        # The reference's defaults (algorithm.py:288-295) at the workload's step.
+        return tg.algorithm.Probabilistic(model, seed=seed, step_size=cfg["step_size"])
+    if cfg["algorithm"] == "deterministic":
+        return tg.algorithm.Deterministic(model, step_size=cfg["step_size"])
     if cfg["algorithm"] == "transport":
         return tg.algorithm.Transport(model, step_size=cfg["step_size"], gamma=cfg["gamma"])
     return tg.algorithm.Diffusion(model, step_size=cfg["step_size"], gamma=cfg["gamma"],
@@ -816,6 +821,9 @@ def main():
                              "(reference dispatch for hardware byte counters)")
     parser.add_argument("--no-traffic", action="store_true",
                         help="skip the two hardware-counter runs that measure the tracking kernel's HBM bytes")
+    parser.add_argument("--algorithm", default=None, choices=["transport", "diffusion", "probabilistic", "deterministic"],
+                        help="replace the workload's algorithm (Probabilistic and Deterministic: plain path only, "
+                             "the reference's default parameters)")
     parser.add_argument("--product", default="tractogram", choices=["tractogram", "tod", "connectome"],
                         help="tod / connectome: time that product over boundary seeds of the workload's mask")
     args = parser.parse_args()
@@ -840,6 +848,9 @@ def main():
     cfg = generator(n_seeds=args.seeds or default_seeds)
     n_seeds = cfg["n_seeds"] if cfg.get("seeds") is None else len(cfg["seeds"])
     workload = f"{args.workload}: {description.format(n=n_seeds)}"
+    if args.algorithm is not None and args.algorithm != cfg["algorithm"]:
+        cfg["algorithm"] = args.algorithm
+        workload += f"; algorithm replaced by {args.algorithm}"
     print(f"[bench] workload generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     if args.product != "tractogram":
         if cfg.get("mask") is None:

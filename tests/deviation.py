@@ -14,9 +14,12 @@ from __future__ import annotations
 import numpy as np
 
 
-def report(lengths, steps, points, ref64) -> dict:
+def report(lengths, steps, points, ref64, jump: float | None = None) -> dict:
     """lengths/points: test path (ragged points in seed order); ref64: FP64
-    oracle result (dense points, margins)."""
+    oracle result (dense points, margins). With `jump`, a prefix also ends
+    before the first point that deviates by more than `jump`: for sampling
+    algorithms, where the two arithmetics may choose different candidates
+    when a draw falls within rounding of a decision boundary."""
     lengths = np.asarray(lengths, dtype=np.int64)
     offsets = np.concatenate([[0], np.cumsum(lengths)])
     max_dev = 0.0
@@ -32,6 +35,8 @@ def report(lengths, steps, points, ref64) -> dict:
         dev = np.linalg.norm(got - ref, axis=1)
         running = np.maximum.accumulate(dev)
         ok = running < ref64["margins"][i, :n]
+        if jump is not None:
+            ok &= np.maximum.accumulate(dev > jump) == 0
         k = n if np.all(ok) else int(np.argmin(ok))
         if k > 0:
             max_dev = max(max_dev, float(running[k - 1]))

@@ -564,9 +564,6 @@ int check_track_args(const tgk_domain *domain, const tgk_field *field, const tgk
         return fail(TGK_ERR_INVALID, "unknown path %d", path);
     if (int r = check_algorithm(algorithm, field, path == TGK_PATH_FAST ? 91 : 45))
         return r;
-    if (path == TGK_PATH_FAST && algorithm->algorithm >= TGK_ALGO_PROBABILISTIC)
-        return fail(TGK_ERR_UNSUPPORTED, "the fast path runs Transport and Diffusion; Probabilistic and "
-                                         "Deterministic run on the plain path");
     if (n_seeds < 0)
         return fail(TGK_ERR_INVALID, "negative seed count");
     if (n_seeds > MAX_SEEDS)

@@ -165,13 +165,19 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
                 q[4 * j + 3] = v.w;
         }
     }
-    const fast::Direction dir = fast::direction_of(L.o);
-    fast::ModelOut mv;
-    if constexpr (MODEL == MODEL_SH)
-        mv = fast::sh_model<LMAX>(q, dir);
-    else
-        mv = fast::dti_model(q, L.o, dir);
-    L.o = fast::update<ALGO>(p.algo, mv, dir, L.o, L.rng);
+    if constexpr (ALGO == ALGO_PROBABILISTIC) {
+        L.o = fast::probabilistic<MODEL, LMAX>(p.algo, q, L.o, L.rng);
+    } else if constexpr (ALGO == ALGO_DETERMINISTIC) {
+        L.o = fast::deterministic<MODEL, LMAX>(p.algo, q, L.o);
+    } else {
+        const fast::Direction dir = fast::direction_of(L.o);
+        fast::ModelOut mv;
+        if constexpr (MODEL == MODEL_SH)
+            mv = fast::sh_model<LMAX>(q, dir);
+        else
+            mv = fast::dti_model(q, L.o, dir);
+        L.o = fast::update<ALGO>(p.algo, mv, dir, L.o, L.rng);
+    }
     L.steps++;
     L.previous = L.point;
     const float step = p.algo.step_size;
@@ -222,7 +228,7 @@ __global__ __launch_bounds__(fast_block<LMAX>()) void track_fast_kernel(FastLaun
                     L.point = v3(s0.x, s0.y, s0.z);
                     L.o = v3(s1.x, s1.y, s1.z);
                     L.previous = L.point;
-                    if constexpr (ALGO == ALGO_DIFFUSION)
+                    if constexpr (ALGO == ALGO_DIFFUSION || ALGO == ALGO_PROBABILISTIC)
                         L.rng = seed_stream(p, p.first_seed + L.g);
                     L.time = 0.0;
                     L.n = 1;
@@ -427,9 +433,12 @@ hipError_t launch_fast_domain(const FastLaunch &f, int grid, bool query, hipStre
 template <int MODEL>
 hipError_t launch_fast_algo(const FastLaunch &f, int grid, bool query, hipStream_t stream)
 {
-    if (f.track.algo.algorithm == ALGO_DIFFUSION)
-        return launch_fast_domain<MODEL, ALGO_DIFFUSION>(f, grid, query, stream);
-    return launch_fast_domain<MODEL, ALGO_TRANSPORT>(f, grid, query, stream);
+    switch (f.track.algo.algorithm) {
+    case ALGO_DIFFUSION: return launch_fast_domain<MODEL, ALGO_DIFFUSION>(f, grid, query, stream);
+    case ALGO_PROBABILISTIC: return launch_fast_domain<MODEL, ALGO_PROBABILISTIC>(f, grid, query, stream);
+    case ALGO_DETERMINISTIC: return launch_fast_domain<MODEL, ALGO_DETERMINISTIC>(f, grid, query, stream);
+    default: return launch_fast_domain<MODEL, ALGO_TRANSPORT>(f, grid, query, stream);
+    }
 }
 
 hipError_t dispatch_fast(const FastLaunch &f, int grid, bool query, hipStream_t stream)

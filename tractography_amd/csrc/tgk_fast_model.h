@@ -256,5 +256,155 @@ __device__ __forceinline__ V3 update(const AlgoParams &a, const ModelOut &mv, co
     return exps2(o, v3(tx, ty, tz));
 }
 
+// ------------------------------------------ Probabilistic, Deterministic ---
+
+// Model value, d/dtheta and (d/dphi)/sin(theta) at a (not necessarily unit)
+// direction, from a cell's coefficients q (SH polynomial form or DTI).
+template <int MODEL, int LMAX>
+__device__ __forceinline__ ModelOut model_at(const float *q, V3 c, const Direction &dir)
+{
+    if constexpr (MODEL == MODEL_SH)
+        return sh_model<LMAX>(q, dir);
+    else
+        return dti_model(q, c, dir);
+}
+
+// The value alone (the derivative terms are dead code here).
+template <int MODEL, int LMAX>
+__device__ __forceinline__ float value_at(const float *q, V3 c)
+{
+    return model_at<MODEL, LMAX>(q, c, direction_of(c)).value;
+}
+
+__device__ __forceinline__ V3 unit_or_z(V3 o)
+{
+    const float n2 = o.x * o.x + o.y * o.y + o.z * o.z;
+    if (!(__builtin_sqrtf(n2) > 1e-8f))
+        return v3(0.0f, 0.0f, 1.0f);
+    const float inv = __builtin_amdgcn_rsqf(n2);
+    return v3(o.x * inv, o.y * inv, o.z * inv);
+}
+
+// algorithm.py:305-376 (Probabilistic), the plain path's two passes with the
+// replayed noise state, on the polynomial model: candidates are Cartesian
+// already, so a candidate costs one polynomial and one softplus (hardware
+// exp2/log2), with no cart2sph and no trigonometric basis. The azimuth
+// 2 pi u is handed to the hardware sine and cosine as u revolutions.
+template <int MODEL, int LMAX>
+__device__ __forceinline__ V3 probabilistic(const AlgoParams &a, const float *q, V3 o_in, uint2 &rng)
+{
+    const V3 o = unit_or_z(o_in);
+    const bool z_helper = fabsf(o.z) < 0.9f;
+    // cross(helper, o) for helper e_z or e_y, then normalised.
+    const V3 c = z_helper ? v3(-o.y, o.x, 0.0f) : v3(o.z, 0.0f, -o.x);
+    const float ic = __builtin_amdgcn_rsqf(c.x * c.x + c.y * c.y + c.z * c.z);
+    const V3 t1 = v3(c.x * ic, c.y * ic, c.z * ic);
+    const V3 t2 = v3(o.y * t1.z - o.z * t1.y, o.z * t1.x - o.x * t1.z, o.x * t1.y - o.y * t1.x);
+    const float one_minus_cos_max = 1.0f - tgk_cosf(a.max_angle);
+    auto candidate = [&](uint2 &state) {
+        const float cos_angle = 1.0f - randu(state) * one_minus_cos_max;
+        const float sin_angle = __builtin_sqrtf(fmaxf(0.0f, 1.0f - cos_angle * cos_angle));
+        const float turns = randu(state);
+        const float ca = __builtin_amdgcn_cosf(turns), sa = __builtin_amdgcn_sinf(turns);
+        return v3(cos_angle * o.x + sin_angle * (ca * t1.x + sa * t2.x),
+                  cos_angle * o.y + sin_angle * (ca * t1.y + sa * t2.y),
+                  cos_angle * o.z + sin_angle * (ca * t1.z + sa * t2.z));
+    };
+    const uint2 initial = rng;
+    float total = 0.0f;
+    for (int i = 0; i < a.n_samples; i++) {
+        const float v = value_at<MODEL, LMAX>(q, candidate(rng));
+        total += __builtin_isfinite(v) ? fmaxf(v, 0.0f) : 0.0f;
+    }
+    if (!(total > 0.0f))
+        return o;
+    const float target = randu(rng) * total;
+    rng = initial;
+    float cumulative = 0.0f;
+    for (int i = 0; i < a.n_samples; i++) {
+        const V3 cand = candidate(rng);
+        const float v = value_at<MODEL, LMAX>(q, cand);
+        cumulative += __builtin_isfinite(v) ? fmaxf(v, 0.0f) : 0.0f;
+        if (cumulative > target)
+            return cand;
+    }
+    return o;
+}
+
+// Gradient on the sphere: dtheta e_theta + (dphi / sin theta) e_phi.
+template <int MODEL, int LMAX>
+__device__ __forceinline__ V3 sphere_gradient(const float *q, V3 p)
+{
+    const Direction d = direction_of(p);
+    const ModelOut m = model_at<MODEL, LMAX>(q, p, d);
+    return v3(m.dtheta * (d.z * d.c1) - m.dphi * d.s1, m.dtheta * (d.z * d.s1) + m.dphi * d.c1, -m.dtheta * d.s);
+}
+
+// algorithm.py:515-598 (Deterministic): the plain path's Newton ascent with
+// the polynomial model, hardware rsqrt frames and the Taylor exps2.
+template <int MODEL, int LMAX>
+__device__ __forceinline__ V3 deterministic(const AlgoParams &a, const float *q, V3 o_in)
+{
+    const float h = 1e-2f, inv_2h = 50.0f;
+    const float n2 = o_in.x * o_in.x + o_in.y * o_in.y + o_in.z * o_in.z;
+    const float inv = __builtin_amdgcn_rsqf(n2);
+    V3 o = v3(o_in.x * inv, o_in.y * inv, o_in.z * inv);
+    for (int iteration = 0; iteration < a.max_iterations; iteration++) {
+        const Direction d = direction_of(o);
+        const V3 e1 = v3(d.z * d.c1, d.z * d.s1, -d.s);
+        const V3 e2 = v3(-d.s1, d.c1, 0.0f);
+        const ModelOut current = model_at<MODEL, LMAX>(q, o, d);
+        const float f0 = current.value, g1 = current.dtheta, g2 = current.dphi;
+        const float gradient_norm = __builtin_sqrtf(g1 * g1 + g2 * g2);
+        if (gradient_norm <= a.tolerance)
+            break;
+        const V3 s1 = v3(h * e1.x, h * e1.y, h * e1.z), s2 = v3(h * e2.x, h * e2.y, 0.0f);
+        const V3 gp1 = sphere_gradient<MODEL, LMAX>(q, exps2(o, s1));
+        const V3 gm1 = sphere_gradient<MODEL, LMAX>(q, exps2(o, v3(-s1.x, -s1.y, -s1.z)));
+        const V3 gp2 = sphere_gradient<MODEL, LMAX>(q, exps2(o, s2));
+        const V3 gm2 = sphere_gradient<MODEL, LMAX>(q, exps2(o, v3(-s2.x, -s2.y, 0.0f)));
+        const V3 dg1 = v3(gp1.x - gm1.x, gp1.y - gm1.y, gp1.z - gm1.z);
+        const V3 dg2 = v3(gp2.x - gm2.x, gp2.y - gm2.y, gp2.z - gm2.z);
+        const float h11 = (dg1.x * e1.x + dg1.y * e1.y + dg1.z * e1.z) * inv_2h;
+        const float h22 = (dg2.x * e2.x + dg2.y * e2.y) * inv_2h;
+        const float h12 = 0.5f * ((dg2.x * e1.x + dg2.y * e1.y + dg2.z * e1.z) + (dg1.x * e2.x + dg1.y * e2.y)) *
+                          inv_2h;
+        const float determinant = h11 * h22 - h12 * h12;
+        float d1, d2;
+        if (h11 < -1e-7f && h22 < -1e-7f && determinant > 1e-12f) {
+            const float r = __builtin_amdgcn_rcpf(determinant);
+            d1 = (-h22 * g1 + h12 * g2) * r;
+            d2 = (h12 * g1 - h11 * g2) * r;
+        } else {
+            const float r = 0.1f * __builtin_amdgcn_rcpf(gradient_norm);
+            d1 = g1 * r;
+            d2 = g2 * r;
+        }
+        const float step_norm = __builtin_sqrtf(d1 * d1 + d2 * d2);
+        if (step_norm > a.max_step_size) {
+            const float scale = a.max_step_size / step_norm;
+            d1 *= scale;
+            d2 *= scale;
+        }
+        if (__builtin_sqrtf(d1 * d1 + d2 * d2) <= 1e-7f)
+            break;
+        const V3 tangent = v3(d1 * e1.x + d2 * e2.x, d1 * e1.y + d2 * e2.y, d1 * e1.z);
+        bool accepted = false;
+        float scale = 1.0f;
+        for (int search = 0; search < 8; search++) {
+            const V3 c = exps2(o, v3(scale * tangent.x, scale * tangent.y, scale * tangent.z));
+            if (value_at<MODEL, LMAX>(q, c) > f0 + 1e-7f) {
+                accepted = true;
+                o = c;
+                break;
+            }
+            scale *= 0.5f;
+        }
+        if (!accepted)
+            break;
+    }
+    return o;
+}
+
 }  // namespace fast
 }  // namespace tgk
