@@ -139,6 +139,74 @@ __device__ __forceinline__ uint2 rng_init(uint32_t seed, uint32_t stream_id)
 
 __device__ __forceinline__ float randu(uint2 &s) { return (float)(mwc64x(s) >> 8) * 0x1p-24f; }
 
+// Weighted choice among n candidates (algorithm.py:345-375): the candidates
+// come from the noise stream (`candidate(state)` draws and returns one),
+// weights are `weight(c)` (non-negative), target = randu * total, and the
+// choice is the first candidate whose running weight exceeds the target. The
+// reference evaluates every candidate twice (a summing pass, then a replay
+// of the stream from its start). Here the summing pass also keeps, for 8
+// equal blocks of candidates, the running sum and the noise state at the
+// block's start; the second pass replays only the block where the sum
+// crosses the target, from its saved state and sum. Every sum is the same
+// float addition in the same order as the reference's, so the choice, the
+// returned candidate and the final noise state (after the chosen
+// candidate's draws) are the same bits, for about n + n/16 evaluations
+// instead of up to 2n. Returns false, with the state after the n
+// candidates' draws, when the total is not positive or no running sum
+// exceeds the target.
+template <class Candidate, class Weight>
+__device__ __forceinline__ bool choose_candidate(int n, uint2 &rng, Candidate &&candidate, Weight &&weight, V3 &chosen)
+{
+    constexpr int BLOCKS = 8;
+    const int per_block = (n + BLOCKS - 1) / BLOCKS;
+    float start_sum[BLOCKS];
+    uint2 start_state[BLOCKS];
+    uint2 state = rng;
+    float total = 0.0f;
+#pragma unroll
+    for (int b = 0; b < BLOCKS; b++) {
+        start_sum[b] = total;
+        start_state[b] = state;
+        const int end = min(n, (b + 1) * per_block);
+        for (int i = b * per_block; i < end; i++)
+            total += weight(candidate(state));
+    }
+    rng = state;
+    if (!(total > 0.0f))
+        return false;
+    uint2 target_state = state;
+    const float target = randu(target_state) * total;
+    // The first block whose running sum at its end exceeds the target.
+    int block = BLOCKS;
+#pragma unroll
+    for (int b = BLOCKS - 1; b >= 0; b--) {
+        const float end_sum = b + 1 < BLOCKS ? start_sum[b + 1] : total;
+        if (end_sum > target)
+            block = b;
+    }
+    if (block == BLOCKS)
+        return false;
+    float cumulative = 0.0f;
+    uint2 replay = state;
+#pragma unroll
+    for (int b = 0; b < BLOCKS; b++)
+        if (b == block) {
+            cumulative = start_sum[b];
+            replay = start_state[b];
+        }
+    const int end = min(n, (block + 1) * per_block);
+    for (int i = block * per_block; i < end; i++) {
+        const V3 c = candidate(replay);
+        cumulative += weight(c);
+        if (cumulative > target) {
+            rng = replay;
+            chosen = c;
+            return true;
+        }
+    }
+    return false;  // unreachable: the block's end sum exceeds the target
+}
+
 __device__ __forceinline__ uint32_t randi(uint2 &s, uint32_t max) { return (uint32_t)(randu(s) * (float)max); }
 
 // The reference seeds stream (batch k, index within batch) from the per-batch

@@ -285,8 +285,8 @@ __device__ __forceinline__ V3 unit_or_z(V3 o)
     return v3(o.x * inv, o.y * inv, o.z * inv);
 }
 
-// algorithm.py:305-376 (Probabilistic), the plain path's two passes with the
-// replayed noise state, on the polynomial model: candidates are Cartesian
+// algorithm.py:305-376 (Probabilistic), the plain path's weighted choice
+// (choose_candidate) on the polynomial model: candidates are Cartesian
 // already, so a candidate costs one polynomial and one softplus (hardware
 // exp2/log2), with no cart2sph and no trigonometric basis. The azimuth
 // 2 pi u is handed to the hardware sine and cosine as u revolutions.
@@ -310,25 +310,12 @@ __device__ __forceinline__ V3 probabilistic(const AlgoParams &a, const float *q,
                   cos_angle * o.y + sin_angle * (ca * t1.y + sa * t2.y),
                   cos_angle * o.z + sin_angle * (ca * t1.z + sa * t2.z));
     };
-    const uint2 initial = rng;
-    float total = 0.0f;
-    for (int i = 0; i < a.n_samples; i++) {
-        const float v = value_at<MODEL, LMAX>(q, candidate(rng));
-        total += __builtin_isfinite(v) ? fmaxf(v, 0.0f) : 0.0f;
-    }
-    if (!(total > 0.0f))
-        return o;
-    const float target = randu(rng) * total;
-    rng = initial;
-    float cumulative = 0.0f;
-    for (int i = 0; i < a.n_samples; i++) {
-        const V3 cand = candidate(rng);
-        const float v = value_at<MODEL, LMAX>(q, cand);
-        cumulative += __builtin_isfinite(v) ? fmaxf(v, 0.0f) : 0.0f;
-        if (cumulative > target)
-            return cand;
-    }
-    return o;
+    auto weight = [&](V3 c) {
+        const float v = value_at<MODEL, LMAX>(q, c);
+        return __builtin_isfinite(v) ? fmaxf(v, 0.0f) : 0.0f;
+    };
+    V3 chosen;
+    return choose_candidate(a.n_samples, rng, candidate, weight, chosen) ? chosen : o;
 }
 
 // Gradient on the sphere: dtheta e_theta + (dphi / sin theta) e_phi.

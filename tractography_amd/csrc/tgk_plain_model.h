@@ -390,12 +390,11 @@ struct VoxelModel {
 
 // algorithm.py:305-376 (Probabilistic): n_samples candidates uniform on the
 // spherical cap of radius max_angle around the normalised orientation, one
-// chosen with probability proportional to its non-negative model value. The
-// first pass sums the weights; the noise state is then replayed so that the
-// second pass regenerates the same candidates and returns the first whose
-// running weight exceeds randu * total. The state after the update is the
-// one left where the second pass returned (2 draws per candidate visited),
-// or after the first pass and the target draw when no candidate was chosen.
+// chosen with probability proportional to its non-negative model value
+// (choose_candidate: the reference's two passes, the second replaying only
+// the block of candidates where the running weight crosses the target). The
+// state after the update is the one left after the chosen candidate's two
+// draws, or after all candidates' draws when none is chosen.
 template <int MODEL, int KT>
 __device__ __forceinline__ V3 probabilistic_update(const AlgoParams &a, const float *__restrict__ row, V3 o_in,
                                                    uint2 &rng)
@@ -418,25 +417,12 @@ __device__ __forceinline__ V3 probabilistic_update(const AlgoParams &a, const fl
                   cos_angle * o.y + sin_angle * (ca * t1.y + sa * t2.y),
                   cos_angle * o.z + sin_angle * (ca * t1.z + sa * t2.z));
     };
-    const uint2 initial = rng;
-    float total = 0.0f;
-    for (int i = 0; i < a.n_samples; i++) {
-        const float v = vox.value(candidate(rng));
-        total += __builtin_isfinite(v) ? fmaxf(v, 0.0f) : 0.0f;
-    }
-    if (total <= 0.0f)
-        return o;
-    const float target = randu(rng) * total;
-    rng = initial;
-    float cumulative = 0.0f;
-    for (int i = 0; i < a.n_samples; i++) {
-        const V3 c = candidate(rng);
+    auto weight = [&](V3 c) {
         const float v = vox.value(c);
-        cumulative += __builtin_isfinite(v) ? fmaxf(v, 0.0f) : 0.0f;
-        if (cumulative > target)
-            return c;
-    }
-    return o;
+        return __builtin_isfinite(v) ? fmaxf(v, 0.0f) : 0.0f;
+    };
+    V3 chosen;
+    return choose_candidate(a.n_samples, rng, candidate, weight, chosen) ? chosen : o;
 }
 
 // Gradient of the model on the sphere at o (algorithm.py:500-513): dphi is
