@@ -13,7 +13,7 @@ import numpy as np
 import numpy.typing as npt
 
 from . import algorithm as algorithm_module
-from . import autotune, core, local, seeds, tracking, utils
+from . import autotune, core, local, seeds, streaming, tracking, utils
 from . import domain as domain_module
 from . import field as field_module
 from .algorithm import BaseAlgorithm
@@ -25,6 +25,21 @@ domain = domain_module
 field = field_module
 seeds_module = seeds
 OpenCLRuntime = HipRuntime
+
+
+def _tracking_components(fod, tracking_algorithm: BaseAlgorithm, domain_image=None):
+    """Algorithm, image domain and image field for a 4-D image
+    (__init__.py:19-37); the domain defaults to the image's nonzero voxels."""
+    _validate_algorithm(tracking_algorithm)
+    if tracking_algorithm.local_model.n_parameters != fod.shape[-1]:
+        raise ValueError(
+            "Algorithm local model parameter count does not match the FOD: "
+            f"{tracking_algorithm.local_model.n_parameters} != {fod.shape[-1]}"
+        )
+    if domain_image is None:
+        mask = np.any(np.asanyarray(fod.dataobj) != 0, axis=-1).astype(np.uint8)
+        domain_image = Nifti1Image(mask, fod.affine)
+    return tracking_algorithm, domain_module.Image(domain_image), field_module.Image(fod)
 
 
 def _validate_algorithm(tracking_algorithm: BaseAlgorithm) -> None:
@@ -378,6 +393,7 @@ __all__ = [
     "field",
     "local",
     "seeds",
+    "streaming",
     "tod",
     "tracking",
     "tractogram",
