@@ -580,7 +580,7 @@ __global__ __launch_bounds__(256) void tod_kernel(TodLaunch t)
                 if (in_grid) {
                     float az, col;
                     plain::cart2sph(o, az, col);
-                    plain::sh45_terms(az, col, [&](int i, float y, float, float) { add(i, y); });
+                    plain::sh45_values(az, col, add);
                 }
             }
             o = plain::update<MODEL, ALGO, KT>(t.algo, t.field.data + voxel * stride, o, rng, add);

@@ -7,6 +7,8 @@
 // compiled with -ffp-contract=off.
 #pragma once
 
+#include <type_traits>
+
 #include "tgk_device.h"
 
 namespace tgk {
@@ -57,162 +59,26 @@ __device__ __forceinline__ float randn(uint2 &s)
     return tgk_sqrtf(-2.0f * tgk_logf(n)) * tgk_cosf((2.0f * 3.14159265359f) * randu(s));
 }
 
-// The l <= 8 real SH basis (45 terms) and its two angular derivatives
-// (spharm.cl:216-413; the values alone are spharm.cl:10-207, the same
-// expressions), handed to visit(i, Y_i, dY_i/dphi / sin(theta), dY_i/dtheta)
-// in index order. The basis is generated band by band: for each (l, m > 0)
-// the reference's prefactor of the value (A), of the azimuth derivative over
-// sin(theta) (D) and of the colatitude derivative (T) is formed once and
-// multiplied by sin(m phi) for index l(l+1)/2 - m and by cos(m phi) for index
-// l(l+1)/2 + m; the D term of the cos partner is -(D sin(m phi)), equal bit
-// for bit to the reference's negated-constant form. A visitor that ignores
-// the derivatives lets the compiler drop them.
-template <class Visit>
-__device__ __forceinline__ void sh45_terms(float phi, float theta, Visit &&visit)
+// SH model contraction (local.py:91-120): value, d/dtheta and
+// (d/dphi)/sin(theta) of sum_i c_i Y_i. Terms are accumulated in index order,
+// so sums round exactly as the reference's `for (i < K)` loop; coefficients
+// past K arrive as +0 and leave the sums unchanged.
+__device__ __forceinline__ void sh45_contract(const float c[45], float phi, float theta, float &v, float &th,
+                                              float &ph)
 {
-    float st, ct, sp, cp, s2p, c2p;
-    tgk_sincosf(theta, &st, &ct);
-    tgk_sincosf(phi, &sp, &cp);
-    tgk_sincosf(2.0f * phi, &s2p, &c2p);
-    const float s3p = 2.0f * cp * s2p - sp, c3p = 2.0f * cp * c2p - cp;
-    const float s4p = 2.0f * cp * s3p - s2p, c4p = 2.0f * cp * c3p - c2p;
-    const float s5p = 2.0f * cp * s4p - s3p, c5p = 2.0f * cp * c4p - c3p;
-    const float s6p = 2.0f * cp * s5p - s4p, c6p = 2.0f * cp * c5p - c4p;
-    const float s7p = 2.0f * cp * s6p - s5p, c7p = 2.0f * cp * c6p - c5p;
-    const float s8p = 2.0f * cp * s7p - s6p, c8p = 2.0f * cp * c7p - c6p;
-
-    const float st2 = st * st, st3 = st2 * st, st4 = st3 * st, st5 = st4 * st;
-    const float st6 = st5 * st, st7 = st6 * st, st8 = st7 * st;
-    const float ct2 = ct * ct, ct3 = ct2 * ct, ct4 = ct3 * ct, ct5 = ct4 * ct;
-    const float ct6 = ct5 * ct, ct7 = ct6 * ct, ct8 = ct7 * ct;
-    const float stct = st * ct;
-
-#define TGK_ACC(i, Y, DP, DT) visit(i, (Y), (DP), (DT));
-#define TGK_PAIR(ineg, ipos, A, D, T, S, C) \
-    TGK_ACC(ineg, (A) * (S), (D) * (C), (T) * (S))
-#define TGK_PAIR_POS(ipos, A, D, T, S, C) TGK_ACC(ipos, (A) * (C), -((D) * (S)), (T) * (C))
-
-    // l = 0
-    TGK_ACC(0, 0.28209479177387814f, 0.0f, 0.0f);
-
-    // l = 2 (indices 1..5)
-    {
-        const float A2 = 0.54627421529f * st2, D2 = 0.54627421529f * st * 2.0f, T2 = 0.54627421529f * 2.0f * stct;
-        const float A1 = -1.0925484305920792f * stct, D1 = -1.0925484305920792f * ct;
-        const float T1 = -1.0925484305920792f * (ct2 - st2);
-        TGK_PAIR(1, 5, A2, D2, T2, s2p, c2p);
-        TGK_PAIR(2, 4, A1, D1, T1, sp, cp);
-        TGK_ACC(3, 0.31539156525252005f * (3.0f * ct2 - 1.0f), 0.0f, -0.31539156525252005f * 6.0f * ct * st);
-        TGK_PAIR_POS(4, A1, D1, T1, sp, cp);
-        TGK_PAIR_POS(5, A2, D2, T2, s2p, c2p);
+    float sv = 0.0f, sth = 0.0f, sph = 0.0f;
+#define TGK_ACC(i, Y, DP, DT)   \
+    {                           \
+        const float ci_ = c[i]; \
+        sv += ci_ * (Y);        \
+        sth += ci_ * (DT);      \
+        sph += ci_ * (DP);      \
     }
-
-    // l = 4 (indices 6..14)
-    {
-        const float A4 = 0.62583573544f * st4, D4 = 0.62583573544f * st3 * 4.0f;
-        const float T4 = 0.62583573544f * 4.0f * st3 * ct;
-        const float A3 = -1.77013076978f * st3 * ct, D3 = -1.77013076978f * st2 * ct * 3.0f;
-        const float T3 = -1.77013076978f * (3.0f * st2 * ct2 - st4);
-        const float q2 = 7.0f * ct2 - 1.0f;
-        const float A2 = 0.47308734787f * st2 * q2, D2 = 0.47308734787f * st * q2 * 2.0f;
-        const float T2 = 0.47308734787f * (2.0f * stct * q2 - 14.0f * st3 * ct);
-        const float q1 = 7.0f * ct2 - 3.0f;
-        const float A1 = -0.66904654355f * st * q1 * ct, D1 = -0.66904654355f * ct * q1;
-        const float T1 = -0.66904654355f * ((ct2 - st2) * q1 - (14.0f * st2 * ct2));
-        TGK_PAIR(6, 14, A4, D4, T4, s4p, c4p);
-        TGK_PAIR(7, 13, A3, D3, T3, s3p, c3p);
-        TGK_PAIR(8, 12, A2, D2, T2, s2p, c2p);
-        TGK_PAIR(9, 11, A1, D1, T1, sp, cp);
-        TGK_ACC(10, 0.10578554691520431f * (35.0f * ct2 * ct2 - 30.0f * ct2 + 3.0f), 0.0f,
-                0.10578554691520431f * (-35.0f * 4.0f * ct2 * stct + 30.0f * 2.0f * stct));
-        TGK_PAIR_POS(11, A1, D1, T1, sp, cp);
-        TGK_PAIR_POS(12, A2, D2, T2, s2p, c2p);
-        TGK_PAIR_POS(13, A3, D3, T3, s3p, c3p);
-        TGK_PAIR_POS(14, A4, D4, T4, s4p, c4p);
-    }
-
-    // l = 6 (indices 15..27)
-    {
-        const float A6 = 0.68318410519f * st6, D6 = 0.68318410519f * st5 * 6.0f;
-        const float T6 = 0.68318410519f * 6.0f * st5 * ct;
-        const float A5 = -2.36661916223f * st5 * ct, D5 = -2.36661916223f * st4 * ct * 5.0f;
-        const float T5 = -2.36661916223f * (5.0f * st4 * ct2 - st6);
-        const float q4 = 11.0f * ct2 - 1.0f;
-        const float A4 = 0.50456490072f * st4 * q4, D4 = 0.50456490072f * st3 * q4 * 4.0f;
-        const float T4 = 0.50456490072f * (4.0f * st3 * ct * q4 - 22.0f * st5 * ct);
-        const float q3 = 11.0f * ct3 - 3.0f * ct;
-        const float A3 = -0.92120525951f * st3 * q3, D3 = -0.92120525951f * st2 * q3 * 3.0f;
-        const float T3 = -0.92120525951f * (3.0f * st2 * ct * q3 - st4 * (33.0f * ct2 - 3.0f));
-        const float q2 = 33.0f * ct4 - 18.0f * ct2 + 1.0f;
-        const float A2 = 0.46060262975f * st2 * q2, D2 = 0.46060262975f * st * q2 * 2.0f;
-        const float T2 = 0.46060262975f * (2.0f * st * ct * q2 - st3 * (33.0f * 4.0f * ct3 - 36.0f * ct));
-        const float q1 = 33.0f * ct5 - 30.0f * ct3 + 5.0f * ct;
-        const float A1 = -0.58262136251f * st * q1, D1 = -0.58262136251f * q1;
-        const float T1 = -0.58262136251f * (ct * q1 - st2 * (33.0f * 5.0f * ct4 - 90.0f * ct2 + 5.0f));
-        TGK_PAIR(15, 27, A6, D6, T6, s6p, c6p);
-        TGK_PAIR(16, 26, A5, D5, T5, s5p, c5p);
-        TGK_PAIR(17, 25, A4, D4, T4, s4p, c4p);
-        TGK_PAIR(18, 24, A3, D3, T3, s3p, c3p);
-        TGK_PAIR(19, 23, A2, D2, T2, s2p, c2p);
-        TGK_PAIR(20, 22, A1, D1, T1, sp, cp);
-        TGK_ACC(21, 0.06356920226f * (231.0f * ct6 - 315.0f * ct4 + 105.0f * ct2 - 5.0f), 0.0f,
-                -0.06356920226f * (1386.0f * ct5 - 1260.0f * ct3 + 210.0f * ct) * st);
-        TGK_PAIR_POS(22, A1, D1, T1, sp, cp);
-        TGK_PAIR_POS(23, A2, D2, T2, s2p, c2p);
-        TGK_PAIR_POS(24, A3, D3, T3, s3p, c3p);
-        TGK_PAIR_POS(25, A4, D4, T4, s4p, c4p);
-        TGK_PAIR_POS(26, A5, D5, T5, s5p, c5p);
-        TGK_PAIR_POS(27, A6, D6, T6, s6p, c6p);
-    }
-
-    // l = 8 (indices 28..44)
-    {
-        const float A8 = 0.72892666017f * st8, D8 = 0.72892666017f * st7 * 8.0f;
-        const float T8 = 0.72892666017f * 8.0f * st7 * ct;
-        const float A7 = -2.9157066407f * st7 * ct, D7 = -2.9157066407f * st6 * ct * 7.0f;
-        const float T7 = -2.9157066407f * (7.0f * st6 * ct2 - st8);
-        const float q6 = 15.0f * ct2 - 1.0f;
-        const float A6 = 0.53233276606f * st6 * q6, D6 = 0.53233276606f * st5 * q6 * 6.0f;
-        const float T6 = 0.53233276606f * (6.0f * st5 * ct * q6 - st7 * 30.0f * ct);
-        const float q5 = 5.0f * ct3 - ct;
-        const float A5 = -3.4499106221f * st5 * q5, D5 = -3.4499106221f * st4 * q5 * 5.0f;
-        const float T5 = -3.4499106221f * (5.0f * st4 * ct * q5 - st6 * q6);
-        const float q4 = 65.0f * ct4 - 26.0f * ct2 + 1.0f;
-        const float A4 = 0.47841652475f * st4 * q4, D4 = 0.47841652475f * st3 * q4 * 4.0f;
-        const float T4 = 0.47841652475f * (4.0f * st3 * ct * q4 - st5 * (260.0f * ct3 - 52.0f * ct));
-        const float q3 = 39.0f * ct5 - 26.0f * ct3 + 3.0f * ct;
-        const float A3 = -1.2352661553f * st3 * q3, D3 = -1.2352661553f * st2 * q3 * 3.0f;
-        const float T3 = -1.2352661553f * (3.0f * st2 * ct * q3 - st4 * (195.0f * ct4 - 78.0f * ct2 + 3.0f));
-        const float q2 = 143.0f * ct6 - 143.0f * ct4 + 33.0f * ct2 - 1.0f;
-        const float A2 = 0.45615225843f * st2 * q2, D2 = 0.45615225843f * st * q2 * 2.0f;
-        const float T2 = 0.45615225843f * (2.0f * st * ct * q2 - st3 * (858.0f * ct5 - 572.0f * ct3 + 66.0f * ct));
-        const float q1 = 715.0f * ct7 - 1001.0f * ct5 + 385.0f * ct3 - 35.0f * ct;
-        const float A1 = -0.10904124589f * st * q1, D1 = -0.10904124589f * q1;
-        const float T1 =
-            -0.10904124589f * (ct * q1 - st2 * (5005.0f * ct6 - 5005.0f * ct4 + 1155.0f * ct2 - 35.0f));
-        TGK_PAIR(28, 44, A8, D8, T8, s8p, c8p);
-        TGK_PAIR(29, 43, A7, D7, T7, s7p, c7p);
-        TGK_PAIR(30, 42, A6, D6, T6, s6p, c6p);
-        TGK_PAIR(31, 41, A5, D5, T5, s5p, c5p);
-        TGK_PAIR(32, 40, A4, D4, T4, s4p, c4p);
-        TGK_PAIR(33, 39, A3, D3, T3, s3p, c3p);
-        TGK_PAIR(34, 38, A2, D2, T2, s2p, c2p);
-        TGK_PAIR(35, 37, A1, D1, T1, sp, cp);
-        TGK_ACC(36, 0.00908677049f * (6435.0f * ct8 - 12012.0f * ct6 + 6930.0f * ct4 - 1260.0f * ct2 + 35.0f), 0.0f,
-                -0.00908677049f *
-                    (8.0f * 6435.0f * ct7 - 6.0f * 12012.0f * ct5 + 4.0f * 6930.0f * ct3 - 2.0f * 1260.0f * ct) * st);
-        TGK_PAIR_POS(37, A1, D1, T1, sp, cp);
-        TGK_PAIR_POS(38, A2, D2, T2, s2p, c2p);
-        TGK_PAIR_POS(39, A3, D3, T3, s3p, c3p);
-        TGK_PAIR_POS(40, A4, D4, T4, s4p, c4p);
-        TGK_PAIR_POS(41, A5, D5, T5, s5p, c5p);
-        TGK_PAIR_POS(42, A6, D6, T6, s6p, c6p);
-        TGK_PAIR_POS(43, A7, D7, T7, s7p, c7p);
-        TGK_PAIR_POS(44, A8, D8, T8, s8p, c8p);
-    }
-#undef TGK_PAIR_POS
-#undef TGK_PAIR
+#include "tgk_sh45_body.inc"
 #undef TGK_ACC
+    v = sv;
+    th = sth;
+    ph = sph;
 }
 
 // No use for the basis values beyond the contraction.
@@ -220,25 +86,36 @@ struct NoTerms {
     __device__ __forceinline__ void operator()(int, float) const {}
 };
 
-// Model value and angular derivatives: terms accumulated in index order, so
-// sums round exactly as the reference's `for (i < K)` loop. Coefficients past
-// K arrive as +0 and leave the sums unchanged. `terms(i, Y_i)` sees every
-// basis value on the way (all 45, whatever K).
-template <class Terms = NoTerms>
-__device__ __forceinline__ void sh45_contract(const float c[45], float phi, float theta, float &v, float &th,
-                                              float &ph, Terms &&terms = Terms{})
+// The same contraction handing each basis value to terms(i, Y_i) as well
+// (all 45, whatever K).
+template <class Terms>
+__device__ __forceinline__ void sh45_contract_terms(const float c[45], float phi, float theta, float &v, float &th,
+                                                    float &ph, Terms &&terms)
 {
     float sv = 0.0f, sth = 0.0f, sph = 0.0f;
-    sh45_terms(phi, theta, [&](int i, float y, float dp, float dt) {
-        const float ci = c[i];
-        sv += ci * y;
-        sth += ci * dt;
-        sph += ci * dp;
-        terms(i, y);
-    });
+#define TGK_ACC(i, Y, DP, DT)   \
+    {                           \
+        const float ci_ = c[i]; \
+        const float y_ = (Y);   \
+        sv += ci_ * y_;         \
+        sth += ci_ * (DT);      \
+        sph += ci_ * (DP);      \
+        terms(i, y_);           \
+    }
+#include "tgk_sh45_body.inc"
+#undef TGK_ACC
     v = sv;
     th = sth;
     ph = sph;
+}
+
+// The basis values alone, handed to visit(i, Y_i) in index order.
+template <class Visit>
+__device__ __forceinline__ void sh45_values(float phi, float theta, Visit &&visit)
+{
+#define TGK_ACC(i, Y, DP, DT) visit(i, (Y));
+#include "tgk_sh45_body.inc"
+#undef TGK_ACC
 }
 
 // Loads the first K of a voxel's coefficients (K <= 45, rows 16-byte aligned)
@@ -283,7 +160,10 @@ __device__ __forceinline__ ModelValue sh_model(const float *__restrict__ row, in
     float az, col;
     cart2sph(o, az, col);
     float v, th, ph;
-    sh45_contract(c, az, col, v, th, ph, terms);
+    if constexpr (std::is_same_v<std::decay_t<Terms>, NoTerms>)
+        sh45_contract(c, az, col, v, th, ph);
+    else
+        sh45_contract_terms(c, az, col, v, th, ph, terms);
     const float d = dsoftplus(v);
     ModelValue r;
     r.value = softplus(v);
