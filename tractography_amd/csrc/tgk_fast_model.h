@@ -269,11 +269,65 @@ __device__ __forceinline__ ModelOut model_at(const float *q, V3 c, const Directi
         return dti_model(q, c, dir);
 }
 
-// The value alone (the derivative terms are dead code here).
+// The SH value alone at a unit direction u: f = A_0(z) + sum_m A_m(z) Re u^m
+// + B_m(z) Im u^m with u = x + i y (= sin(theta) e^{i phi}), so no angles,
+// no normalisation and no derivative terms.
+template <int N>
+__device__ __forceinline__ float horner_value(const float *c, float w)
+{
+    float v = c[N - 1];
+#pragma unroll
+    for (int j = N - 2; j >= 0; j--)
+        v = v * w + c[j];
+    return v;
+}
+
+template <int L, int M>
+__device__ __forceinline__ void sh_value_terms(const float *q, float z, float w, float u1r, float u1i, float &ur,
+                                               float &ui, float &f)
+{
+    if constexpr (M <= L) {
+        using O = Order<L>;
+        const float nur = ur * u1r - ui * u1i;
+        ui = ur * u1i + ui * u1r;
+        ur = nur;
+        float a = horner_value<O::poly_terms(M)>(q + O::a_offset(M), w);
+        float b = horner_value<O::poly_terms(M)>(q + O::b_offset(M), w);
+        if constexpr ((M & 1) == 1) {
+            a *= z;
+            b *= z;
+        }
+        f += a * ur + b * ui;
+        sh_value_terms<L, M + 1>(q, z, w, u1r, u1i, ur, ui, f);
+    }
+}
+
+template <int L>
+__device__ __forceinline__ float sh_value_unit(const float *q, V3 u)
+{
+    using O = Order<L>;
+    const float w = u.z * u.z;
+    float f = horner_value<O::poly_terms(0)>(q + O::a_offset(0), w);
+    float ur = 1.0f, ui = 0.0f;
+    sh_value_terms<L, 1>(q, u.z, w, u.x, u.y, ur, ui, f);
+    return f;
+}
+
+// The model value alone at a direction of unit length up to rounding (cap
+// candidates, exps2 results): the reference normalises by the radius, which
+// here differs from 1 by ~1e-7.
 template <int MODEL, int LMAX>
 __device__ __forceinline__ float value_at(const float *q, V3 c)
 {
-    return model_at<MODEL, LMAX>(q, c, direction_of(c)).value;
+    if constexpr (MODEL == MODEL_SH) {
+        const float raw = sh_value_unit<LMAX>(q, c);
+        if (raw * 100.0f >= 30.0f)
+            return raw;
+        return __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(raw * 144.26950408889634f)) *
+               0.006931471805599453f;
+    } else {
+        return model_at<MODEL, LMAX>(q, c, direction_of(c)).value;
+    }
 }
 
 __device__ __forceinline__ V3 unit_or_z(V3 o)
