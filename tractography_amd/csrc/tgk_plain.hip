@@ -419,33 +419,64 @@ __global__ __launch_bounds__(256) void connectome_kernel(ConnectomeLaunch c)
             tracked++;
             continue;
         }
-        // The track has ended: its 9 entries.
-        int32_t *rows = c.rows + 9 * g;
-        int32_t *cols = c.cols + 9 * g;
-        float *weights = c.weights + 9 * g;
-        uint32_t hit = 0;
-        float t, bary[3];
-        const bool accepted = !((float)tracked * step < c.min_length) &&
-                              bvh_segment_hit_bary(c.boundary, previous, point, hit, t, bary);
-        if (accepted) {
-            const uint4 si = c.boundary.triangles[start_triangle];
-            const uint4 ei = c.boundary.triangles[hit];
-            const uint32_t sv[3] = {si.x, si.y, si.z};
-            const uint32_t ev[3] = {ei.x, ei.y, ei.z};
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++) {
-                    rows[3 * i + j] = (int32_t)sv[i];
-                    cols[3 * i + j] = (int32_t)ev[j];
-                    weights[3 * i + j] = start_bary[i] * bary[j];
-                }
-        } else {
-            for (int i = 0; i < 9; i++) {
-                rows[i] = -1;
-                cols[i] = -1;
-                weights[i] = 0.0f;
-            }
-        }
+        // The track has ended: its last segment and start go into the
+        // seed's own output slots (weights: previous, point, start
+        // barycentrics; rows: start triangle, length test) for
+        // connectome_hits_kernel, which tests every segment with all lanes
+        // busy instead of one ending lane at a time.
+        float *record = c.weights + 9 * g;
+        record[0] = previous.x;
+        record[1] = previous.y;
+        record[2] = previous.z;
+        record[3] = point.x;
+        record[4] = point.y;
+        record[5] = point.z;
+        record[6] = start_bary[0];
+        record[7] = start_bary[1];
+        record[8] = start_bary[2];
+        c.rows[9 * g] = (int32_t)start_triangle;
+        c.rows[9 * g + 1] = (float)tracked * step < c.min_length ? 0 : 1;
         g = -1;
+    }
+}
+
+// tracking.py:526-558: the last segment of seed g against the boundary BVH
+// (closest hit with barycentrics) and the 9 (row, col, weight) entries, from
+// the record connectome_kernel left in the seed's slots.
+__global__ __launch_bounds__(256) void connectome_hits_kernel(ConnectomeLaunch c)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= c.n_seeds)
+        return;
+    int32_t *rows = c.rows + 9 * g;
+    int32_t *cols = c.cols + 9 * g;
+    float *weights = c.weights + 9 * g;
+    const V3 previous = v3(weights[0], weights[1], weights[2]);
+    const V3 point = v3(weights[3], weights[4], weights[5]);
+    const float start_bary[3] = {weights[6], weights[7], weights[8]};
+    const uint32_t start_triangle = (uint32_t)rows[0];
+    // Every seed's record is written before this kernel runs; the index is
+    // still checked so that no record can send a read out of bounds.
+    const bool long_enough = rows[1] != 0 && start_triangle < c.n_triangles;
+    uint32_t hit = 0;
+    float t, bary[3];
+    if (long_enough && bvh_segment_hit_bary(c.boundary, previous, point, hit, t, bary)) {
+        const uint4 si = c.boundary.triangles[start_triangle];
+        const uint4 ei = c.boundary.triangles[hit];
+        const uint32_t sv[3] = {si.x, si.y, si.z};
+        const uint32_t ev[3] = {ei.x, ei.y, ei.z};
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                rows[3 * i + j] = (int32_t)sv[i];
+                cols[3 * i + j] = (int32_t)ev[j];
+                weights[3 * i + j] = start_bary[i] * bary[j];
+            }
+    } else {
+        for (int i = 0; i < 9; i++) {
+            rows[i] = -1;
+            cols[i] = -1;
+            weights[i] = 0.0f;
+        }
     }
 }
 
@@ -769,9 +800,13 @@ hipError_t launch_persistent(Kernel kernel, const Launch &args, uint32_t *queue,
 template <int MODEL, int ALGO>
 hipError_t launch_connectome_kt(const ConnectomeLaunch &c, hipStream_t stream)
 {
-    if (MODEL == MODEL_SH && c.algo.n_parameters == 45)
-        return launch_persistent(connectome_kernel<MODEL, ALGO, 45>, c, c.queue, c.n_seeds, stream);
-    return launch_persistent(connectome_kernel<MODEL, ALGO, 0>, c, c.queue, c.n_seeds, stream);
+    const hipError_t e = MODEL == MODEL_SH && c.algo.n_parameters == 45
+                             ? launch_persistent(connectome_kernel<MODEL, ALGO, 45>, c, c.queue, c.n_seeds, stream)
+                             : launch_persistent(connectome_kernel<MODEL, ALGO, 0>, c, c.queue, c.n_seeds, stream);
+    if (e != hipSuccess)
+        return e;
+    connectome_hits_kernel<<<blocks_for(c.n_seeds, 256), 256, 0, stream>>>(c);
+    return hipGetLastError();
 }
 
 template <int MODEL>
