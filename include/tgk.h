@@ -203,8 +203,8 @@ int tgk_tod(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundar
             const uint32_t output_dims[3], const float output_inverse_affine[16], int32_t normalize,
             float *coefficients);
 /* The same on the runtime stream, into device memory the caller owns
- * (X * Y * Z * 45 floats, cleared by the call). No allocation, so it may be
- * recorded into a graph. */
+ * (X * Y * Z * 45 floats from hipMalloc, cleared by the call: the sums are
+ * device float atomics). No allocation, so it may be recorded into a graph. */
 int tgk_tod_device(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
                    const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
                    const uint32_t output_dims[3], const float output_inverse_affine[16], int32_t normalize,

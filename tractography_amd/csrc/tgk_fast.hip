@@ -15,7 +15,9 @@
 // Compiled with -ffp-contract=fast.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "tgk_fast_model.h"
@@ -345,16 +347,19 @@ template <int L> std::vector<double> sh_to_poly_matrix()
 }
 
 // The map of order L on the current device, uploaded on first use and kept
-// for the life of the process.
+// for the life of the process. Runtimes on different threads may ask for it
+// at once: the first upload per device happens under a lock.
 template <int L> hipError_t device_matrix(const double **out)
 {
     static const double *matrices[64] = {};
+    static std::mutex lock;
     int device = 0;
     hipError_t e = hipGetDevice(&device);
     if (e != hipSuccess)
         return e;
     if (device < 0 || device >= 64)
         return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> guard(lock);
     if (!matrices[device]) {
         const std::vector<double> T = sh_to_poly_matrix<L>();
         void *ptr = nullptr;
@@ -372,16 +377,18 @@ template <int L> hipError_t device_matrix(const double **out)
     return hipSuccess;
 }
 
+// Blocks that fill the current device once, computed on first use per
+// device (atomically published: runtimes on other threads may read it).
 template <int MODEL, int ALGO, int DOMAIN, int LMAX> hipError_t occupancy_grid(int *blocks)
 {
-    static int cache[64] = {};
+    static std::atomic<int> cache[64] = {};
     int device = 0;
     hipError_t e = hipGetDevice(&device);
     if (e != hipSuccess)
         return e;
     if (device < 0 || device >= 64)
         return hipErrorInvalidDevice;
-    if (cache[device] == 0) {
+    if (cache[device].load(std::memory_order_acquire) == 0) {
         int per_cu = 0;
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, reinterpret_cast<const void *>(track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX>), fast_block<LMAX>(), 0);
@@ -391,9 +398,9 @@ template <int MODEL, int ALGO, int DOMAIN, int LMAX> hipError_t occupancy_grid(i
         e = hipGetDeviceProperties(&prop, device);
         if (e != hipSuccess)
             return e;
-        cache[device] = prop.multiProcessorCount * (per_cu > 0 ? per_cu : 1);
+        cache[device].store(prop.multiProcessorCount * (per_cu > 0 ? per_cu : 1), std::memory_order_release);
     }
-    *blocks = cache[device];
+    *blocks = cache[device].load(std::memory_order_acquire);
     return hipSuccess;
 }
 
