@@ -16,7 +16,8 @@ Other workloads (--workload): sphere32 (32^3 SH45 Transport, 10k seeds), dti
 (145x174x145 tensors, Transport, 1M seeds), shell (the SH45 field bounded by
 two displaced ellipsoid meshes of ~200k triangles each, Diffusion, 2M seeds).
 --product tod / connectome times that product instead (tracks from boundary
-seeds of the workload's mask, plain FP32 path) and prints its own line.
+seeds of the workload's mask, plain FP32 path) and prints its own line;
+--product seeds times seeds.from_fod on the device (VALU and MFMA methods).
 The module also holds the synthetic workload generators used by the tests.
 """
 
@@ -685,6 +686,66 @@ def host_leg(cfg, gpu_results: dict, seconds: float = 15.0, n_accuracy: int = 40
 
 
 Y0_F32 = float(np.float32(0.28209479177387814))  # the constant l = 0 basis value, as deposited
+MI355X_BF16_DENSE_TFLOPS = 2500.0  # dense MFMA bf16 peak (no sparsity)
+
+
+def seeds_leg(args, tg, torch, cfg, workload: str):
+    """Time seeds.from_fod on the device (seeds.py:145-240): n seeds drawn
+    from the workload's FOD inside its mask, states resident in HBM, for the
+    VALU method (the reference's arithmetic) and the split-bf16 MFMA method.
+    The MFMA rate is priced at its executed product flops, 2 passes x 1000
+    directions x 2K flop x 3 split products per seed (an upper bound: the
+    second pass stops once every lane of a wavefront has chosen)."""
+    from tractography_amd import _native
+
+    lib = _native.lib()
+    runtime = tg.HipRuntime()
+    fod = cfg["fod"] * cfg["mask"][..., None]
+    K = fod.shape[-1]
+    inline, voxels, affine = tg.seeds.sparse_fod(tg.Nifti1Image(fod, np.eye(4)))
+    n = int(args.seeds or 10_000_000)
+    dev = torch.device("cuda", runtime.device)
+    d_fod = torch.from_numpy(inline).to(dev)
+    d_voxels = torch.from_numpy(voxels).to(dev)
+    d_affine = torch.from_numpy(affine).to(dev)
+    states = np.random.default_rng(0).integers(0, 2**32 - 1, (n, 2), dtype=np.uint32)
+    d_states0 = torch.from_numpy(states.view(np.int32)).to(dev)
+    d_states = d_states0.clone()
+    d_seeds = torch.empty((n, 8), dtype=torch.float32, device=dev)
+    stream = torch.cuda.ExternalStream(runtime.stream(), device=dev)
+    methods = {"valu": _native.SEEDS_VALU, "mfma": _native.SEEDS_MFMA} if K <= 45 else {"mfma": _native.SEEDS_MFMA}
+    results = {}
+    for name, method in methods.items():
+        _native.check(lib.tgk_seeds_prepare(runtime.handle, 0, K, method))
+        times = []
+        for i in range(args.warmup + args.steps):
+            d_states.copy_(d_states0)
+            torch.cuda.synchronize()
+            start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record(stream)
+            _native.check(lib.tgk_seeds_from_fod_device(runtime.handle, 0, K, d_fod.data_ptr(), d_voxels.data_ptr(),
+                                                        len(voxels), d_affine.data_ptr(), d_states.data_ptr(), n,
+                                                        method, d_seeds.data_ptr()))
+            stop.record(stream)
+            runtime.finish()
+            torch.cuda.synchronize()
+            if i >= args.warmup:
+                times.append(start.elapsed_time(stop))
+        ms = float(np.mean(times))
+        results[name] = {"ms": ms, "seeds_per_s": n / (ms * 1e-3)}
+        print(f"[bench] seeds {name}: {ms:.1f} ms, {n / (ms * 1e-3):.3e} seeds/s", file=sys.stderr, flush=True)
+    line = {"metric": "seeds.from_fod seeds/s on 1 MI355X", "unit": "seeds/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "data": "synthetic (seeded numpy generators)",
+            "config": {"workload": workload, "product": "seeds", "seeds": n, "K": K, "nnz": len(voxels)},
+            "methods": results, "value": results["mfma"]["seeds_per_s"], "ms_per_step": results["mfma"]["ms"],
+            "dtype": "bf16 (hi+lo split) MFMA, f32 accumulate"}
+    flops = n * 2 * 1000 * 2 * K * 3
+    achieved = flops / (results["mfma"]["ms"] * 1e-3) / 1e12
+    line["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": MI355X_BF16_DENSE_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / MI355X_BF16_DENSE_TFLOPS, "kernel": "seeds_mfma_kernel",
+                        "flops_per_seed_upper_bound": 2 * 1000 * 2 * K * 3}
+    print(json.dumps(line), flush=True)
 
 
 def product_leg(args, tg, torch, cfg, workload: str):
@@ -824,8 +885,9 @@ def main():
     parser.add_argument("--algorithm", default=None, choices=["transport", "diffusion", "probabilistic", "deterministic"],
                         help="replace the workload's algorithm (Probabilistic and Deterministic: plain path only, "
                              "the reference's default parameters)")
-    parser.add_argument("--product", default="tractogram", choices=["tractogram", "tod", "connectome"],
-                        help="tod / connectome: time that product over boundary seeds of the workload's mask")
+    parser.add_argument("--product", default="tractogram", choices=["tractogram", "tod", "connectome", "seeds"],
+                        help="tod / connectome: time that product over boundary seeds of the workload's mask; "
+                             "seeds: time seeds.from_fod on the device (VALU and MFMA methods)")
     args = parser.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (seeds shard with no "
@@ -852,6 +914,9 @@ def main():
         cfg["algorithm"] = args.algorithm
         workload += f"; algorithm replaced by {args.algorithm}"
     print(f"[bench] workload generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    if args.product == "seeds":
+        seeds_leg(args, tg, torch, cfg, workload)
+        return
     if args.product != "tractogram":
         if cfg.get("mask") is None:
             raise SystemExit(f"--product {args.product} needs a workload with a mask domain")
