@@ -1,10 +1,12 @@
 """MI355X-native streamline tracking, a drop-in for the OpenCL backend of the
 reference ``tractography`` package on its tracking path.
 
-``import tractography_amd as tg`` then ``tg.tractogram(...)``,
-``tg.algorithm.Transport/Diffusion``, ``tg.local.*``, ``tg.field.Image``,
-``tg.domain.Image`` and ``tg.seeds.*`` with the reference's signatures. The
-device work runs in hand-written HIP kernels (libtgk.so) on gfx950.
+``import tractography_amd as tg`` then ``tg.tractogram(...)``, ``tg.tod(...)``,
+``tg.connectome(...)``, ``tg.algorithm.Transport/Diffusion/Probabilistic/
+Deterministic``, ``tg.local.*``, ``tg.field.Image``, ``tg.domain.Image`` and
+``tg.seeds.*`` with the reference's signatures, and ``tg.streaming`` for
+batch-streamed TCK/TRX output. The device work runs in hand-written HIP
+kernels (libtgk.so) on gfx950.
 """
 
 from __future__ import annotations

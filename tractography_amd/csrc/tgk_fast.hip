@@ -6,10 +6,11 @@
 //   streamlines of very different lengths (5 to 3000 steps) do not leave
 //   lanes idle until the longest of a wavefront ends.
 // * Per-lane cell cache: mask value and the voxel's model coefficients stay
-//   in registers until the point crosses into another cell (about every ten
-//   steps at step 0.1 voxel).
+//   on chip (LDS at order 8, registers at order 12 and for DTI) until the
+//   point crosses into another cell (about every ten steps at step 0.1).
 // * Transcendental-free SH model (tgk_fast_model.h), hardware exp2/log2/cos
-//   where the reference calls exp/log/cos, FP64 arc length.
+//   where the reference calls exp/log/cos, FP64 arc length; the four
+//   orientation updates (Transport, Diffusion, Probabilistic, Deterministic).
 // * Seed-major dense output rows, compacted into ragged points afterwards.
 //
 // Compiled with -ffp-contract=fast.

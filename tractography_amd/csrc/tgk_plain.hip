@@ -1,8 +1,10 @@
-// tgk_plain.hip -- the plain FP32 tracking path and the one-op drivers.
+// tgk_plain.hip -- the plain FP32 tracking path, the boundary-seeded
+// products (connectome, TOD) and the one-op drivers.
 //
 // One integrator per seed, one lane per seed, exactly the reference's loop
 // (tracking.py:112-148). Compiled with -ffp-contract=off: every result is bit
-// identical to the FP32 host restatement.
+// identical to the FP32 host restatement. The connectome and TOD kernels run
+// the same arithmetic on persistent lanes that claim boundary seeds.
 //
 // Output layout: lanes of a wavefront start together and advance `time` by
 // the same step, so all live lanes save on the same iteration. Points are
