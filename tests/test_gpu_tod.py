@@ -222,3 +222,27 @@ def test_tod_persistent_lanes_refill(runtime):
     raw, abs_sum, flushes = oracle.tod(domain, field, algorithm, seed, 400_000, 12, dims, inverse)
     assert flushes.sum() > 400_000
     assert_same_sums(got, raw, abs_sum, flushes, "refill")
+
+
+def test_tod_rejects_invalid_grids_and_domains(runtime):
+    domain, field, algorithm, shape = _scene("sh45-transport")
+    handle = domain.upload(runtime, with_boundary=True)
+    lib = _native.lib()
+    out = np.zeros(45, np.float32)
+
+    def call(dims, inverse, dom=handle):
+        dims = np.ascontiguousarray(dims, dtype=np.uint32)
+        inverse = np.ascontiguousarray(inverse, dtype=np.float32).reshape(16)
+        return lib.tgk_tod(runtime.handle, dom.handle, dom.handle, field.upload(runtime).handle,
+                           algorithm.descriptor(), 1, 10, 10, _native.ptr(dims), _native.ptr(inverse), 0,
+                           _native.ptr(out))
+
+    assert call([2**31, 2**31, 2**31], np.eye(4)) == _native.ERR_INVALID
+    assert b"too large" in lib.tgk_last_error()
+    bad = np.eye(4)
+    bad[0, 3] = np.nan
+    assert call([1, 1, 1], bad) == _native.ERR_INVALID
+    assert b"finite" in lib.tgk_last_error()
+    mesh, _, _, _ = fixtures.shell_scene(n_seeds=10, seed=1)
+    assert call([1, 1, 1], np.eye(4), dom=mesh.upload(runtime)) == _native.ERR_INVALID
+    assert b"mask domain" in lib.tgk_last_error()

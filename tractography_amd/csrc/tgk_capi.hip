@@ -1133,9 +1133,13 @@ int tod_launch(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boun
     for (int i = 0; i < 12; i++)
         if (!std::isfinite(output_inverse_affine[i]))
             return fail(TGK_ERR_INVALID, "output inverse affine must be finite");
-    const uint64_t cells = (uint64_t)output_dims[0] * output_dims[1] * output_dims[2];
-    if (cells > (uint64_t)INT64_MAX / (45 * sizeof(float)))
+    // Cells and bytes of the output grid, refusing any product that would
+    // not fit (each factor is below 2^32, so the first product cannot wrap).
+    const uint64_t limit = (uint64_t)INT64_MAX / (45 * sizeof(float));
+    const uint64_t plane = (uint64_t)output_dims[0] * output_dims[1];
+    if (output_dims[2] != 0 && plane > limit / output_dims[2])
         return fail(TGK_ERR_INVALID, "output grid too large");
+    const uint64_t cells = plane * output_dims[2];
     memset(t, 0, sizeof *t);
     t->mask = mask_view(domain);
     t->field = field_view(field);
