@@ -68,12 +68,18 @@ struct Lane {
 // a streamline in flight; on termination the lane's results are written and
 // the lane becomes idle (L.g = -1).
 // Where a lane keeps its cell's coefficients between steps: SH rows of
-// order 8 in LDS (one float4 column per lane, read back every step), so no
-// register array is carried around the loop; DTI rows and order-12 rows in
-// registers.
+// order 8 mostly in LDS (float4 columns, one per lane, read back every
+// step), so no large register array is carried around the loop; DTI rows
+// and order-12 rows in registers.
 template <int MODEL, int LMAX> constexpr bool rows_in_lds() { return MODEL == MODEL_SH && LMAX == 8; }
 // Threads per block of the fast kernel.
 template <int LMAX> constexpr int fast_block() { return 256; }
+// float4 columns of a row kept in LDS. At order 8, 10 of the 12 (the first
+// 40 of 45 coefficients): 40 KiB per 256-lane block, so four blocks share a
+// CU's 160 KiB and each SIMD runs four waves (at 109 VGPRs the registers
+// would allow it; the full 48-float row would allow three). The other 5
+// coefficients stay in registers.
+template <int MODEL, int LMAX> constexpr int lds_vectors() { return rows_in_lds<MODEL, LMAX>() ? 10 : 0; }
 
 template <int MODEL, int ALGO, int DOMAIN, int LMAX>
 __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__restrict__ coeffs, uint32_t cstride,
@@ -81,6 +87,7 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
 {
     constexpr int TERMS = Order<LMAX>::TERMS;
     constexpr int VEC = Order<LMAX>::STRIDE / 4;
+    constexpr int LDS_VEC = lds_vectors<MODEL, LMAX>();
     bool stop = !(L.n < p.n_points && L.steps < p.max_steps);
     uint32_t hit = NO_HIT;
     if (!stop) {
@@ -127,13 +134,12 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
     if (cell != L.field_cell) {
         L.field_cell = cell;
         const float4 *row = reinterpret_cast<const float4 *>(coeffs + cell * (int64_t)cstride);
-        if constexpr (rows_in_lds<MODEL, LMAX>()) {
+        if constexpr (MODEL == MODEL_SH) {
 #pragma unroll
-            for (int j = 0; j < VEC; j++)
+            for (int j = 0; j < LDS_VEC; j++)
                 lds_rows[j * fast_block<LMAX>() + threadIdx.x] = row[j];
-        } else if constexpr (MODEL == MODEL_SH) {
 #pragma unroll
-            for (int j = 0; j < VEC; j++) {
+            for (int j = LDS_VEC; j < VEC; j++) {
                 const float4 v = row[j];
                 if (4 * j < TERMS)
                     q[4 * j] = v.x;
@@ -156,7 +162,7 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
     }
     if constexpr (rows_in_lds<MODEL, LMAX>()) {
 #pragma unroll
-        for (int j = 0; j < VEC; j++) {
+        for (int j = 0; j < LDS_VEC; j++) {
             const float4 v = lds_rows[j * fast_block<LMAX>() + threadIdx.x];
             if (4 * j < TERMS)
                 q[4 * j] = v.x;
@@ -198,7 +204,7 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
 template <int MODEL, int ALGO, int DOMAIN, int LMAX>
 __global__ __launch_bounds__(fast_block<LMAX>()) void track_fast_kernel(FastLaunch f)
 {
-    __shared__ float4 lds_rows[rows_in_lds<MODEL, LMAX>() ? Order<LMAX>::STRIDE / 4 * fast_block<LMAX>() : 1];
+    __shared__ float4 lds_rows[rows_in_lds<MODEL, LMAX>() ? lds_vectors<MODEL, LMAX>() * fast_block<LMAX>() : 1];
     const TrackLaunch &p = f.track;
     const int lane_id = __lane_id();
     const float *coeffs = static_cast<const float *>(f.coeffs);
