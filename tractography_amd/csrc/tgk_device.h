@@ -112,10 +112,11 @@ __device__ __forceinline__ uint32_t mwc64x(uint2 &s)
 {
     const uint32_t A = 4294883355u;
     const uint32_t res = s.x ^ s.y;
-    const uint32_t hi = __umulhi(s.x, A);
-    const uint32_t x = s.x * A + s.y;
-    s.y = hi + (x < s.y ? 1u : 0u);
-    s.x = x;
+    // x A + c as one 64-bit multiply-add (v_mad_u64_u32): low half the new
+    // x, high half the new carry.
+    const uint64_t next = (uint64_t)s.x * A + s.y;
+    s.x = (uint32_t)next;
+    s.y = (uint32_t)(next >> 32);
     return res;
 }
 

@@ -178,14 +178,11 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
         L.o = fast::probabilistic<MODEL, LMAX>(p.algo, q, L.o, L.rng);
     } else if constexpr (ALGO == ALGO_DETERMINISTIC) {
         L.o = fast::deterministic<MODEL, LMAX>(p.algo, q, L.o);
+    } else if constexpr (MODEL == MODEL_SH) {
+        L.o = fast::sh_update<ALGO, LMAX>(p.algo, q, L.o, L.rng);
     } else {
         const fast::Direction dir = fast::direction_of(L.o);
-        fast::ModelOut mv;
-        if constexpr (MODEL == MODEL_SH)
-            mv = fast::sh_model<LMAX>(q, dir);
-        else
-            mv = fast::dti_model(q, L.o, dir);
-        L.o = fast::update<ALGO>(p.algo, mv, dir, L.o, L.rng);
+        L.o = fast::update<ALGO>(p.algo, fast::dti_model(q, L.o, dir), dir, L.o, L.rng);
     }
     L.steps++;
     L.previous = L.point;

@@ -186,6 +186,78 @@ __device__ __forceinline__ ModelOut sh_model(const float *q, const Direction &di
     return m;
 }
 
+// The SH value and its gradient on the sphere without angles. The
+// tangential gradient of a function on the unit sphere is the ambient
+// gradient G of any extension, minus its normal part: G - (G.n) n. The
+// extension taken is P(x, y, z) = A_0(z) + sum_m A_m(z) Re (x+iy)^m +
+// B_m(z) Im (x+iy)^m, whose partial derivatives are
+//   dP/dx = sum_m m (A_m Re w_m + B_m Im w_m),
+//   dP/dy = sum_m m (B_m Re w_m - A_m Im w_m),
+//   dP/dz = A_0'(z) + sum_m A_m'(z) Re u^m + B_m'(z) Im u^m,
+// with u = x + iy and w_m = u^(m-1): one complex power per order and no
+// sin(theta) factors, so it is also regular at the poles.
+struct Cartesian {
+    float value, gx, gy, gz;
+};
+
+template <int L, int M>
+__device__ __forceinline__ void sh_cartesian_terms(const float *q, float z, float w, float x, float y, float &pr,
+                                                   float &pi, Cartesian &g)
+{
+    if constexpr (M <= L) {
+        using O = Order<L>;
+        float ur, ui;
+        if constexpr (M == 1) {
+            ur = x;
+            ui = y;
+        } else {
+            ur = pr * x - pi * y;
+            ui = pr * y + pi * x;
+        }
+        float a, ad, b, bd;
+        poly_m<M, O::poly_terms(M)>(q + O::a_offset(M), z, w, a, ad);
+        poly_m<M, O::poly_terms(M)>(q + O::b_offset(M), z, w, b, bd);
+        g.value += a * ur + b * ui;
+        g.gz += ad * ur + bd * ui;
+        if constexpr (M == 1) {
+            g.gx += a;
+            g.gy += b;
+        } else {
+            g.gx += (float)M * (a * pr + b * pi);
+            g.gy += (float)M * (b * pr - a * pi);
+        }
+        pr = ur;
+        pi = ui;
+        sh_cartesian_terms<L, M + 1>(q, z, w, x, y, pr, pi, g);
+    }
+}
+
+// Raw (pre-softplus) value and ambient gradient of P at the unit vector n.
+template <int L>
+__device__ __forceinline__ Cartesian sh_cartesian(const float *q, V3 n)
+{
+    using O = Order<L>;
+    const float w = n.z * n.z;
+    Cartesian g;
+    poly_m<0, O::poly_terms(0)>(q + O::a_offset(0), n.z, w, g.value, g.gz);
+    g.gx = 0.0f;
+    g.gy = 0.0f;
+    float pr = 1.0f, pi = 0.0f;
+    sh_cartesian_terms<L, 1>(q, n.z, w, n.x, n.y, pr, pi, g);
+    return g;
+}
+
+// The direction cart2sph evaluates the model at: o / |o|, and the zero
+// vector at colatitude 0.
+__device__ __forceinline__ V3 unit_direction(V3 o)
+{
+    const float r2 = o.x * o.x + o.y * o.y + o.z * o.z;
+    if (!(r2 > 0.0f))
+        return v3(0.0f, 0.0f, 1.0f);
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    return v3(o.x * inv, o.y * inv, o.z * inv);
+}
+
 // local.py:212-230: t = D o, f = o.t, df/dtheta = 2 t.e_theta,
 // (df/dphi)/sin(theta) = 2 t.e_phi (o as given, unit frame vectors).
 __device__ __forceinline__ ModelOut dti_model(const float m[8], V3 o, const Direction &dir)
@@ -251,6 +323,44 @@ __device__ __forceinline__ V3 update(const AlgoParams &a, const ModelOut &mv, co
             tx += sn * (r1 * etx + r2 * epx);
             ty += sn * (r1 * ety + r2 * epy);
             tz += sn * (r1 * etz);
+        }
+    }
+    return exps2(o, v3(tx, ty, tz));
+}
+
+// Transport / Diffusion with the SH model: the same update as above, the
+// model gradient taken in Cartesian form (sh_cartesian). The tangent frame
+// (e_theta, e_phi) is formed only for the Diffusion noise, which is drawn
+// in it.
+template <int ALGO, int L>
+__device__ __forceinline__ V3 sh_update(const AlgoParams &a, const float *q, V3 o, uint2 &rng)
+{
+    const V3 n = unit_direction(o);
+    const Cartesian g = sh_cartesian<L>(q, n);
+    float value, d;
+    softplus(g.value, value, d);
+    const float normal = g.gx * n.x + g.gy * n.y + g.gz * n.z;
+    const float scale = a.gamma * a.step_size * __builtin_amdgcn_rcpf(fmaxf(value, 1e-6f)) * d;
+    float tx = scale * (g.gx - normal * n.x);
+    float ty = scale * (g.gy - normal * n.y);
+    float tz = scale * (g.gz - normal * n.z);
+    if constexpr (ALGO == ALGO_DIFFUSION) {
+        if (a.noise_variance > 0.0f) {
+            const float rho2 = n.x * n.x + n.y * n.y;
+            float c1 = 1.0f, s1 = 0.0f, sin_theta = 0.0f;
+            if (rho2 > 0.0f) {
+                const float inv_rho = __builtin_amdgcn_rsqf(rho2);
+                c1 = n.x * inv_rho;
+                s1 = n.y * inv_rho;
+                sin_theta = rho2 * inv_rho;
+            }
+            const float z = fminf(fmaxf(n.z, -1.0f), 1.0f);
+            const float r1 = randn(rng);
+            const float r2 = randn(rng);
+            const float sn = __builtin_sqrtf(a.noise_variance * a.gamma * a.step_size);
+            tx += sn * (r1 * (z * c1) - r2 * s1);
+            ty += sn * (r1 * (z * s1) + r2 * c1);
+            tz += sn * (r1 * -sin_theta);
         }
     }
     return exps2(o, v3(tx, ty, tz));
@@ -372,10 +482,19 @@ __device__ __forceinline__ V3 probabilistic(const AlgoParams &a, const float *q,
     return choose_candidate(a.n_samples, rng, candidate, weight, chosen) ? chosen : o;
 }
 
-// Gradient on the sphere: dtheta e_theta + (dphi / sin theta) e_phi.
+// Gradient on the sphere: dtheta e_theta + (dphi / sin theta) e_phi; for
+// SH in Cartesian form (sh_cartesian), which needs no frame.
 template <int MODEL, int LMAX>
 __device__ __forceinline__ V3 sphere_gradient(const float *q, V3 p)
 {
+    if constexpr (MODEL == MODEL_SH) {
+        const V3 n = unit_direction(p);
+        const Cartesian g = sh_cartesian<LMAX>(q, n);
+        float value, d;
+        softplus(g.value, value, d);
+        const float normal = g.gx * n.x + g.gy * n.y + g.gz * n.z;
+        return v3(d * (g.gx - normal * n.x), d * (g.gy - normal * n.y), d * (g.gz - normal * n.z));
+    }
     const Direction d = direction_of(p);
     const ModelOut m = model_at<MODEL, LMAX>(q, p, d);
     return v3(m.dtheta * (d.z * d.c1) - m.dphi * d.s1, m.dtheta * (d.z * d.s1) + m.dphi * d.c1, -m.dtheta * d.s);
