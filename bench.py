@@ -403,6 +403,9 @@ def brain145_sh91_fod_seeds(seed: int = 0, n_seeds: int = 10_000_000, shape=SHAP
 # --------------------------------------------------------------------------
 
 MI355X_HBM_PEAK_GBS = 8000.0
+# Wavefront VALU instructions per second, one per SIMD every 4 cycles (a
+# wave64 FP32 instruction on a 16-lane SIMD): 256 CUs x 4 SIMDs x 2.4 GHz / 4.
+MI355X_VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 
 def bytes_per_step(K: int, step: float, save_at: float, mask: bool = True) -> float:
@@ -982,7 +985,13 @@ def main():
                      "kernel": f"track_{headline}", "bytes_per_step": bps},
     }
     if traffic and headline == "fast":
-        line["roofline"]["traffic_detail"] = traffic
+        line["roofline"]["traffic_detail"] = {k: v for k, v in traffic.items() if k != "valu_instructions"}
+        # The binding resource: wavefront VALU instructions issued per second
+        # of the kernel against the issue rate of all SIMDs.
+        issued = traffic["valu_instructions"] / (h["kernel_ms"] * 1e-3)
+        line["roofline"]["issue"] = {"bound": "valu", "achieved": issued, "peak": MI355X_VALU_ISSUE_PEAK,
+                                     "unit": "wave instructions/s", "frac": issued / MI355X_VALU_ISSUE_PEAK,
+                                     "wave_instructions_per_64_steps": traffic["valu_instructions"] * 64.0 / h["steps"]}
     if "fast" in results and "plain" in results:
         line["speedup_fast_over_plain"] = (results["fast"]["steps"] / results["fast"]["ms_per_step"]) / (
             results["plain"]["steps"] / results["plain"]["ms_per_step"])
@@ -1031,8 +1040,9 @@ def traffic_leg(args):
     """HBM bytes per launch of the fast tracking kernel from two rocprofv3
     counter runs of this script (FETCH_SIZE, then WRITE_SIZE, KiB per
     dispatch), each scaled by what the counter reads for an elementwise copy
-    that reads and writes 1 GiB in the same run. Returns None if the profiler
-    is unavailable or a run fails."""
+    that reads and writes 1 GiB in the same run, and the kernel's wavefront
+    VALU instructions per launch from a third run (SQ_INSTS_VALU). Returns
+    None if the profiler is unavailable or a run fails."""
     import shutil
     import subprocess
     import tempfile
@@ -1042,7 +1052,7 @@ def traffic_leg(args):
         return None
     measured = {}
     with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
-        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        for counter in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU"):
             out = Path(tmp) / counter
             cmd = [profiler, "--pmc", counter, "--output-format", "csv", "-d", str(out), "-o", "run", "--",
                    sys.executable, str(ROOT / "bench.py"), "--workload", args.workload, "--paths", "fast",
@@ -1062,6 +1072,11 @@ def traffic_leg(args):
                 return None
             values = _counter_values(files[0], counter)
             kernel = [v for k, v in values.items() if "track_fast_kernel" in k]
+            if counter == "SQ_INSTS_VALU":
+                if not kernel:
+                    return None
+                measured[counter] = float(np.mean(kernel[0]))
+                continue
             copy = [max(v) for k, v in values.items() if "CUDAFunctorOnSelf_add" in k]
             if not kernel or not copy or copy[0] <= 0:
                 return None
@@ -1070,7 +1085,8 @@ def traffic_leg(args):
                                  "bytes": float(np.mean(kernel[0])) * 1024.0 * scale}
     return {"bytes_per_launch": measured["FETCH_SIZE"]["bytes"] + measured["WRITE_SIZE"]["bytes"],
             "fetch_bytes": measured["FETCH_SIZE"]["bytes"], "write_bytes": measured["WRITE_SIZE"]["bytes"],
-            "fetch_scale": measured["FETCH_SIZE"]["scale"], "write_scale": measured["WRITE_SIZE"]["scale"]}
+            "fetch_scale": measured["FETCH_SIZE"]["scale"], "write_scale": measured["WRITE_SIZE"]["scale"],
+            "valu_instructions": measured["SQ_INSTS_VALU"]}
 
 
 def calibration_copy(torch, n_bytes: int):

@@ -144,7 +144,8 @@ int tgk_domain_destroy(tgk_domain *domain);
  * (x, y, z, 1, dx, dy, dz, 0); seed g draws its noise stream from
  * rng_init(rng_seeds[g / batch_size], g % batch_size) like the reference's
  * per-batch get_rng_seed(); n_points = floor(max_length / save_at) is the
- * point capacity including the seed. Results stay in the returned handle. */
+ * point capacity including the seed. Results stay in the returned handle:
+ * the per-seed counts in host memory, the ragged points in device memory. */
 int tgk_track(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field, const tgk_algorithm *algorithm,
               int32_t path, const float *seeds, int64_t n_seeds, const uint32_t *rng_seeds, int64_t batch_size,
               uint32_t n_points, float save_at, tgk_tracks **out);
@@ -152,7 +153,9 @@ int tgk_track(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field,
 int tgk_tracks_size(const tgk_tracks *tracks, int64_t *n_seeds, int64_t *total_points);
 /* Copies per-seed saved-point counts, orientation-update counts, crossed
  * boundary triangle (0xffffffff when none) and the ragged points
- * (total_points, 3) in seed order. Any output pointer may be NULL. */
+ * (total_points, 3) in seed order. Any output pointer may be NULL. The
+ * points come straight from device memory: the call blocks until they are
+ * in `points`. */
 int tgk_tracks_copy(const tgk_tracks *tracks, uint32_t *lengths, uint32_t *steps, uint32_t *hits, float *points);
 int tgk_tracks_destroy(tgk_tracks *tracks);
 

@@ -124,8 +124,13 @@ struct tgk_domain {
 };
 
 struct tgk_tracks {
+    int device = 0;
     std::vector<uint32_t> lengths, steps, hits;
-    std::vector<float> points;
+    // The ragged points of each chunk, left in device memory (exactly sized)
+    // until tgk_tracks_copy moves them once into the caller's array.
+    std::vector<std::unique_ptr<DeviceBuffer>> pieces;
+    std::vector<int64_t> piece_points;
+    int64_t total_points = 0;
 };
 
 namespace {
@@ -740,13 +745,19 @@ int tgk_track(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field,
         *out = tracks.release();
         return TGK_OK;
     }
-    // Bound the dense per-chunk buffer (points are compacted before copy-back).
+    // Chunk size: the dense per-seed rows of a chunk and its compacted copy
+    // take chunk * per_seed bytes each; a chunk may use a quarter of the free
+    // device memory, between 2 and 16 GiB (1.2M seeds of 1200 points).
     const size_t per_seed = (size_t)n_points * 3 * sizeof(float);
-    size_t budget = (size_t)1 << 31;
+    size_t budget = (size_t)2 << 30;
+    size_t free_bytes = 0, total_bytes = 0;
+    if (hipMemGetInfo(&free_bytes, &total_bytes) == hipSuccess)
+        budget = std::min<size_t>((size_t)16 << 30, std::max<size_t>(budget, free_bytes / 8));
     if (const char *env = getenv("TGK_CHUNK_BYTES"))
         budget = std::max<size_t>((size_t)strtoull(env, nullptr, 10), per_seed);
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(n_seeds, (int64_t)(budget / per_seed)));
     const int64_t n_batches = (n_seeds + batch_size - 1) / batch_size;
+    tracks->device = rt->device;
 
     DeviceBuffer d_seeds, d_rng, d_lengths, d_steps, d_hits, d_dense, d_offsets, d_points;
     TGK_HIP(d_seeds.allocate((size_t)chunk * 8 * sizeof(float)), "seed buffer");
@@ -786,12 +797,20 @@ int tgk_track(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field,
                                rt->stream),
                 "hits copy");
         TGK_HIP(hipStreamSynchronize(rt->stream), "tracking");
-        const size_t before = tracks->points.size();
-        tracks->points.resize(before + (size_t)total * 3);
-        TGK_HIP(hipMemcpy(tracks->points.data() + before, d_points.ptr, (size_t)total * 3 * sizeof(float),
-                          hipMemcpyDeviceToHost),
-                "points copy");
+        std::unique_ptr<DeviceBuffer> piece(new (std::nothrow) DeviceBuffer());
+        if (!piece)
+            return fail(TGK_ERR_NO_MEMORY, "tracks allocation");
+        const size_t piece_bytes = (size_t)total * 3 * sizeof(float);
+        TGK_HIP(piece->allocate(piece_bytes), "points allocation");
+        if (piece_bytes)
+            TGK_HIP(hipMemcpyAsync(piece->ptr, d_points.ptr, piece_bytes, hipMemcpyDeviceToDevice, rt->stream),
+                    "points copy");
+        tracks->pieces.push_back(std::move(piece));
+        tracks->piece_points.push_back(total);
+        tracks->total_points += total;
     }
+    // The working buffers are freed on return: finish the piece copies first.
+    TGK_HIP(hipStreamSynchronize(rt->stream), "points copy");
     *out = tracks.release();
     return TGK_OK;
 }
@@ -803,7 +822,7 @@ int tgk_tracks_size(const tgk_tracks *tracks, int64_t *n_seeds, int64_t *total_p
     if (n_seeds)
         *n_seeds = (int64_t)tracks->lengths.size();
     if (total_points)
-        *total_points = (int64_t)(tracks->points.size() / 3);
+        *total_points = tracks->total_points;
     return TGK_OK;
 }
 
@@ -818,13 +837,23 @@ int tgk_tracks_copy(const tgk_tracks *tracks, uint32_t *lengths, uint32_t *steps
         memcpy(steps, tracks->steps.data(), n * 4);
     if (hits && n)
         memcpy(hits, tracks->hits.data(), n * 4);
-    if (points && !tracks->points.empty())
-        memcpy(points, tracks->points.data(), tracks->points.size() * sizeof(float));
+    if (points && tracks->total_points > 0) {
+        TGK_HIP(hipSetDevice(tracks->device), "hipSetDevice");
+        float *dst = points;
+        for (size_t i = 0; i < tracks->pieces.size(); i++) {
+            const size_t bytes = (size_t)tracks->piece_points[i] * 3 * sizeof(float);
+            if (bytes)
+                TGK_HIP(hipMemcpy(dst, tracks->pieces[i]->ptr, bytes, hipMemcpyDeviceToHost), "points copy");
+            dst += (size_t)tracks->piece_points[i] * 3;
+        }
+    }
     return TGK_OK;
 }
 
 int tgk_tracks_destroy(tgk_tracks *tracks)
 {
+    if (tracks && !tracks->pieces.empty())
+        (void)hipSetDevice(tracks->device);
     delete tracks;
     return TGK_OK;
 }
