@@ -482,19 +482,13 @@ __device__ __forceinline__ V3 probabilistic(const AlgoParams &a, const float *q,
     return choose_candidate(a.n_samples, rng, candidate, weight, chosen) ? chosen : o;
 }
 
-// Gradient on the sphere: dtheta e_theta + (dphi / sin theta) e_phi; for
-// SH in Cartesian form (sh_cartesian), which needs no frame.
+// Gradient on the sphere: dtheta e_theta + (dphi / sin theta) e_phi, from
+// the angular derivatives. (Not the Cartesian form of sh_cartesian: its
+// normal part, about L f, cancels to rounding that the central differences
+// of the Hessian divide by 2h.)
 template <int MODEL, int LMAX>
 __device__ __forceinline__ V3 sphere_gradient(const float *q, V3 p)
 {
-    if constexpr (MODEL == MODEL_SH) {
-        const V3 n = unit_direction(p);
-        const Cartesian g = sh_cartesian<LMAX>(q, n);
-        float value, d;
-        softplus(g.value, value, d);
-        const float normal = g.gx * n.x + g.gy * n.y + g.gz * n.z;
-        return v3(d * (g.gx - normal * n.x), d * (g.gy - normal * n.y), d * (g.gz - normal * n.z));
-    }
     const Direction d = direction_of(p);
     const ModelOut m = model_at<MODEL, LMAX>(q, p, d);
     return v3(m.dtheta * (d.z * d.c1) - m.dphi * d.s1, m.dtheta * (d.z * d.s1) + m.dphi * d.c1, -m.dtheta * d.s);
