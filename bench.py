@@ -753,9 +753,10 @@ def seeds_leg(args, tg, torch, cfg, workload: str):
 
 def product_leg(args, tg, torch, cfg, workload: str):
     """Time one TOD or connectome pass over boundary seeds of the workload's
-    mask domain (n_seeds tracks of at most max_length / step steps) on the
-    plain FP32 path, and the FP32 host restatement (one core) on a bounded
-    sample of the same tracks. Prints one JSON line."""
+    mask domain (n_seeds tracks of at most max_length / step steps): the TOD
+    on the plain FP32 path, the connectome on each path of --paths; and the
+    FP32 host restatement (one core) on a bounded sample of the same tracks.
+    Prints one JSON line."""
     from tractography_amd import _native
     from tractography_amd.domain import _mesh_handle_for
 
@@ -773,14 +774,17 @@ def product_leg(args, tg, torch, cfg, workload: str):
     stream = torch.cuda.ExternalStream(runtime.stream(), device=dev)
     dims = np.ascontiguousarray(cfg["mask"].shape, dtype=np.uint32)
     inverse = np.ascontiguousarray(np.eye(4), dtype=np.float32)
+    path_ids = {"plain": _native.PATH_PLAIN, "fast": _native.PATH_FAST}
     if args.product == "tod":
+        paths = ["plain"]
         out = torch.empty(int(np.prod(dims)) * 45, dtype=torch.float32, device=dev)
 
-        def launch():
+        def launch(path):
             _native.check(lib.tgk_tod_device(runtime.handle, domain_handle.handle, domain_handle.handle,
                                              field_handle.handle, algorithm.descriptor(), rng_seed, n, n_steps,
                                              _native.ptr(dims), _native.ptr(inverse), 0, out.data_ptr()))
     else:
+        paths = [p for p in args.paths.split(",") if p in path_ids]
         boundary = _mesh_handle_for(domain, runtime)
         # The host entry point copies its 9 entries per seed back; the timed
         # region is the whole call.
@@ -788,42 +792,48 @@ def product_leg(args, tg, torch, cfg, workload: str):
         cols_h = np.empty(9 * n, np.int32)
         weights_h = np.empty(9 * n, np.float32)
 
-        def launch():
+        def launch(path):
             _native.check(lib.tgk_connectome(runtime.handle, domain_handle.handle, boundary.handle,
-                                             field_handle.handle, algorithm.descriptor(), rng_seed, n, n_steps,
-                                             0.0, _native.ptr(rows_h), _native.ptr(cols_h),
+                                             field_handle.handle, algorithm.descriptor(), path_ids[path], rng_seed,
+                                             n, n_steps, 0.0, _native.ptr(rows_h), _native.ptr(cols_h),
                                              _native.ptr(weights_h)))
-    for _ in range(args.warmup):
-        launch()
-    runtime.finish()
-    torch.cuda.synchronize()
-    times = []
-    for _ in range(args.steps):
-        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        start.record(stream)
-        t0 = time.perf_counter()
-        launch()
-        stop.record(stream)
+    results = {}
+    for path in paths:
+        for _ in range(args.warmup):
+            launch(path)
         runtime.finish()
         torch.cuda.synchronize()
-        times.append(((time.perf_counter() - t0) * 1e3, start.elapsed_time(stop)))
-    wall = float(np.mean([t[0] for t in times]))
-    device_ms = float(np.mean([t[1] for t in times]))
-    line = {"metric": f"{args.product} tracks/s on 1 MI355X (plain FP32 path)", "unit": "tracks/s",
-            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-            "data": "synthetic (seeded numpy generators)",
-            "config": {"workload": workload, "product": args.product, "boundary_seeds": n, "max_steps": n_steps}}
-    if args.product == "tod":
-        raw = out.view(-1, 45)[:, 0].to(torch.float64)
-        deposits = float(raw.sum().item()) / Y0_F32
-        line["value"] = n / (device_ms * 1e-3)
-        line["deposited_steps"] = deposits
-        line["steps_per_s"] = deposits / (device_ms * 1e-3)
-        line["kernel_ms"] = device_ms
-    else:
-        line["value"] = n / (wall * 1e-3)
-        line["entries_set"] = int((rows_h >= 0).sum())
+        times = []
+        for _ in range(args.steps):
+            start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record(stream)
+            t0 = time.perf_counter()
+            launch(path)
+            stop.record(stream)
+            runtime.finish()
+            torch.cuda.synchronize()
+            times.append(((time.perf_counter() - t0) * 1e3, start.elapsed_time(stop)))
+        wall = float(np.mean([t[0] for t in times]))
+        device_ms = float(np.mean([t[1] for t in times]))
+        results[path] = {"ms_per_step": wall, "kernel_ms": device_ms}
+        if args.product == "tod":
+            raw = out.view(-1, 45)[:, 0].to(torch.float64)
+            deposits = float(raw.sum().item()) / Y0_F32
+            results[path].update(tracks_per_s=n / (device_ms * 1e-3), deposited_steps=deposits,
+                                 steps_per_s=deposits / (device_ms * 1e-3))
+        else:
+            results[path].update(tracks_per_s=n / (wall * 1e-3), entries_set=int((rows_h >= 0).sum()))
+        print(f"[bench] {args.product} {path}: {wall:.1f} ms per call, {results[path]['tracks_per_s']:.3e} tracks/s",
+              file=sys.stderr, flush=True)
+    headline = "fast" if "fast" in results else "plain"
+    h = results[headline]
+    line = {"metric": f"{args.product} tracks/s on 1 MI355X ({headline} path)", "unit": "tracks/s",
+            "value": h["tracks_per_s"], "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": h["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic (seeded numpy generators)",
+            "config": {"workload": workload, "product": args.product, "path": headline, "boundary_seeds": n,
+                       "max_steps": n_steps},
+            "paths": results}
     if not args.no_cpu_baseline:
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle  # noqa: E402  (the host restatement)

@@ -185,10 +185,12 @@ int tgk_field_set_fast_precision(tgk_field *field, int32_t precision);
  * domain's boundary surface, with cumulative areas and BVH); seed g uses
  * rng_init(rng_seed, g) for its boundary sample and its noise. rows, cols,
  * weights (n_seeds * 9): boundary-vertex pairs and barycentric weight
- * products, row -1 where a track contributes nothing. */
+ * products, row -1 where a track contributes nothing. `path` as for
+ * tgk_track: TGK_PATH_PLAIN is the reference's FP32 arithmetic,
+ * TGK_PATH_FAST the polynomial model (SH up to 91 coefficients). */
 int tgk_connectome(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
-                   const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
-                   float minimum_length, int32_t *rows, int32_t *cols, float *weights);
+                   const tgk_algorithm *algorithm, int32_t path, uint32_t rng_seed, int64_t n_seeds,
+                   uint32_t n_steps, float minimum_length, int32_t *rows, int32_t *cols, float *weights);
 
 /* Replaces the kernel of tracking.tod (tracking.py:181-385): n_seeds
  * boundary-seeded tracks of at most n_steps steps in the mask `domain`,

@@ -1082,14 +1082,16 @@ int tgk_math(tgk_runtime *rt, int32_t fn, const float *x, const float *y, int64_
 }
 
 int tgk_connectome(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
-                   const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
-                   float minimum_length, int32_t *rows, int32_t *cols, float *weights)
+                   const tgk_algorithm *algorithm, int32_t path, uint32_t rng_seed, int64_t n_seeds,
+                   uint32_t n_steps, float minimum_length, int32_t *rows, int32_t *cols, float *weights)
 {
     if (!rt || !domain || !boundary || !field || n_seeds < 0 || (n_seeds > 0 && (!rows || !cols || !weights)))
         return fail(TGK_ERR_INVALID, "bad arguments");
+    if (path != TGK_PATH_PLAIN && path != TGK_PATH_FAST)
+        return fail(TGK_ERR_INVALID, "unknown path %d", path);
     if (n_seeds > MAX_SEEDS)
         return fail(TGK_ERR_INVALID, "%lld seeds: at most %lld per call", (long long)n_seeds, (long long)MAX_SEEDS);
-    if (int r = check_algorithm(algorithm, field))
+    if (int r = check_algorithm(algorithm, field, path == TGK_PATH_FAST ? 91 : 45))
         return r;
     if (domain->kind != tgk::DOMAIN_MASK)
         return fail(TGK_ERR_INVALID, "the connectome's domain must be a mask domain");
@@ -1129,7 +1131,19 @@ int tgk_connectome(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *
     c.cols = dcols;
     c.weights = dweights;
     c.queue = rt->queue.as<uint32_t>();
-    TGK_HIP(tgk::launch_connectome(c, rt->stream), "connectome kernel");
+    if (path == TGK_PATH_PLAIN) {
+        TGK_HIP(tgk::launch_connectome(c, rt->stream), "connectome kernel");
+    } else {
+        tgk::FastConnectomeLaunch f;
+        int32_t sh_order = 8;
+        if (int r = ensure_fast_field(rt, const_cast<tgk_field *>(field), algorithm, &sh_order))
+            return r;
+        f.connectome = c;
+        f.coeffs = field->fast.ptr;
+        f.coeff_stride = field->fast_stride;
+        f.sh_order = sh_order;
+        TGK_HIP(tgk::launch_connectome_fast(f, rt->stream), "fast connectome kernel");
+    }
     if (int r = fetch(rows, drows, (size_t)n_seeds * 9, rt->stream))
         return r;
     if (int r = fetch(cols, dcols, (size_t)n_seeds * 9, rt->stream))

@@ -12,6 +12,8 @@
 //   where the reference calls exp/log/cos, FP64 arc length; the four
 //   orientation updates (Transport, Diffusion, Probabilistic, Deterministic).
 // * Seed-major dense output rows, compacted into ragged points afterwards.
+// * The boundary connectome's tracking loop on the same machinery
+//   (connectome_fast_kernel); its hits pass is the plain path's.
 //
 // Compiled with -ffp-contract=fast.
 #include <hip/hip_runtime.h>
@@ -81,13 +83,86 @@ template <int LMAX> constexpr int fast_block() { return 256; }
 // coefficients stay in registers.
 template <int MODEL, int LMAX> constexpr int lds_vectors() { return rows_in_lds<MODEL, LMAX>() ? 10 : 0; }
 
-template <int MODEL, int ALGO, int DOMAIN, int LMAX>
-__device__ __forceinline__ void advance(const TrackLaunch &p, const float *__restrict__ coeffs, uint32_t cstride,
-                                        Lane &L, float *q, float4 *__restrict__ lds_rows)
+// A cell's row into the lane's cache on a cell change: order-8 SH rows to
+// LDS (the last TERMS - 4 LDS_VEC coefficients to registers), others to
+// registers.
+template <int MODEL, int LMAX>
+__device__ __forceinline__ void load_row(const float *__restrict__ coeffs, int64_t cell, uint32_t cstride, float *q,
+                                         float4 *__restrict__ lds_rows)
 {
     constexpr int TERMS = Order<LMAX>::TERMS;
     constexpr int VEC = Order<LMAX>::STRIDE / 4;
     constexpr int LDS_VEC = lds_vectors<MODEL, LMAX>();
+    const float4 *row = reinterpret_cast<const float4 *>(coeffs + cell * (int64_t)cstride);
+    if constexpr (MODEL == MODEL_SH) {
+#pragma unroll
+        for (int j = 0; j < LDS_VEC; j++)
+            lds_rows[j * fast_block<LMAX>() + threadIdx.x] = row[j];
+#pragma unroll
+        for (int j = LDS_VEC; j < VEC; j++) {
+            const float4 v = row[j];
+            if (4 * j < TERMS)
+                q[4 * j] = v.x;
+            if (4 * j + 1 < TERMS)
+                q[4 * j + 1] = v.y;
+            if (4 * j + 2 < TERMS)
+                q[4 * j + 2] = v.z;
+            if (4 * j + 3 < TERMS)
+                q[4 * j + 3] = v.w;
+        }
+    } else {
+        const float4 a = row[0], b = row[1];
+        q[0] = a.x;
+        q[1] = a.y;
+        q[2] = a.z;
+        q[3] = a.w;
+        q[4] = b.x;
+        q[5] = b.y;
+    }
+}
+
+// The LDS part of the cached row back into q, every step.
+template <int MODEL, int LMAX>
+__device__ __forceinline__ void read_row(float *q, const float4 *__restrict__ lds_rows)
+{
+    constexpr int TERMS = Order<LMAX>::TERMS;
+    constexpr int LDS_VEC = lds_vectors<MODEL, LMAX>();
+    if constexpr (rows_in_lds<MODEL, LMAX>()) {
+#pragma unroll
+        for (int j = 0; j < LDS_VEC; j++) {
+            const float4 v = lds_rows[j * fast_block<LMAX>() + threadIdx.x];
+            if (4 * j < TERMS)
+                q[4 * j] = v.x;
+            if (4 * j + 1 < TERMS)
+                q[4 * j + 1] = v.y;
+            if (4 * j + 2 < TERMS)
+                q[4 * j + 2] = v.z;
+            if (4 * j + 3 < TERMS)
+                q[4 * j + 3] = v.w;
+        }
+    }
+}
+
+// One orientation update on the fast model.
+template <int MODEL, int ALGO, int LMAX>
+__device__ __forceinline__ V3 update_orientation(const AlgoParams &a, const float *q, V3 o, uint2 &rng)
+{
+    if constexpr (ALGO == ALGO_PROBABILISTIC) {
+        return fast::probabilistic<MODEL, LMAX>(a, q, o, rng);
+    } else if constexpr (ALGO == ALGO_DETERMINISTIC) {
+        return fast::deterministic<MODEL, LMAX>(a, q, o);
+    } else if constexpr (MODEL == MODEL_SH) {
+        return fast::sh_update<ALGO, LMAX>(a, q, o, rng);
+    } else {
+        const fast::Direction dir = fast::direction_of(o);
+        return fast::update<ALGO>(a, fast::dti_model(q, o, dir), dir, o, rng);
+    }
+}
+
+template <int MODEL, int ALGO, int DOMAIN, int LMAX>
+__device__ __forceinline__ void advance(const TrackLaunch &p, const float *__restrict__ coeffs, uint32_t cstride,
+                                        Lane &L, float *q, float4 *__restrict__ lds_rows)
+{
     bool stop = !(L.n < p.n_points && L.steps < p.max_steps);
     uint32_t hit = NO_HIT;
     if (!stop) {
@@ -133,57 +208,10 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
     }
     if (cell != L.field_cell) {
         L.field_cell = cell;
-        const float4 *row = reinterpret_cast<const float4 *>(coeffs + cell * (int64_t)cstride);
-        if constexpr (MODEL == MODEL_SH) {
-#pragma unroll
-            for (int j = 0; j < LDS_VEC; j++)
-                lds_rows[j * fast_block<LMAX>() + threadIdx.x] = row[j];
-#pragma unroll
-            for (int j = LDS_VEC; j < VEC; j++) {
-                const float4 v = row[j];
-                if (4 * j < TERMS)
-                    q[4 * j] = v.x;
-                if (4 * j + 1 < TERMS)
-                    q[4 * j + 1] = v.y;
-                if (4 * j + 2 < TERMS)
-                    q[4 * j + 2] = v.z;
-                if (4 * j + 3 < TERMS)
-                    q[4 * j + 3] = v.w;
-            }
-        } else {
-            const float4 a = row[0], b = row[1];
-            q[0] = a.x;
-            q[1] = a.y;
-            q[2] = a.z;
-            q[3] = a.w;
-            q[4] = b.x;
-            q[5] = b.y;
-        }
+        load_row<MODEL, LMAX>(coeffs, cell, cstride, q, lds_rows);
     }
-    if constexpr (rows_in_lds<MODEL, LMAX>()) {
-#pragma unroll
-        for (int j = 0; j < LDS_VEC; j++) {
-            const float4 v = lds_rows[j * fast_block<LMAX>() + threadIdx.x];
-            if (4 * j < TERMS)
-                q[4 * j] = v.x;
-            if (4 * j + 1 < TERMS)
-                q[4 * j + 1] = v.y;
-            if (4 * j + 2 < TERMS)
-                q[4 * j + 2] = v.z;
-            if (4 * j + 3 < TERMS)
-                q[4 * j + 3] = v.w;
-        }
-    }
-    if constexpr (ALGO == ALGO_PROBABILISTIC) {
-        L.o = fast::probabilistic<MODEL, LMAX>(p.algo, q, L.o, L.rng);
-    } else if constexpr (ALGO == ALGO_DETERMINISTIC) {
-        L.o = fast::deterministic<MODEL, LMAX>(p.algo, q, L.o);
-    } else if constexpr (MODEL == MODEL_SH) {
-        L.o = fast::sh_update<ALGO, LMAX>(p.algo, q, L.o, L.rng);
-    } else {
-        const fast::Direction dir = fast::direction_of(L.o);
-        L.o = fast::update<ALGO>(p.algo, fast::dti_model(q, L.o, dir), dir, L.o, L.rng);
-    }
+    read_row<MODEL, LMAX>(q, lds_rows);
+    L.o = update_orientation<MODEL, ALGO, LMAX>(p.algo, q, L.o, L.rng);
     L.steps++;
     L.previous = L.point;
     const float step = p.algo.step_size;
@@ -251,6 +279,117 @@ __global__ __launch_bounds__(fast_block<LMAX>()) void track_fast_kernel(FastLaun
             break;
         if (L.g >= 0)
             advance<MODEL, ALGO, DOMAIN, LMAX>(p, coeffs, cstride, L, q, lds_rows);
+    }
+}
+
+// tracking.py:457-558 (kernel connectome) on the fast model: the plain
+// connectome kernel's loop (mask, field, update, finite and length test,
+// move) with the tracking kernel's persistent lanes, cell cache and
+// polynomial model. A finished track leaves the same record in its seed's
+// output slots as on the plain path (last segment, start barycentrics,
+// start triangle, length test) for the shared hits kernel.
+template <int MODEL, int ALGO, int LMAX>
+__global__ __launch_bounds__(fast_block<LMAX>()) void connectome_fast_kernel(FastConnectomeLaunch f)
+{
+    __shared__ float4 lds_rows[rows_in_lds<MODEL, LMAX>() ? lds_vectors<MODEL, LMAX>() * fast_block<LMAX>() : 1];
+    const ConnectomeLaunch &c = f.connectome;
+    const float *coeffs = static_cast<const float *>(f.coeffs);
+    const uint32_t cstride = f.coeff_stride;
+    const float step = c.algo.step_size;
+    const int lane_id = __lane_id();
+
+    int64_t g = -1;
+    bool drained = false;
+    uint2 rng = make_uint2(0u, 0u);
+    V3 point = v3(0.0f, 0.0f, 0.0f), o = point, previous = point;
+    uint32_t tracked = 0, start_triangle = 0;
+    float b0 = 0.0f, b1 = 0.0f, b2 = 0.0f;
+    int64_t field_cell = -2, cached_mask_cell = -2;
+    bool mask_inside = false;
+    float q[MODEL == MODEL_SH ? Order<LMAX>::TERMS : 6];
+
+    while (true) {
+        // Refill idle lanes from the queue, one atomic per wavefront.
+        const uint64_t need = __ballot(g < 0);
+        if (need != 0 && !drained) {
+            const int leader = __ffsll((unsigned long long)need) - 1;
+            const uint32_t count = (uint32_t)__popcll(need);
+            uint32_t base = 0;
+            if (lane_id == leader)
+                base = atomicAdd(c.queue, count);
+            base = __shfl(base, leader);
+            if ((uint64_t)base + count >= (uint64_t)c.n_seeds)
+                drained = true;
+            if (g < 0) {
+                const uint64_t claimed = (uint64_t)base + (uint64_t)__popcll(need & ((1ull << lane_id) - 1ull));
+                if (claimed < (uint64_t)c.n_seeds) {
+                    g = (int64_t)claimed;
+                    rng = rng_init(c.rng_seed, (uint32_t)g);
+                    const BoundarySample sample = sample_boundary(c.boundary.vertices, c.boundary.triangles,
+                                                                  c.cumulative, c.n_triangles, rng);
+                    point = sample.point;
+                    o = sample.orientation;
+                    previous = point;
+                    tracked = 0;
+                    start_triangle = sample.triangle;
+                    b0 = sample.bary[0];
+                    b1 = sample.bary[1];
+                    b2 = sample.bary[2];
+                    field_cell = -2;
+                    cached_mask_cell = -2;
+                    mask_inside = false;
+                }
+            }
+        }
+        if (__ballot(g >= 0) == 0)
+            break;
+        if (g < 0)
+            continue;
+        bool stop = tracked >= c.n_steps;
+        if (!stop) {
+            const int64_t cell = mask_cell(c.mask, point);
+            if (cell != cached_mask_cell) {
+                cached_mask_cell = cell;
+                mask_inside = cell >= 0 && c.mask.mask[cell] != 0;
+            }
+            stop = !mask_inside;
+        }
+        int64_t cell = -1;
+        if (!stop) {
+            cell = field_voxel(c.field, point);
+            stop = cell < 0;
+        }
+        if (!stop) {
+            if (cell != field_cell) {
+                field_cell = cell;
+                load_row<MODEL, LMAX>(coeffs, cell, cstride, q, lds_rows);
+            }
+            read_row<MODEL, LMAX>(q, lds_rows);
+            o = update_orientation<MODEL, ALGO, LMAX>(c.algo, q, o, rng);
+            stop = !(__builtin_isfinite(o.x) && __builtin_isfinite(o.y) && __builtin_isfinite(o.z)) ||
+                   length3(o) < 0.5f;
+        }
+        if (!stop) {
+            previous = point;
+            point.x += step * o.x;
+            point.y += step * o.y;
+            point.z += step * o.z;
+            tracked++;
+            continue;
+        }
+        float *record = c.weights + 9 * g;
+        record[0] = previous.x;
+        record[1] = previous.y;
+        record[2] = previous.z;
+        record[3] = point.x;
+        record[4] = point.y;
+        record[5] = point.z;
+        record[6] = b0;
+        record[7] = b1;
+        record[8] = b2;
+        c.rows[9 * g] = (int32_t)start_triangle;
+        c.rows[9 * g + 1] = (float)tracked * step < c.min_length ? 0 : 1;
+        g = -1;
     }
 }
 
@@ -459,6 +598,62 @@ hipError_t dispatch_fast(const FastLaunch &f, int grid, bool query, hipStream_t 
     return launch_fast_algo<MODEL_SH>(f, grid, query, stream);
 }
 
+// Connectome launches: the persistent grid of the kernel's occupancy
+// (cached per device), at most one lane per seed, then the hits pass.
+template <int MODEL, int ALGO, int LMAX> hipError_t launch_connectome_fast_kernel(const FastConnectomeLaunch &f,
+                                                                                  hipStream_t stream)
+{
+    static std::atomic<int> cache[64] = {};
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess)
+        return e;
+    if (device < 0 || device >= 64)
+        return hipErrorInvalidDevice;
+    if (cache[device].load(std::memory_order_acquire) == 0) {
+        int per_cu = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(connectome_fast_kernel<MODEL, ALGO, LMAX>), fast_block<LMAX>(),
+            0);
+        if (e != hipSuccess)
+            return e;
+        int cus = 0;
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+        if (e != hipSuccess)
+            return e;
+        cache[device].store((cus > 0 ? cus : 1) * (per_cu > 0 ? per_cu : 1), std::memory_order_release);
+    }
+    const int64_t fill = cache[device].load(std::memory_order_acquire);
+    const int64_t needed = (f.connectome.n_seeds + fast_block<LMAX>() - 1) / fast_block<LMAX>();
+    zero_counter<<<1, 1, 0, stream>>>(f.connectome.queue);
+    connectome_fast_kernel<MODEL, ALGO, LMAX><<<(unsigned)std::min(fill, needed), fast_block<LMAX>(), 0, stream>>>(f);
+    e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    return launch_connectome_hits(f.connectome, stream);
+}
+
+template <int MODEL, int ALGO>
+hipError_t launch_connectome_fast_order(const FastConnectomeLaunch &f, hipStream_t stream)
+{
+    if constexpr (MODEL == MODEL_SH) {
+        if (f.sh_order == 12)
+            return launch_connectome_fast_kernel<MODEL, ALGO, 12>(f, stream);
+    }
+    return launch_connectome_fast_kernel<MODEL, ALGO, 8>(f, stream);
+}
+
+template <int MODEL>
+hipError_t launch_connectome_fast_algo(const FastConnectomeLaunch &f, hipStream_t stream)
+{
+    switch (f.connectome.algo.algorithm) {
+    case ALGO_DIFFUSION: return launch_connectome_fast_order<MODEL, ALGO_DIFFUSION>(f, stream);
+    case ALGO_PROBABILISTIC: return launch_connectome_fast_order<MODEL, ALGO_PROBABILISTIC>(f, stream);
+    case ALGO_DETERMINISTIC: return launch_connectome_fast_order<MODEL, ALGO_DETERMINISTIC>(f, stream);
+    default: return launch_connectome_fast_order<MODEL, ALGO_TRANSPORT>(f, stream);
+    }
+}
+
 }  // namespace
 
 hipError_t launch_track_fast(const FastLaunch &f, int grid_blocks, hipStream_t stream)
@@ -469,6 +664,15 @@ hipError_t launch_track_fast(const FastLaunch &f, int grid_blocks, hipStream_t s
 }
 
 hipError_t prepare_track_fast(const FastLaunch &f) { return dispatch_fast(f, 0, true, nullptr); }
+
+hipError_t launch_connectome_fast(const FastConnectomeLaunch &f, hipStream_t stream)
+{
+    if (f.connectome.n_seeds == 0)
+        return hipSuccess;
+    if (f.connectome.algo.model == MODEL_DTI)
+        return launch_connectome_fast_algo<MODEL_DTI>(f, stream);
+    return launch_connectome_fast_algo<MODEL_SH>(f, stream);
+}
 
 hipError_t launch_compact_seed_major(const float *dense, int64_t n_seeds, uint32_t n_points,
                                      const uint32_t *lengths, const int64_t *offsets, float *out,

@@ -100,7 +100,11 @@ __device__ __forceinline__ Direction direction_of(V3 o)
         d.c1 = o.x * inv_rho;
         d.s1 = o.y * inv_rho;
     } else {
-        d.c1 = 1.0f;
+        // On the axis the azimuth is atan2(y, x) of signed zeros: 0 for
+        // x = +0, +-pi for x = -0 (axis-aligned orientations, such as the
+        // inward normals of boundary seeds, carry such zeros); 0 for the
+        // zero vector.
+        d.c1 = r2 > 0.0f && __builtin_signbit(o.x) ? -1.0f : 1.0f;
         d.s1 = 0.0f;
     }
     return d;
@@ -347,7 +351,8 @@ __device__ __forceinline__ V3 sh_update(const AlgoParams &a, const float *q, V3 
     if constexpr (ALGO == ALGO_DIFFUSION) {
         if (a.noise_variance > 0.0f) {
             const float rho2 = n.x * n.x + n.y * n.y;
-            float c1 = 1.0f, s1 = 0.0f, sin_theta = 0.0f;
+            // On the axis, the azimuth of signed zeros (see direction_of).
+            float c1 = __builtin_signbit(n.x) ? -1.0f : 1.0f, s1 = 0.0f, sin_theta = 0.0f;
             if (rho2 > 0.0f) {
                 const float inv_rho = __builtin_amdgcn_rsqf(rho2);
                 c1 = n.x * inv_rho;

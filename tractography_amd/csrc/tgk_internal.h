@@ -34,6 +34,18 @@ struct ConnectomeLaunch {
     uint32_t *queue;         // seed counter of the persistent lanes (zeroed by the launch)
 };
 hipError_t launch_connectome(const ConnectomeLaunch &c, hipStream_t stream);
+// The last-segment hits and the 9 entries per seed from the records a
+// connectome tracking kernel left (tgk_plain.hip).
+hipError_t launch_connectome_hits(const ConnectomeLaunch &c, hipStream_t stream);
+// The connectome on the fast path (tgk_fast.hip): `coeffs` as in
+// FastLaunch; the hits pass is launch_connectome_hits.
+struct FastConnectomeLaunch {
+    ConnectomeLaunch connectome;
+    const void *coeffs;
+    uint32_t coeff_stride;
+    int32_t sh_order;
+};
+hipError_t launch_connectome_fast(const FastConnectomeLaunch &f, hipStream_t stream);
 
 // Track orientation distribution (tgk_plain.hip): boundary-seeded tracks
 // deposit the SH basis of their orientation on an output grid.

@@ -807,8 +807,7 @@ hipError_t launch_connectome_kt(const ConnectomeLaunch &c, hipStream_t stream)
                              : launch_persistent(connectome_kernel<MODEL, ALGO, 0>, c, c.queue, c.n_seeds, stream);
     if (e != hipSuccess)
         return e;
-    connectome_hits_kernel<<<blocks_for(c.n_seeds, 256), 256, 0, stream>>>(c);
-    return hipGetLastError();
+    return launch_connectome_hits(c, stream);
 }
 
 template <int MODEL>
@@ -820,6 +819,14 @@ hipError_t launch_connectome_model(const ConnectomeLaunch &c, hipStream_t stream
     case ALGO_DETERMINISTIC: return launch_connectome_kt<MODEL, ALGO_DETERMINISTIC>(c, stream);
     default: return launch_connectome_kt<MODEL, ALGO_TRANSPORT>(c, stream);
     }
+}
+
+hipError_t launch_connectome_hits(const ConnectomeLaunch &c, hipStream_t stream)
+{
+    if (c.n_seeds == 0)
+        return hipSuccess;
+    connectome_hits_kernel<<<blocks_for(c.n_seeds, 256), 256, 0, stream>>>(c);
+    return hipGetLastError();
 }
 
 hipError_t launch_connectome(const ConnectomeLaunch &c, hipStream_t stream)

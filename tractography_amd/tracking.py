@@ -274,7 +274,7 @@ def connectome(
     _native.check(
         _native.lib().tgk_connectome(
             runtime.handle, domain_handle.handle, boundary_handle.handle, field_handle.handle,
-            algorithm.descriptor(), int(algorithm.get_rng_seed()), int(n_seeds), n_steps,
+            algorithm.descriptor(), runtime.path_id, int(algorithm.get_rng_seed()), int(n_seeds), n_steps,
             float(np.float32(minimum_streamline_length)), _native.ptr(rows), _native.ptr(cols), _native.ptr(weights),
         )
     )
