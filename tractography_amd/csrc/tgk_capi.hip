@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <new>
 #include <string>
 #include <deque>
 #include <vector>
@@ -127,9 +128,14 @@ struct tgk_tracks {
     int device = 0;
     std::vector<uint32_t> lengths, steps, hits;
     // The ragged points of each chunk, left in device memory (exactly sized)
-    // until tgk_tracks_copy moves them once into the caller's array.
-    std::vector<std::unique_ptr<DeviceBuffer>> pieces;
-    std::vector<int64_t> piece_points;
+    // until tgk_tracks_copy moves them once into the caller's array; in host
+    // memory instead when the device has no room left for them.
+    struct Piece {
+        std::unique_ptr<DeviceBuffer> device;
+        std::vector<float> host;
+        int64_t n_points = 0;
+    };
+    std::vector<Piece> pieces;
     int64_t total_points = 0;
 };
 
@@ -797,16 +803,31 @@ int tgk_track(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field,
                                rt->stream),
                 "hits copy");
         TGK_HIP(hipStreamSynchronize(rt->stream), "tracking");
-        std::unique_ptr<DeviceBuffer> piece(new (std::nothrow) DeviceBuffer());
-        if (!piece)
-            return fail(TGK_ERR_NO_MEMORY, "tracks allocation");
+        tgk_tracks::Piece piece;
+        piece.n_points = total;
         const size_t piece_bytes = (size_t)total * 3 * sizeof(float);
-        TGK_HIP(piece->allocate(piece_bytes), "points allocation");
-        if (piece_bytes)
-            TGK_HIP(hipMemcpyAsync(piece->ptr, d_points.ptr, piece_bytes, hipMemcpyDeviceToDevice, rt->stream),
-                    "points copy");
+        piece.device.reset(new (std::nothrow) DeviceBuffer());
+        if (!piece.device)
+            return fail(TGK_ERR_NO_MEMORY, "tracks allocation");
+        const hipError_t e = piece.device->allocate(piece_bytes);
+        if (e == hipSuccess) {
+            if (piece_bytes)
+                TGK_HIP(hipMemcpyAsync(piece.device->ptr, d_points.ptr, piece_bytes, hipMemcpyDeviceToDevice,
+                                       rt->stream),
+                        "points copy");
+        } else if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();  // clear the failed allocation's error
+            piece.device.reset();
+            try {
+                piece.host.resize((size_t)total * 3);
+            } catch (const std::bad_alloc &) {
+                return fail(TGK_ERR_NO_MEMORY, "points: no room in device or host memory");
+            }
+            TGK_HIP(hipMemcpy(piece.host.data(), d_points.ptr, piece_bytes, hipMemcpyDeviceToHost), "points copy");
+        } else {
+            return hip_fail(e, "points allocation");
+        }
         tracks->pieces.push_back(std::move(piece));
-        tracks->piece_points.push_back(total);
         tracks->total_points += total;
     }
     // The working buffers are freed on return: finish the piece copies first.
@@ -840,11 +861,13 @@ int tgk_tracks_copy(const tgk_tracks *tracks, uint32_t *lengths, uint32_t *steps
     if (points && tracks->total_points > 0) {
         TGK_HIP(hipSetDevice(tracks->device), "hipSetDevice");
         float *dst = points;
-        for (size_t i = 0; i < tracks->pieces.size(); i++) {
-            const size_t bytes = (size_t)tracks->piece_points[i] * 3 * sizeof(float);
-            if (bytes)
-                TGK_HIP(hipMemcpy(dst, tracks->pieces[i]->ptr, bytes, hipMemcpyDeviceToHost), "points copy");
-            dst += (size_t)tracks->piece_points[i] * 3;
+        for (const tgk_tracks::Piece &piece : tracks->pieces) {
+            const size_t bytes = (size_t)piece.n_points * 3 * sizeof(float);
+            if (bytes && piece.device)
+                TGK_HIP(hipMemcpy(dst, piece.device->ptr, bytes, hipMemcpyDeviceToHost), "points copy");
+            else if (bytes)
+                memcpy(dst, piece.host.data(), bytes);
+            dst += (size_t)piece.n_points * 3;
         }
     }
     return TGK_OK;
