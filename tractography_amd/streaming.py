@@ -38,6 +38,9 @@ from .algorithm import BaseAlgorithm
 from .runtime import HipRuntime
 
 _TCK_COUNT_WIDTH = 10
+_TCK_SEPARATOR = np.full(3, np.nan, dtype="<f4").tobytes()
+# Buffers per writev call (below the usual IOV_MAX of 1024).
+_IOV_BATCH = 1024
 
 
 class TckWriter:
@@ -58,9 +61,28 @@ class TckWriter:
             offset = len(str(offset)) + len(body) + len("file: . \nEND\n")
         text = body + f"file: . {offset}\nEND\n"
         self._count_at = text.index("count: ") + len("count: ")
-        self._file = open(self.path, "wb")
-        self._file.write(text.encode("ascii"))
+        # Unbuffered: the data goes out with os.writev, straight from the
+        # point array.
+        self._file = open(self.path, "wb", buffering=0)
+        self._write_all([text.encode("ascii")])
         assert self._file.tell() == offset
+
+    def _write_all(self, buffers) -> None:
+        """Writes the buffers in order, gathering them into one system call
+        per _IOV_BATCH (no copy of the point data)."""
+        fd = self._file.fileno()
+        for start in range(0, len(buffers), _IOV_BATCH):
+            views = [memoryview(b).cast("B") for b in buffers[start : start + _IOV_BATCH]]
+            written = os.writev(fd, views) if hasattr(os, "writev") else 0
+            # A short write (or no writev): finish the rest one buffer at a time.
+            for view in views:
+                if written >= len(view):
+                    written -= len(view)
+                    continue
+                rest = view[written:]
+                written = 0
+                while len(rest):
+                    rest = rest[os.write(fd, rest) :]
 
     def append(self, points: np.ndarray, lengths: np.ndarray) -> None:
         """Streamlines given as ragged points (sum(lengths), 3) and lengths."""
@@ -70,23 +92,25 @@ class TckWriter:
         points = np.ascontiguousarray(points, dtype="<f4").reshape(-1, 3)
         if points.shape[0] != int(lengths.sum()):
             raise ValueError("points and lengths disagree")
-        # Each streamline followed by a NaN triplet, in one buffer.
-        out = np.empty((points.shape[0] + len(lengths), 3), dtype="<f4")
+        # Each streamline's rows, then a NaN triplet: views of the point
+        # array interleaved with one separator buffer.
         ends = np.cumsum(lengths)
-        rows = np.arange(points.shape[0]) + np.repeat(np.arange(len(lengths)), lengths)
-        out[rows] = points
-        out[ends + np.arange(len(lengths))] = np.nan
-        self._file.write(out.tobytes())
+        starts = ends - lengths
+        buffers = []
+        for a, b in zip(starts.tolist(), ends.tolist()):
+            buffers.append(points[a:b])
+            buffers.append(_TCK_SEPARATOR)
+        self._write_all(buffers)
         self.count += len(lengths)
 
     def close(self) -> None:
         if self._file is None:
             return
-        self._file.write(np.full(3, np.inf, dtype="<f4").tobytes())
+        self._write_all([np.full(3, np.inf, dtype="<f4").tobytes()])
         if self.count >= 10**_TCK_COUNT_WIDTH:
             raise ValueError("too many streamlines for the count field")
         self._file.seek(self._count_at)
-        self._file.write(f"{self.count:0{_TCK_COUNT_WIDTH}d}".encode("ascii"))
+        self._write_all([f"{self.count:0{_TCK_COUNT_WIDTH}d}".encode("ascii")])
         self._file.close()
         self._file = None
 
