@@ -682,6 +682,7 @@ def host_leg(cfg, gpu_results: dict, seconds: float = 15.0, n_accuracy: int = 40
     for name, (lengths, steps_, points) in gpu_results.items():
         rep = deviation.report(lengths[:n_acc], steps_[:n_acc], points, ref64)
         accuracy[name] = {"max_point_deviation_voxel": rep["max_deviation"],
+                          "max_angular_error_rad": rep["max_angular_error_rad"],
                           "compared_fraction": rep["compared_fraction"],
                           "length_equal_fraction": rep["length_equal_fraction"],
                           "seeds": n_acc}

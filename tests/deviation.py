@@ -5,8 +5,9 @@ diverge by O(step) once one of them crosses a voxel face the other does not.
 Saved points are therefore compared only while the running maximum of the
 point deviation stays below the FP64 trajectory's running minimum distance
 to a voxel face (the prefix over which both provably used the same voxels);
-the report gives the maximum deviation over those prefixes and how much of
-the streamlines they cover.
+the report gives the maximum deviation over those prefixes, the largest
+angle between corresponding saved segments of the two paths there, and how
+much of the streamlines they cover.
 """
 
 from __future__ import annotations
@@ -23,6 +24,7 @@ def report(lengths, steps, points, ref64, jump: float | None = None) -> dict:
     lengths = np.asarray(lengths, dtype=np.int64)
     offsets = np.concatenate([[0], np.cumsum(lengths)])
     max_dev = 0.0
+    max_angle = 0.0
     compared = 0
     total = 0
     full = 0
@@ -40,11 +42,21 @@ def report(lengths, steps, points, ref64, jump: float | None = None) -> dict:
         k = n if np.all(ok) else int(np.argmin(ok))
         if k > 0:
             max_dev = max(max_dev, float(running[k - 1]))
+        if k > 1:
+            # Angle between the saved segments of both paths over the prefix.
+            a = np.diff(got[:k], axis=0)
+            b = np.diff(ref[:k], axis=0)
+            norms = np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1)
+            live = norms > 0
+            if np.any(live):
+                cos = np.einsum("ij,ij->i", a[live], b[live]) / norms[live]
+                max_angle = max(max_angle, float(np.max(np.arccos(np.clip(cos, -1.0, 1.0)))))
         compared += k
         if k == n and int(lengths[i]) == n_ref:
             full += 1
     return {
         "max_deviation": max_dev,
+        "max_angular_error_rad": max_angle,
         "compared_fraction": compared / max(total, 1),
         "identical_length_and_voxels_fraction": full / max(len(lengths), 1),
         "length_equal_fraction": float(np.mean(lengths == ref64["lengths"])),
