@@ -122,6 +122,35 @@ int tgk_mesh_domain_create(tgk_runtime *rt, const float *vertices, uint32_t n_ve
                            const uint8_t *near, const uint32_t near_dims[3], const float near_inverse_affine[16],
                            float reach, tgk_domain **out);
 
+/* BVH over triangle bounding boxes (domain.py:72-116), built on the host:
+ * leaves hold <= leaf_size triangles; an inner node splits on the longest
+ * axis of its box (first axis on ties) at the median of a stable centroid
+ * sort; nodes are numbered in pre-order, left first. tgk_bvh_node_count
+ * gives N for n_triangles (it depends on the count alone). Inputs are the
+ * per-triangle box corners (n_triangles, 3) f64; outputs as taken by
+ * tgk_mesh_domain_create: bounds (N, 4) f32 (w = 0), nodes (N, 4) i32
+ * (left, right, first, count; left = right = -1 for leaves), ordered
+ * triangle slots (n_triangles,) u32. Host only: no device is touched. */
+int tgk_bvh_node_count(uint32_t n_triangles, uint32_t leaf_size, uint32_t *n_nodes);
+int tgk_bvh_build(const double *triangle_min, const double *triangle_max, uint32_t n_triangles, uint32_t leaf_size,
+                  float *bounds_min, float *bounds_max, int32_t *nodes, uint32_t n_nodes, uint32_t *ordered);
+
+/* Per-cell candidate lists of a mesh domain, built on the host for
+ * tgk_domain_set_segment_grid (no counterpart in the reference, whose
+ * tracking always traverses the BVH, bvh.cl:83-131). Triangle slot t is a
+ * candidate of every cell of its inclusive box lo[t]..hi[t] (n_triangles, 3)
+ * i64 on a grid of dims cells, numbered (x * dims[1] + y) * dims[2] + z.
+ * _count fills cell_start (n_cells + 1) u32 CSR offsets and the entry count;
+ * _fill writes entries (n_entries, 12) f32 rows (a.xyz, slot bits,
+ * (b - a).xyz, 0, (c - a).xyz, 0), in BVH leaf order (leaf_order, the BVH's
+ * ordered slots) within each cell. vertices (V, 4) f32, triangles (T, 4)
+ * u32 as for tgk_mesh_domain_create. Host only. */
+int tgk_segment_grid_count(const int64_t dims[3], const int64_t *lo, const int64_t *hi, const uint32_t *leaf_order,
+                           uint32_t n_triangles, uint32_t *cell_start, uint64_t *n_entries);
+int tgk_segment_grid_fill(const int64_t dims[3], const int64_t *lo, const int64_t *hi, const uint32_t *leaf_order,
+                          uint32_t n_triangles, const float *vertices, uint32_t n_vertices, const uint32_t *triangles,
+                          const uint32_t *cell_start, float *entries, uint64_t n_entries);
+
 /* Attaches the boundary surface used by boundary seeding (domain.py:48-70)
  * to a mask domain; mesh domains already carry theirs. */
 int tgk_domain_set_boundary(tgk_domain *domain, const float *vertices, uint32_t n_vertices,

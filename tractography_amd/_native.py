@@ -107,6 +107,10 @@ _SIGNATURES = {
     "tgk_field_evaluate": (c_int, [_P, _P, _P, c_int64, _P]),
     "tgk_domain_contains": (c_int, [_P, _P, _P, c_int64, _P]),
     "tgk_bvh_segment_hit": (c_int, [_P, _P, _P, _P, c_int64, _P, _P, _P]),
+    "tgk_bvh_node_count": (c_int, [c_uint32, c_uint32, POINTER(c_uint32)]),
+    "tgk_bvh_build": (c_int, [_P, _P, c_uint32, c_uint32, _P, _P, _P, c_uint32, _P]),
+    "tgk_segment_grid_count": (c_int, [_P, _P, _P, _P, c_uint32, _P, POINTER(c_uint64)]),
+    "tgk_segment_grid_fill": (c_int, [_P, _P, _P, _P, c_uint32, _P, c_uint32, _P, _P, _P, c_uint64]),
     "tgk_grid_segment_hit": (c_int, [_P, _P, _P, _P, c_int64, _P, _P, _P]),
     "tgk_domain_set_segment_grid": (c_int, [_P, _P, c_uint64, _P, c_uint64]),
     "tgk_math": (c_int, [_P, c_int32, _P, _P, c_int64, _P]),

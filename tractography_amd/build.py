@@ -47,6 +47,7 @@ SOURCES = [
     ("tgk_capi.hip", "off"),
     ("tgk_seeds.hip", "off"),
     ("tgk_scan.hip", "off"),
+    ("tgk_bvh.hip", "off"),
     ("tgk_fast.hip", "fast"),
 ]
 

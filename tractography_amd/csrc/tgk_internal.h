@@ -9,6 +9,9 @@
 
 namespace tgk {
 
+// Sets the thread's tgk_last_error() message and returns `code`.
+int report_error(int code, const char *fmt, ...);
+
 // Plain FP32 path (tgk_plain.hip): one lane per seed, point-major dense output.
 hipError_t launch_track_plain(const TrackLaunch &p, hipStream_t stream);
 hipError_t launch_compact_point_major(const float *dense, int64_t n_seeds, uint32_t n_points,
@@ -67,6 +70,9 @@ struct TodLaunch {
 hipError_t launch_tod(const TodLaunch &t, hipStream_t stream);
 // Fill n floats with +0 (a kernel, so recorded graphs hold no memset node).
 hipError_t launch_zero(float *out, int64_t n, hipStream_t stream);
+// Rows of K packed floats (src) -> rows of `stride` floats (dst), pad = +0.
+hipError_t launch_pad_rows(const float *src, int64_t n_rows, uint32_t K, float *dst, uint32_t stride,
+                           hipStream_t stream);
 // reference utils.py:6-22 (normalize_odf) on n_cells rows of 45, in place.
 hipError_t launch_normalize_odf(float *coefficients, int64_t n_cells, hipStream_t stream);
 

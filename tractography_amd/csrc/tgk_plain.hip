@@ -646,6 +646,19 @@ __global__ __launch_bounds__(256) void zero_kernel(float *__restrict__ out, int6
         out[i] = 0.0f;
 }
 
+// Rows of K packed floats -> rows of `stride` floats (stride >= K), the pad
+// written as +0. One thread per output float, so the stores are coalesced.
+__global__ __launch_bounds__(256) void pad_rows_kernel(const float *__restrict__ src, int64_t n_rows, uint32_t K,
+                                                       float *__restrict__ dst, uint32_t stride)
+{
+    const int64_t n = n_rows * stride;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = i / stride;
+        const uint32_t k = (uint32_t)(i - row * stride);
+        dst[i] = k < K ? src[row * K + k] : 0.0f;
+    }
+}
+
 // utils.py:6-22 (normalize_odf): numpy forms odf / sqrt(4 pi) in float64
 // (sqrt(4 pi) is a float64 scalar), divides by the float32 c0 promoted to
 // float64 and rounds once to float32 on the store; cells with c0 == 0 become
@@ -872,6 +885,17 @@ hipError_t launch_zero(float *out, int64_t n, hipStream_t stream)
         return hipSuccess;
     const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
     zero_kernel<<<(unsigned)blocks, 256, 0, stream>>>(out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pad_rows(const float *src, int64_t n_rows, uint32_t K, float *dst, uint32_t stride,
+                           hipStream_t stream)
+{
+    const int64_t n = n_rows * stride;
+    if (n == 0)
+        return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 16384);
+    pad_rows_kernel<<<(unsigned)blocks, 256, 0, stream>>>(src, n_rows, K, dst, stride);
     return hipGetLastError();
 }
 

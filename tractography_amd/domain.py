@@ -78,52 +78,35 @@ class BoundaryMesh:
 
 
 def build_bvh(triangle_min: np.ndarray, triangle_max: np.ndarray, leaf_size: int = 4):
-    """BVH over triangle bounding boxes (domain.py:72-116).
+    """BVH over triangle bounding boxes (domain.py:72-116), built natively.
 
-    Returns (bounds_min (N, 4) f32, bounds_max (N, 4) f32, nodes (N, 4) i32
-    as (left, right, first, count) with left = right = -1 for leaves, ordered
-    triangle slots (M,) u32). The centroid sort is stable, so equal centroids
-    keep their slot order.
+    Leaves hold at most `leaf_size` triangles; an inner node splits on the
+    longest axis of its box at the median of a stable centroid sort. Returns
+    (bounds_min (N, 4) f32, bounds_max (N, 4) f32, nodes (N, 4) i32 as
+    (left, right, first, count) with left = right = -1 for leaves, ordered
+    triangle slots (M,) u32), nodes in pre-order with the left subtree first.
     """
-    triangle_min = np.asarray(triangle_min, dtype=np.float64)
-    triangle_max = np.asarray(triangle_max, dtype=np.float64)
-    centroids = (triangle_min + triangle_max) * 0.5
-    node_min: list[np.ndarray] = []
-    node_max: list[np.ndarray] = []
-    nodes: list[list[int]] = []
-    ordered: list[np.ndarray] = []
-    n_ordered = 0
-    stack = [(np.arange(len(triangle_min), dtype=np.int64), -1, 0)]
-    while stack:
-        ids, parent, side = stack.pop()
-        node_id = len(nodes)
-        if parent >= 0:
-            nodes[parent][side] = node_id
-        bmin = triangle_min[ids].min(axis=0)
-        bmax = triangle_max[ids].max(axis=0)
-        node_min.append(bmin)
-        node_max.append(bmax)
-        if len(ids) <= leaf_size:
-            nodes.append([-1, -1, n_ordered, len(ids)])
-            ordered.append(ids)
-            n_ordered += len(ids)
-        else:
-            nodes.append([0, 0, 0, 0])
-            axis = int(np.argmax(bmax - bmin))
-            sorted_ids = ids[np.argsort(centroids[ids, axis], kind="stable")]
-            middle = len(sorted_ids) // 2
-            stack.append((sorted_ids[middle:], node_id, 1))
-            stack.append((sorted_ids[:middle], node_id, 0))
-    bounds_min = np.zeros((len(nodes), 4), dtype=np.float32)
-    bounds_max = np.zeros((len(nodes), 4), dtype=np.float32)
-    bounds_min[:, :3] = np.asarray(node_min)
-    bounds_max[:, :3] = np.asarray(node_max)
-    return (
-        bounds_min,
-        bounds_max,
-        np.ascontiguousarray(nodes, dtype=np.int32),
-        np.ascontiguousarray(np.concatenate(ordered), dtype=np.uint32),
+    triangle_min = np.ascontiguousarray(triangle_min, dtype=np.float64)
+    triangle_max = np.ascontiguousarray(triangle_max, dtype=np.float64)
+    if triangle_min.ndim != 2 or triangle_min.shape[1] != 3 or triangle_max.shape != triangle_min.shape:
+        raise ValueError("triangle bounds must be two (n, 3) arrays")
+    n = len(triangle_min)
+    if n >= 2**32:
+        raise ValueError(f"{n} triangles exceed the BVH's 32-bit slots")
+    lib = _native.lib()
+    count = ctypes.c_uint32()
+    _native.check(lib.tgk_bvh_node_count(n, leaf_size, ctypes.byref(count)))
+    n_nodes = count.value
+    bounds_min = np.empty((n_nodes, 4), dtype=np.float32)
+    bounds_max = np.empty((n_nodes, 4), dtype=np.float32)
+    nodes = np.empty((n_nodes, 4), dtype=np.int32)
+    ordered = np.empty(n, dtype=np.uint32)
+    _native.check(
+        lib.tgk_bvh_build(_native.ptr(triangle_min), _native.ptr(triangle_max), n, leaf_size,
+                          _native.ptr(bounds_min), _native.ptr(bounds_max), _native.ptr(nodes), n_nodes,
+                          _native.ptr(ordered))
     )
+    return bounds_min, bounds_max, nodes, ordered
 
 
 class BaseDomain(ABC):
@@ -360,17 +343,33 @@ class Mesh(BaseDomain):
         affine[:3, 3] = origin
         return np.ascontiguousarray(np.linalg.inv(affine), dtype=np.float32)
 
-    def culling_grid(self):
-        """near (X, Y, Z) u8, its inverse affine, and the reach it certifies."""
+    def _cell_lists(self):
+        """CSR cell -> candidate triangles of the grid (built natively once):
+        (dims, origin, lo, hi, cell_start (n_cells + 1,) u32, n_entries)."""
+        cached = getattr(self, "_cell_lists_cache", None)
+        if cached is not None:
+            return cached
+        data = self._ensure_boundary_data()
         dims, origin, lo, hi = self._grid_cells()
-        near = np.zeros(tuple(dims), dtype=np.uint8)
-        span = (hi - lo).max(axis=0)
-        for ox in range(span[0] + 1):
-            for oy in range(span[1] + 1):
-                for oz in range(span[2] + 1):
-                    cell = lo + np.array([ox, oy, oz])
-                    ok = np.all(cell <= hi, axis=1)
-                    near[cell[ok, 0], cell[ok, 1], cell[ok, 2]] = 1
+        dims = np.ascontiguousarray(dims, dtype=np.int64)
+        lo = np.ascontiguousarray(lo, dtype=np.int64)
+        hi = np.ascontiguousarray(hi, dtype=np.int64)
+        cell_start = np.empty(int(np.prod(dims)) + 1, dtype=np.uint32)
+        n_entries = ctypes.c_uint64()
+        _native.check(
+            _native.lib().tgk_segment_grid_count(
+                _native.ptr(dims), _native.ptr(lo), _native.ptr(hi), _native.ptr(data["bvh_triangles"]),
+                len(data["indices"]), _native.ptr(cell_start), ctypes.byref(n_entries),
+            )
+        )
+        self._cell_lists_cache = (dims, origin, lo, hi, cell_start, n_entries.value)
+        return self._cell_lists_cache
+
+    def culling_grid(self):
+        """near (X, Y, Z) u8 (1 where a triangle bounding box comes within
+        `reach` of the cell), its inverse affine, and the reach it certifies."""
+        dims, origin, _, _, cell_start, _ = self._cell_lists()
+        near = (np.diff(cell_start) > 0).astype(np.uint8).reshape(tuple(dims))
         return near, self._grid_inverse_affine(origin), np.float32(self.reach)
 
     def segment_grid(self):
@@ -385,37 +384,16 @@ class Mesh(BaseDomain):
         rule gives the traversal's closest hit.
         """
         data = self._ensure_boundary_data()
-        dims, _, lo, hi = self._grid_cells()
-        n_tri = len(data["indices"])
-        position = np.empty(n_tri, dtype=np.int64)
-        position[data["bvh_triangles"].astype(np.int64)] = np.arange(n_tri)
-        cells, slots = [], []
-        span = (hi - lo).max(axis=0)
-        slot_ids = np.arange(n_tri, dtype=np.int64)
-        for ox in range(span[0] + 1):
-            for oy in range(span[1] + 1):
-                for oz in range(span[2] + 1):
-                    cell = lo + np.array([ox, oy, oz])
-                    ok = np.all(cell <= hi, axis=1)
-                    c = cell[ok]
-                    cells.append((c[:, 0] * dims[1] + c[:, 1]) * dims[2] + c[:, 2])
-                    slots.append(slot_ids[ok])
-        cells = np.concatenate(cells)
-        slots = np.concatenate(slots)
-        order = np.lexsort((position[slots], cells))
-        cells, slots = cells[order], slots[order]
-        n_cells = int(np.prod(dims))
-        cell_start = np.zeros(n_cells + 1, dtype=np.uint32)
-        np.cumsum(np.bincount(cells, minlength=n_cells), out=cell_start[1:])
-        vertices = data["vertices"][:, :3]
-        idx = data["indices"][slots, :3].astype(np.int64)
-        a = vertices[idx[:, 0]]
-        entries = np.zeros((len(slots), 12), dtype=np.float32)
-        entries[:, 0:3] = a
-        entries[:, 3] = slots.astype(np.uint32).view(np.float32)
-        entries[:, 4:7] = vertices[idx[:, 1]] - a
-        entries[:, 8:11] = vertices[idx[:, 2]] - a
-        return cell_start, np.ascontiguousarray(entries)
+        dims, _, lo, hi, cell_start, n_entries = self._cell_lists()
+        entries = np.empty((n_entries, 12), dtype=np.float32)
+        _native.check(
+            _native.lib().tgk_segment_grid_fill(
+                _native.ptr(dims), _native.ptr(lo), _native.ptr(hi), _native.ptr(data["bvh_triangles"]),
+                len(data["indices"]), _native.ptr(data["vertices"]), len(data["vertices"]),
+                _native.ptr(data["indices"]), _native.ptr(cell_start), _native.ptr(entries), n_entries,
+            )
+        )
+        return cell_start, entries
 
     def upload(self, runtime: HipRuntime, with_boundary: bool = False) -> DeviceHandle:
         key = id(runtime)
