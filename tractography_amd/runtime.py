@@ -42,10 +42,15 @@ class HipRuntime:
         self.device = int(device)
         self.path = path
         self.precision = precision
+        # Handles made on this runtime: closing the runtime destroys them
+        # first, since each one frees its device memory through it.
+        self._children: weakref.WeakSet = weakref.WeakSet()
         self._finalizer = weakref.finalize(self, lib.tgk_runtime_destroy, handle)
 
     @property
     def handle(self) -> ctypes.c_void_p:
+        if not self._finalizer.alive:
+            raise RuntimeError("the runtime has been closed")
         return self._handle
 
     @property
@@ -76,6 +81,10 @@ class HipRuntime:
         return s.value or 0
 
     def close(self) -> None:
+        """Free the runtime and every field and domain handle made on it
+        (they become unusable)."""
+        for child in list(self._children):
+            child.release()
         self._finalizer()
 
 
@@ -83,9 +92,22 @@ class DeviceHandle:
     """A libtgk object (field or domain) owned by one runtime."""
 
     def __init__(self, runtime: HipRuntime, handle: ctypes.c_void_p, destroy):
+        if not runtime._finalizer.alive:
+            destroy(handle)
+            raise RuntimeError("the runtime has been closed")
         self.runtime = runtime
-        self.handle = handle
+        self._handle = handle
         self._finalizer = weakref.finalize(self, destroy, handle)
+        runtime._children.add(self)
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if not self._finalizer.alive:
+            raise RuntimeError("the device handle has been released (its runtime was closed)")
+        return self._handle
+
+    def release(self) -> None:
+        self._finalizer()
 
 
 def as_float32_affine(matrix) -> np.ndarray:
