@@ -206,3 +206,11 @@ def test_tod_argument_errors():
     empty = tg.tod(fod, n_seeds=3, domain=tg.Nifti1Image(np.zeros((2, 2, 2), dtype=np.uint8), np.eye(4)))
     assert empty.shape == (2, 2, 2, 45)
     np.testing.assert_array_equal(empty.get_fdata(), 0.0)
+
+
+def test_opencl_module_names_the_hip_runtime():
+    # docs/user-guide/tracking.md:12, 36 build `tg.opencl.OpenCLRuntime()`
+    import tractography_amd as tg
+
+    assert tg.opencl.OpenCLRuntime is tg.HipRuntime
+    assert tg.OpenCLRuntime is tg.HipRuntime

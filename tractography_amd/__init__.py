@@ -20,13 +20,14 @@ from . import domain as domain_module
 from . import field as field_module
 from .algorithm import BaseAlgorithm
 from .image import Nifti1Image, Streamlines, Tractogram, is_image
+from . import opencl
+from .opencl import OpenCLRuntime
 from .runtime import HipRuntime
 
 algorithm = algorithm_module
 domain = domain_module
 field = field_module
 seeds_module = seeds
-OpenCLRuntime = HipRuntime
 
 
 def _tracking_components(fod, tracking_algorithm: BaseAlgorithm, domain_image=None):
@@ -393,6 +394,7 @@ __all__ = [
     "core",
     "domain",
     "field",
+    "opencl",
     "local",
     "seeds",
     "streaming",
