@@ -396,25 +396,42 @@ __global__ __launch_bounds__(fast_block<LMAX>()) void connectome_fast_kernel(Fas
 __global__ void zero_counter(uint32_t *counter) { *counter = 0u; }
 
 // Dense seed-major rows -> ragged points. One wavefront per streamline copies
-// its lengths[g] * 3 floats, coalesced on both sides.
+// its lengths[g] * 3 floats, coalesced on both sides, eight loads in flight
+// per lane before their stores (a load-store pair per iteration left the
+// copy latency-bound).
 __global__ __launch_bounds__(256) void compact_seed_major_kernel(const float *__restrict__ dense, int64_t n_seeds,
                                                                  uint32_t n_points,
                                                                  const uint32_t *__restrict__ lengths,
                                                                  const int64_t *__restrict__ offsets,
                                                                  float *__restrict__ out)
 {
+    constexpr int UNROLL = 8;
     const int lane = threadIdx.x & 63;
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     const int64_t capacity = n_seeds * (int64_t)n_points * 3;
     for (int64_t g = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < n_seeds; g += waves) {
         // Lengths are clamped to the row and writes to the output capacity,
-        // whatever the caller's lengths and offsets hold.
+        // whatever the caller's lengths and offsets hold: only i in [lo, hi)
+        // is copied, for which 0 <= first + i < capacity.
         const int64_t count = (int64_t)min(lengths[g], n_points) * 3;
         const float *src = dense + g * (int64_t)n_points * 3;
         const int64_t first = offsets[g] * 3;
-        for (int64_t i = lane; i < count; i += 64)
-            if (first + i >= 0 && first + i < capacity)
-                out[first + i] = src[i];
+        const int64_t lo = first < 0 ? -first : 0;
+        const int64_t hi = min(count, capacity - first);
+        for (int64_t base = lo + lane; base < hi; base += 64 * UNROLL) {
+            float v[UNROLL];
+#pragma unroll
+            for (int k = 0; k < UNROLL; k++) {
+                const int64_t i = base + 64 * k;
+                v[k] = i < hi ? src[i] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < UNROLL; k++) {
+                const int64_t i = base + 64 * k;
+                if (i < hi)
+                    out[first + i] = v[k];
+            }
+        }
     }
 }
 
