@@ -202,8 +202,12 @@ int tgk_track_device(tgk_runtime *rt, const tgk_domain *domain, const tgk_field 
 int tgk_compact_device(tgk_runtime *rt, int32_t path, const float *d_dense, int64_t n_seeds, uint32_t n_points,
                        const uint32_t *d_lengths, int64_t *d_offsets, float *d_points);
 
-/* Storage precision of the coefficients the FAST path reads from `field`
- * (default FP32); converts once on the device and caches. */
+/* Storage precision of the coefficients the FAST path reads from `field`.
+ * Only TGK_PRECISION_FP32 (the default) is served: a FAST-path call on a
+ * field set to a 16-bit storage fails with TGK_ERR_UNSUPPORTED, because
+ * rounding the coefficients to FP16 or BF16 moves tracks by 3e-2 to 5e-2
+ * voxel from the FP64 oracle (the budget is 1e-3), and the fast kernel is
+ * not bound by coefficient bytes, so narrower rows would buy no time. */
 int tgk_field_set_fast_precision(tgk_field *field, int32_t precision);
 
 /* -------------------------------------------------------------- seeds --- */

@@ -291,3 +291,16 @@ def test_fast_results_do_not_depend_on_the_grid(scene):
     np.testing.assert_array_equal(got.lengths, one_block.lengths)
     np.testing.assert_array_equal(got.steps, one_block.steps)
     np.testing.assert_array_equal(got.points, one_block.points)
+
+
+@pytest.mark.parametrize("precision", ["bf16x2", "fp16", "bf16"])
+def test_fast_path_refuses_16_bit_coefficient_storage(scene, precision):
+    cfg, field, domain = scene
+    algorithm = tg.algorithm.Transport(tg.local.RealSymmetricSphericalHarmonics(45), step_size=0.1)
+    seeds = cfg["seeds"][:8]
+    with pytest.raises(NotImplementedError, match="only FP32"):
+        tg.tracking.tractogram(algorithm, domain, field, seeds, save_at=0.25, maximum_streamline_length=5.0,
+                               runtime=tg.HipRuntime(path="fast", precision=precision))
+    got = tg.tracking.tractogram(algorithm, domain, field, seeds, save_at=0.25, maximum_streamline_length=5.0,
+                                 runtime=tg.HipRuntime(path="fast"))
+    assert len(got.streamlines) == len(seeds)

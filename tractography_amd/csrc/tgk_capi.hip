@@ -640,7 +640,10 @@ tgk::TrackLaunch make_launch(const tgk_domain *domain, const tgk_field *field, c
 int ensure_fast_field(tgk_runtime *rt, tgk_field *field, const tgk_algorithm *algorithm, int32_t *sh_order)
 {
     if (field->fast_precision != TGK_PRECISION_FP32)
-        return fail(TGK_ERR_UNSUPPORTED, "fast-path coefficient storage %d is not available", field->fast_precision);
+        return fail(TGK_ERR_UNSUPPORTED,
+                    "fast-path coefficient storage %d is not served (only FP32): 16-bit coefficients move tracks "
+                    "past the 1e-3 voxel budget",
+                    field->fast_precision);
     const int32_t model = algorithm->model;
     *sh_order = algorithm->n_parameters > 45 ? 12 : 8;
     if (field->fast_cached != model) {

@@ -26,8 +26,11 @@ class HipRuntime:
         path: ``"plain"`` (the reference's integrator, bit-exact with the FP32
             host restatement; default) or ``"fast"`` (persistent lanes with
             refill, transcendental-free model evaluation, FP64 arc length).
-        precision: coefficient storage read by the fast path: ``"fp32"``,
-            ``"bf16x2"`` (split hi + lo bfloat16), ``"fp16"`` or ``"bf16"``.
+        precision: coefficient storage read by the fast path. Only
+            ``"fp32"`` is served; ``"bf16x2"``, ``"fp16"`` and ``"bf16"`` are
+            accepted names, but a fast-path call then raises
+            NotImplementedError (16-bit coefficients move tracks by 3e-2 to
+            5e-2 voxel, past the 1e-3 budget).
     """
 
     def __init__(self, device: int = 0, *, path: str = "plain", precision: str = "fp32"):
