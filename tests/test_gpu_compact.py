@@ -43,7 +43,7 @@ def test_compaction_matches_its_contract(runtime, path, case):
     else:  # lengths past the rows: later seeds' offsets run past the output
         lengths = rng.integers(n_points, 3 * n_points, n).astype(np.uint32)
     seed_major = rng.standard_normal((n, n_points, 3)).astype(np.float32)
-    dense = seed_major if path == _native.PATH_FAST else np.ascontiguousarray(seed_major.transpose(1, 0, 2))
+    dense = seed_major  # both paths write seed-major dense rows
     want_offsets, want_points = _expected(seed_major, lengths, n_points)
     dev = torch.device("cuda", runtime.device)
     d_dense = torch.from_numpy(dense.reshape(-1)).to(dev)

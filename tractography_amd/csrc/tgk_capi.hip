@@ -692,14 +692,9 @@ int run_compact(tgk_runtime *rt, int32_t path, const float *d_dense, int64_t n_s
         return r;
     TGK_HIP(tgk::launch_offsets(d_lengths, n_seeds, d_offsets, rt->workspace.ptr, scan_bytes, rt->stream),
             "offsets scan");
-    if (path == TGK_PATH_PLAIN)
-        TGK_HIP(tgk::launch_compact_point_major(d_dense, n_seeds, n_points, d_lengths, d_offsets, d_points,
-                                                rt->stream),
-                "compaction");
-    else
-        TGK_HIP(tgk::launch_compact_seed_major(d_dense, n_seeds, n_points, d_lengths, d_offsets, d_points,
-                                               rt->stream),
-                "compaction");
+    // Both paths write seed-major dense rows.
+    TGK_HIP(tgk::launch_compact_seed_major(d_dense, n_seeds, n_points, d_lengths, d_offsets, d_points, rt->stream),
+            "compaction");
     return TGK_OK;
 }
 

@@ -12,11 +12,9 @@ namespace tgk {
 // Sets the thread's tgk_last_error() message and returns `code`.
 int report_error(int code, const char *fmt, ...);
 
-// Plain FP32 path (tgk_plain.hip): one lane per seed, point-major dense output.
+// Plain FP32 path (tgk_plain.hip): one lane per seed, seed-major dense output
+// (compacted by launch_compact_seed_major, as the fast path's).
 hipError_t launch_track_plain(const TrackLaunch &p, hipStream_t stream);
-hipError_t launch_compact_point_major(const float *dense, int64_t n_seeds, uint32_t n_points,
-                                      const uint32_t *lengths, const int64_t *offsets, float *out,
-                                      hipStream_t stream);
 
 // Boundary connectome (tgk_plain.hip, plain arithmetic): a mask domain, its
 // boundary surface with BVH, 9 (row, col, weight) entries per seed.
@@ -92,6 +90,7 @@ hipError_t launch_track_fast(const FastLaunch &p, int grid_blocks, hipStream_t s
 // nothing (so later launches make no device queries, e.g. while a graph is
 // being recorded).
 hipError_t prepare_track_fast(const FastLaunch &p);
+// Dense seed-major rows (both paths) -> ragged points.
 hipError_t launch_compact_seed_major(const float *dense, int64_t n_seeds, uint32_t n_points,
                                      const uint32_t *lengths, const int64_t *offsets, float *out,
                                      hipStream_t stream);

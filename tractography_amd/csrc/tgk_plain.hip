@@ -6,10 +6,12 @@
 // identical to the FP32 host restatement. The connectome and TOD kernels run
 // the same arithmetic on persistent lanes that claim boundary seeds.
 //
-// Output layout: lanes of a wavefront start together and advance `time` by
-// the same step, so all live lanes save on the same iteration. Points are
-// therefore stored point-major, dense[(n * n_seeds + g) * 3 + c], which makes
-// each save of a wavefront one contiguous 768-byte store.
+// Output layout: seed-major, dense[(g * n_points + n) * 3 + c], the fast
+// path's layout, so one compaction kernel serves both paths. (A point-major
+// layout made each save of a wavefront one contiguous store, but its
+// compaction had to gather rows of every point index: 9.2 ms at 1M seeds
+// against 2.5 ms, while the saves cost this VALU-bound kernel nothing
+// measurable either way.)
 #include <hip/hip_runtime.h>
 
 #include "tgk_internal.h"
@@ -19,9 +21,9 @@ namespace tgk {
 
 namespace {
 
-__device__ __forceinline__ void store_point_major(float *dense, int64_t n_seeds, uint32_t n, int64_t g, V3 p)
+__device__ __forceinline__ void store_point(float *dense, uint32_t n_points, uint32_t n, int64_t g, V3 p)
 {
-    float *dst = dense + ((int64_t)n * n_seeds + g) * 3;
+    float *dst = dense + (g * (int64_t)n_points + n) * 3;
     dst[0] = p.x;
     dst[1] = p.y;
     dst[2] = p.z;
@@ -41,7 +43,7 @@ __global__ __launch_bounds__(256) void track_plain_kernel(TrackLaunch p)
     if constexpr (ALGO == ALGO_DIFFUSION || ALGO == ALGO_PROBABILISTIC)
         rng = seed_stream(p, p.first_seed + g);
 
-    store_point_major(p.points, p.n_seeds, 0, g, point);
+    store_point(p.points, p.n_points, 0, g, point);
     const float step = p.algo.step_size;
     const float save = p.save_at;
     const uint32_t stride = p.field.stride;
@@ -79,8 +81,7 @@ __global__ __launch_bounds__(256) void track_plain_kernel(TrackLaunch p)
         time += step;
         if (time >= save) {
             time -= save;
-            store_point_major(p.points, p.n_seeds, n, g,
-                              v3(point.x - time * o.x, point.y - time * o.y, point.z - time * o.z));
+            store_point(p.points, p.n_points, n, g, v3(point.x - time * o.x, point.y - time * o.y, point.z - time * o.z));
             n++;
         }
     }
@@ -117,68 +118,6 @@ hipError_t launch_plain_algo(const TrackLaunch &p, hipStream_t stream)
     case ALGO_PROBABILISTIC: return launch_plain_domain<MODEL, ALGO_PROBABILISTIC>(p, stream);
     case ALGO_DETERMINISTIC: return launch_plain_domain<MODEL, ALGO_DETERMINISTIC>(p, stream);
     default: return launch_plain_domain<MODEL, ALGO_TRANSPORT>(p, stream);
-    }
-}
-
-// Point-major dense [n][seed][3] -> ragged points in seed order. A block owns
-// 64 consecutive seeds and walks their points in chunks of 16 up to the
-// block's longest streamline: it reads a chunk as rows of 64 seeds (768
-// contiguous bytes per point index, skipping points past each length), turns
-// it around in LDS and writes each seed's run of up to 16 points (192 bytes)
-// contiguously. Lengths are clamped to n_points.
-constexpr int COMPACT_SEEDS = 64;
-constexpr int COMPACT_POINTS = 16;
-
-__global__ __launch_bounds__(256) void compact_point_major_kernel(const float *__restrict__ dense, int64_t n_seeds,
-                                                                  uint32_t n_points,
-                                                                  const uint32_t *__restrict__ lengths,
-                                                                  const int64_t *__restrict__ offsets,
-                                                                  float *__restrict__ out)
-{
-    __shared__ float tile[COMPACT_SEEDS][COMPACT_POINTS * 3 + 1];
-    __shared__ uint32_t s_len[COMPACT_SEEDS];
-    __shared__ int64_t s_off[COMPACT_SEEDS];
-    __shared__ uint32_t s_longest;
-    const int64_t g0 = (int64_t)blockIdx.x * COMPACT_SEEDS;
-    const int t = threadIdx.x;
-    if (t == 0)
-        s_longest = 0;
-    __syncthreads();
-    if (t < COMPACT_SEEDS) {
-        const int64_t g = g0 + t;
-        const uint32_t len = g < n_seeds ? min(lengths[g], n_points) : 0u;
-        s_len[t] = len;
-        s_off[t] = g < n_seeds ? offsets[g] : 0;
-        atomicMax(&s_longest, len);
-    }
-    __syncthreads();
-    const uint32_t longest = s_longest;
-    const int width = (int)(n_seeds - g0 < COMPACT_SEEDS ? n_seeds - g0 : COMPACT_SEEDS);
-    const int row = width * 3;
-    for (uint32_t n0 = 0; n0 < longest; n0 += COMPACT_POINTS) {
-        for (int e = t; e < COMPACT_POINTS * row; e += 256) {
-            const int pn = e / row;
-            const int rem = e - pn * row;
-            const int gl = rem / 3;
-            const uint32_t n = n0 + (uint32_t)pn;
-            if (n < s_len[gl])
-                tile[gl][pn * 3 + (rem - gl * 3)] = dense[((int64_t)n * n_seeds + g0) * 3 + rem];
-        }
-        __syncthreads();
-        const int gl = t >> 2, quarter = t & 3;
-        if (gl < width) {
-            const uint32_t len = s_len[gl];
-            const int valid = len > n0 ? (int)min(len - n0, (uint32_t)COMPACT_POINTS) * 3 : 0;
-            // Offsets come from the caller's lengths: stay inside the
-            // n_seeds * n_points capacity whatever they hold.
-            const int64_t first = (s_off[gl] + (int64_t)n0) * 3;
-            const int64_t capacity = n_seeds * (int64_t)n_points * 3;
-            const int end = min(12 * quarter + 12, valid);
-            for (int i = 12 * quarter; i < end; i++)
-                if (first + i >= 0 && first + i < capacity)
-                    out[first + i] = tile[gl][i];
-        }
-        __syncthreads();
     }
 }
 
@@ -691,18 +630,6 @@ hipError_t launch_track_plain(const TrackLaunch &p, hipStream_t stream)
     if (p.algo.model == MODEL_DTI)
         return launch_plain_algo<MODEL_DTI>(p, stream);
     return launch_plain_algo<MODEL_SH>(p, stream);
-}
-
-hipError_t launch_compact_point_major(const float *dense, int64_t n_seeds, uint32_t n_points,
-                                      const uint32_t *lengths, const int64_t *offsets, float *out,
-                                      hipStream_t stream)
-{
-    if (n_seeds == 0)
-        return hipSuccess;
-    const int64_t blocks = (n_seeds + COMPACT_SEEDS - 1) / COMPACT_SEEDS;
-    compact_point_major_kernel<<<(unsigned)blocks, 256, 0, stream>>>(dense, n_seeds, n_points, lengths, offsets,
-                                                                    out);
-    return hipGetLastError();
 }
 
 hipError_t launch_evaluate(int model, int K, const float *params, const float *orientations, int64_t n, float *out,
