@@ -889,8 +889,9 @@ def main():
                         help="blocks of the fast kernel (default: occupancy-sized)")
     parser.add_argument("--autotune", action="store_true",
                         help="choose the fast kernel's grid on 100k seeds of the workload before timing")
-    parser.add_argument("--api", action="store_true",
-                        help="also time one tg.tractogram call on host arrays (transfers included)")
+    parser.add_argument("--no-api", dest="api", action="store_false",
+                        help="skip timing tg.tractogram on host arrays (transfers and uploads included)")
+    parser.add_argument("--api", dest="api", action="store_true", help=argparse.SUPPRESS)
     parser.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                         help="before timing, run one elementwise copy that reads and writes BYTES each "
                              "(reference dispatch for hardware byte counters)")
@@ -1014,22 +1015,33 @@ def main():
 
 
 def api_leg(tg, cfg, path: str):
-    """One tg.tractogram call on host arrays (seeds in, ragged streamlines
-    out, so PCIe transfers and the host-side split are included), after one
-    untimed call on a small prefix."""
+    """tg.tractogram on host arrays (seeds in, ragged streamlines out, so
+    PCIe transfers and the host-side split are included), on a runtime
+    warmed by one call on other objects. `cold_seconds`: the first call on
+    fresh field and domain objects, which uploads the volumes (and, on the
+    fast path, transforms the field); `seconds`: the same call again, with
+    the volumes resident."""
     runtime = tg.HipRuntime(path=path)
-    field = tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4)))
-    domain = make_domain(tg, cfg)
     algorithm = make_algorithm(tg, cfg)
     kwargs = dict(save_at=cfg["save_at"], maximum_streamline_length=cfg["max_length"], runtime=runtime)
-    tg.tractogram(field, cfg["seeds"][:1000], domain, algorithm, **kwargs)
+    tg.tractogram(tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4))), cfg["seeds"][:1000], make_domain(tg, cfg),
+                  algorithm, **kwargs)
+    field = tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4)))
+    domain = make_domain(tg, cfg)
+    if cfg.get("mesh") is not None:
+        domain._ensure_boundary_data()  # host-side mesh tables are not part of the call
+    t0 = time.perf_counter()
+    result = tg.tractogram(field, cfg["seeds"], domain, algorithm, **kwargs)
+    cold = time.perf_counter() - t0
+    del result
     t0 = time.perf_counter()
     result = tg.tractogram(field, cfg["seeds"], domain, algorithm, **kwargs)
     seconds = time.perf_counter() - t0
     points = int(np.sum(result.streamlines.lengths))
-    print(f"[bench] tg.tractogram on host arrays: {seconds:.2f} s, {len(result.streamlines)} streamlines, "
-          f"{points} points", file=sys.stderr, flush=True)
-    return {"seconds": seconds, "streamlines": len(result.streamlines), "points": points, "path": path}
+    print(f"[bench] tg.tractogram on host arrays: {seconds:.2f} s ({cold:.2f} s with the uploads), "
+          f"{len(result.streamlines)} streamlines, {points} points", file=sys.stderr, flush=True)
+    return {"seconds": seconds, "cold_seconds": cold, "streamlines": len(result.streamlines), "points": points,
+            "path": path}
 
 
 TRAFFIC_CALIBRATION_BYTES = 1 << 30
@@ -1067,7 +1079,7 @@ def traffic_leg(args):
             out = Path(tmp) / counter
             cmd = [profiler, "--pmc", counter, "--output-format", "csv", "-d", str(out), "-o", "run", "--",
                    sys.executable, str(ROOT / "bench.py"), "--workload", args.workload, "--paths", "fast",
-                   "--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--no-traffic",
+                   "--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--no-traffic", "--no-api",
                    "--calibrate-copy", str(TRAFFIC_CALIBRATION_BYTES)]
             if args.seeds:
                 cmd += ["--seeds", str(args.seeds)]
