@@ -29,8 +29,19 @@ __device__ __forceinline__ void store_point(float *dense, uint32_t n_points, uin
     dst[2] = p.z;
 }
 
+// Waves per SIMD the compiler must fit: the SH Transport and Diffusion
+// kernels would take 146-148 VGPRs (three waves); held to 128 they spill a
+// few values to scratch and run four waves, 5% faster at 145x174x145 SH45.
+// Other models and algorithms keep the compiler's choice (DTI, already
+// below 128, lost 11% when held to exactly four).
+template <int MODEL, int ALGO> constexpr int plain_min_waves()
+{
+    return MODEL == MODEL_SH && (ALGO == ALGO_TRANSPORT || ALGO == ALGO_DIFFUSION) ? 4 : 1;
+}
+
 template <int MODEL, int ALGO, int DOMAIN, int KT>
-__global__ __launch_bounds__(256) void track_plain_kernel(TrackLaunch p)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_min_waves<MODEL, ALGO>())))
+void track_plain_kernel(TrackLaunch p)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= p.n_seeds)
