@@ -29,14 +29,20 @@ __device__ __forceinline__ void store_point(float *dense, uint32_t n_points, uin
     dst[2] = p.z;
 }
 
-// Waves per SIMD the compiler must fit: the SH Transport and Diffusion
-// kernels would take 146-148 VGPRs (three waves); held to 128 they spill a
-// few values to scratch and run four waves, 5% faster at 145x174x145 SH45.
-// Other models and algorithms keep the compiler's choice (DTI, already
-// below 128, lost 11% when held to exactly four).
+// Waves per SIMD the compiler must fit in the SH tracking kernels, which
+// otherwise take 146-148 VGPRs (Transport, Diffusion; three waves), 164-170
+// (Probabilistic; three) or 291-295 (Deterministic; one). Held to four, four
+// and two waves they spill 10-54 VGPRs to scratch and run 5%, 12% and 75%
+// faster at 145x174x145 SH45. DTI kernels keep the compiler's allocation
+// (already below 128 VGPRs, they lost 11% when held to exactly four), and
+// so do the connectome and TOD kernels (the SH Diffusion connectome held to
+// four waves ran 11% slower).
 template <int MODEL, int ALGO> constexpr int plain_min_waves()
 {
-    return MODEL == MODEL_SH && (ALGO == ALGO_TRANSPORT || ALGO == ALGO_DIFFUSION) ? 4 : 1;
+    if constexpr (MODEL != MODEL_SH)
+        return 1;
+    else
+        return ALGO == ALGO_DETERMINISTIC ? 2 : 4;
 }
 
 template <int MODEL, int ALGO, int DOMAIN, int KT>
