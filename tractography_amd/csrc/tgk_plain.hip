@@ -34,9 +34,10 @@ __device__ __forceinline__ void store_point(float *dense, uint32_t n_points, uin
 // (Probabilistic; three) or 291-295 (Deterministic; one). Held to four, four
 // and two waves they spill 10-54 VGPRs to scratch and run 5%, 12% and 75%
 // faster at 145x174x145 SH45. DTI kernels keep the compiler's allocation
-// (already below 128 VGPRs, they lost 11% when held to exactly four), and
-// so do the connectome and TOD kernels (the SH Diffusion connectome held to
-// four waves ran 11% slower).
+// (already below 128 VGPRs, they lost 11% when held to exactly four). The
+// connectome and TOD kernels are held to two waves for Deterministic only
+// (296 and 342 VGPRs, one wave otherwise: 1.52x faster each); the SH
+// Diffusion connectome held to four waves ran 11% slower.
 template <int MODEL, int ALGO> constexpr int plain_min_waves()
 {
     if constexpr (MODEL != MODEL_SH)
@@ -329,7 +330,9 @@ __device__ __forceinline__ int64_t claim_seed(bool idle, uint32_t *queue, int64_
 // idle until the longest of their wavefront ends. Every track is computed
 // exactly as in a one-lane-per-seed launch; only which lane runs it changes.
 template <int MODEL, int ALGO, int KT>
-__global__ __launch_bounds__(256) void connectome_kernel(ConnectomeLaunch c)
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MODEL == MODEL_SH && ALGO == ALGO_DETERMINISTIC ? 2 : 1)))
+void connectome_kernel(ConnectomeLaunch c)
 {
     const float step = c.algo.step_size;
     const uint32_t stride = c.field.stride;
@@ -509,7 +512,9 @@ __device__ __forceinline__ void flush_runs(bool need, int64_t cell, const float 
 // points (leaving a cell before the update, ending after it) where the
 // wavefront is converged.
 template <int MODEL, int ALGO, int KT>
-__global__ __launch_bounds__(256) void tod_kernel(TodLaunch t)
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MODEL == MODEL_SH && ALGO == ALGO_DETERMINISTIC ? 2 : 1)))
+void tod_kernel(TodLaunch t)
 {
     __shared__ RunStage stages[4];
     RunStage &stage = stages[threadIdx.x >> 6];
