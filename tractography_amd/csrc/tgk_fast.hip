@@ -226,8 +226,22 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
     }
 }
 
+// Waves per SIMD the compiler must fit: the order-8 SH Deterministic kernel
+// would take 147-150 VGPRs (three waves); held to 128 it spills 4-8 and runs
+// four waves, the most its 40 KiB of LDS per block allows: 684 -> 643 ms at
+// 1M seeds. Elsewhere the compiler's allocation is kept: held to four waves,
+// Probabilistic (158-165 VGPRs) ran 3% slower, and the order-12 kernels
+// (241-268 VGPRs, two waves) held to three spilled 70-107 VGPRs and ran
+// 3.8x slower.
+template <int MODEL, int ALGO, int LMAX> constexpr int fast_min_waves()
+{
+    return MODEL == MODEL_SH && LMAX == 8 && ALGO == ALGO_DETERMINISTIC ? 4 : 1;
+}
+
 template <int MODEL, int ALGO, int DOMAIN, int LMAX>
-__global__ __launch_bounds__(fast_block<LMAX>()) void track_fast_kernel(FastLaunch f)
+__global__ __launch_bounds__(fast_block<LMAX>())
+__attribute__((amdgpu_waves_per_eu(fast_min_waves<MODEL, ALGO, LMAX>())))
+void track_fast_kernel(FastLaunch f)
 {
     __shared__ float4 lds_rows[rows_in_lds<MODEL, LMAX>() ? lds_vectors<MODEL, LMAX>() * fast_block<LMAX>() : 1];
     const TrackLaunch &p = f.track;
