@@ -1,8 +1,14 @@
-"""FP64 geometry and the real symmetric SH basis (core.py of the reference).
+"""FP64 geometry and the real symmetric SH basis.
 
-These host functions define the spherical-harmonic convention the engine
-tracks with (MRtrix order, index l(l+1)/2 + m) and are the closed form the
-device basis is checked against.
+These host functions fix the spherical-harmonic convention the engine tracks
+with and are the closed form the device basis is checked against. Public
+names and return shapes are those of the reference's ``core.py:6-95``.
+
+Convention (reference ``core.py:36-69``): coefficient i belongs to the i-th
+pair (l, m) of the walk l = 0, 2, 4, ..., m = -l..l (so i = l(l+1)/2 + m for
+a complete expansion). Its real harmonic is Y_l^0 for m = 0, sqrt(2) Re
+Y_l^m for m > 0 and sqrt(2) Im Y_l^|m| for m < 0, with Y_l^m the complex
+harmonic of ``scipy.special.sph_harm_y`` (Condon-Shortley phase included).
 """
 
 from __future__ import annotations
@@ -10,90 +16,87 @@ from __future__ import annotations
 import numpy as np
 import scipy.special
 
+_SQRT2 = np.sqrt(2.0)
+
 
 def cart2sph(x, y, z):
-    """Cartesian to (azimuth in [0, 2pi), colatitude in [0, pi], radius).
-
-    At the origin both angles and the radius are zero (core.py:6-26).
-    """
+    """(azimuth in [0, 2 pi), colatitude in [0, pi], radius) of the points
+    (x, y, z), arrays broadcast together. The origin maps to angles 0 and
+    radius 0. Float32 input keeps float32 angles (reference ``core.py:6-26``)."""
     x, y, z = np.broadcast_arrays(x, y, z)
-    radius = np.sqrt(x**2 + y**2 + z**2)
-    nonzero = radius > 0
-    cos_colatitude = np.divide(
-        z,
-        radius,
-        out=np.zeros_like(radius, dtype=np.result_type(radius, np.float32)),
-        where=nonzero,
-    )
-    azimuth = np.mod(np.arctan2(y, x), 2 * np.pi)
-    colatitude = np.arccos(np.clip(cos_colatitude, -1.0, 1.0))
-    azimuth = np.where(nonzero, azimuth, 0.0)
-    colatitude = np.where(nonzero, colatitude, 0.0)
+    radius = np.sqrt(x * x + y * y + z * z)
+    away = radius > 0
+    # Divide only away from the origin; the origin's cosine is never used.
+    cosine = z / np.where(away, radius, 1)
+    colatitude = np.where(away, np.arccos(np.clip(cosine, -1, 1)), 0)
+    azimuth = np.where(away, np.mod(np.arctan2(y, x), 2 * np.pi), 0)
     return azimuth, colatitude, radius
 
 
 def sph2cart(azimuth, colatitude, radius):
-    """Spherical to Cartesian coordinates (core.py:29-33)."""
-    x = radius * np.sin(colatitude) * np.cos(azimuth)
-    y = radius * np.sin(colatitude) * np.sin(azimuth)
-    z = radius * np.cos(colatitude)
-    return x, y, z
+    """Inverse of :func:`cart2sph` (reference ``core.py:29-33``)."""
+    ring = radius * np.sin(colatitude)
+    return ring * np.cos(azimuth), ring * np.sin(azimuth), radius * np.cos(colatitude)
+
+
+def sh_degrees_orders(n_coefficients: int):
+    """Degree and order of each of the first ``n_coefficients`` terms of the
+    symmetric expansion, as two int arrays."""
+    degrees, orders = [], []
+    degree = 0
+    while len(degrees) < n_coefficients:
+        span = np.arange(-degree, degree + 1)
+        degrees.extend([degree] * span.size)
+        orders.extend(span.tolist())
+        degree += 2
+    return np.asarray(degrees[:n_coefficients]), np.asarray(orders[:n_coefficients])
 
 
 def sh_order(n_coefficients: int) -> int:
-    """Largest even degree of a symmetric SH expansion with this many terms."""
-    lmax, count = 0, 1
-    while count < n_coefficients:
-        lmax += 2
-        count += 2 * lmax + 1
-    if count != n_coefficients:
+    """Largest even degree of a complete symmetric SH expansion with this many
+    terms ((L+1)(L+2)/2 = 1, 6, 15, 28, 45, 66, 91, ...)."""
+    degree = 0
+    while (degree + 1) * (degree + 2) // 2 < n_coefficients:
+        degree += 2
+    if (degree + 1) * (degree + 2) // 2 != n_coefficients:
         raise ValueError(f"{n_coefficients} is not a symmetric SH coefficient count (1, 6, 15, 28, 45, 66, 91, ...)")
-    return lmax
+    return degree
 
 
 def ishtmtx(azimuths, colatitudes, n_coefficients):
-    """MRtrix-compatible inverse SH transform and its angular derivatives.
+    """The real symmetric SH basis at n directions and its angular derivatives.
 
-    Returns ``(matrix, derivatives)`` with ``matrix`` of shape (n, K) and
-    ``derivatives`` of shape (2, n, K) holding d/dcolatitude and d/dazimuth
-    (core.py:36-69). Coefficient i walks degrees l = 0, 2, 4, ... and orders
-    m = -l..l; negative orders use sqrt(2) Im(Y), positive sqrt(2) Re(Y).
+    Returns ``(matrix, derivatives)``: ``matrix`` of shape (n, K) and
+    ``derivatives`` of shape (2, n, K), d/dcolatitude then d/dazimuth
+    (reference ``core.py:36-69``). All K harmonics are evaluated in one
+    broadcast call over the (direction, term) grid; each column then keeps
+    the real or imaginary part its order selects.
     """
-    colatitudes = np.asarray(colatitudes, dtype=np.float64)
-    azimuths = np.asarray(azimuths, dtype=np.float64)
-    matrix = np.zeros((n_coefficients, len(colatitudes)))
-    matrix_derivative = np.zeros((n_coefficients, len(colatitudes), 2))
-    degree = 0
-    order = 0
-    for i in range(n_coefficients):
-        if order < 0:
-            m, d = scipy.special.sph_harm_y(degree, -order, colatitudes, azimuths, diff_n=1)
-            matrix[i] = np.sqrt(2) * np.imag(m)
-            matrix_derivative[i] = np.sqrt(2) * np.imag(d)
-        elif order == 0:
-            m, d = scipy.special.sph_harm_y(degree, order, colatitudes, azimuths, diff_n=1)
-            matrix[i] = np.real(m)
-            matrix_derivative[i] = np.real(d)
-        else:
-            m, d = scipy.special.sph_harm_y(degree, order, colatitudes, azimuths, diff_n=1)
-            matrix[i] = np.sqrt(2) * np.real(m)
-            matrix_derivative[i] = np.sqrt(2) * np.real(d)
-        if degree == order:
-            degree += 2
-            order = -degree
-        else:
-            order += 1
-    return matrix.T, matrix_derivative.T
+    colatitudes = np.atleast_1d(np.asarray(colatitudes, dtype=np.float64))
+    azimuths = np.atleast_1d(np.asarray(azimuths, dtype=np.float64))
+    degree, order = sh_degrees_orders(n_coefficients)
+    value, jacobian = scipy.special.sph_harm_y(degree[None, :], np.abs(order)[None, :], colatitudes[:, None],
+                                               azimuths[:, None], diff_n=1)
+    weight = np.where(order == 0, 1.0, _SQRT2)
+    imaginary = order < 0
+
+    def real_part(z):
+        # z: (n, K) complex -> the real basis column by column.
+        return weight * np.where(imaginary, z.imag, z.real)
+
+    derivatives = np.stack([real_part(jacobian[..., 0]), real_part(jacobian[..., 1])])
+    return real_part(value), derivatives
 
 
 def fibonacci_sphere(samples=1000):
-    """Points on the unit sphere by the Fibonacci lattice (core.py:72-95)."""
-    phi = (1 + np.sqrt(5)) / 2
-    golden_angle = 2 * np.pi / phi
-    indices = np.arange(samples)
-    theta = golden_angle * indices
-    z = 1 - (2 * indices / (samples - 1))
-    radius = np.sqrt(1 - z**2)
-    x = radius * np.cos(theta)
-    y = radius * np.sin(theta)
-    return np.vstack((x, y, z)).T
+    """``samples`` points of the golden-angle spiral on the unit sphere, from
+    the north pole (z = 1) to the south pole in equal steps of z (reference
+    ``core.py:72-95``). The points set the sample directions of FOD seeding,
+    so the formula is the reference's: point k at z_k = 1 - 2k/(samples-1),
+    azimuth k times the golden angle 2 pi / phi."""
+    k = np.arange(samples)
+    golden_angle = 2 * np.pi / ((1 + np.sqrt(5)) / 2)
+    z = 1 - (2 * k / (samples - 1))
+    turn = golden_angle * k
+    ring = np.sqrt(1 - z * z)
+    return np.stack([ring * np.cos(turn), ring * np.sin(turn), z], axis=1)
