@@ -41,6 +41,42 @@ def brain145():
     return cfg, field, domain
 
 
+@pytest.fixture(scope="module")
+def brain145_dti():
+    # The DTI workload: tensors whose principal axis follows the first smooth
+    # direction field, isotropic outside the ellipsoid; Transport gamma 1,
+    # streamlines of 1 to 3000 steps.
+    cfg = bench.brain145_dti_transport(seed=0, n_seeds=2000)
+    field = tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4)))
+    domain = tg.domain.Image(tg.Nifti1Image(cfg["mask"], np.eye(4)))
+    return cfg, field, domain
+
+
+def _compare_tracks(got, ref):
+    np.testing.assert_array_equal(got.lengths, ref["lengths"])
+    np.testing.assert_array_equal(got.steps, ref["steps"])
+    np.testing.assert_array_equal(got.points, oracle.ragged(ref))
+
+
+def test_dti_workload_plain_bit_exact_and_fast_within_tolerance(fast_runtime, runtime, brain145_dti, oracle_threads):
+    cfg, field, domain = brain145_dti
+    algorithm = tg.algorithm.Transport(tg.local.DiffusionTensorImaging(), step_size=cfg["step_size"],
+                                       gamma=cfg["gamma"])
+    args = (cfg["seeds"], [0], 2000, cfg["save_at"], cfg["max_length"])
+    plain = tg.tracking.track(algorithm, domain, field, *args, runtime)
+    _compare_tracks(plain, oracle.track(domain, field, algorithm, *args, precision=32, threads=oracle_threads))
+    # Uneven lengths: from a single step up to the 3000-step cap.
+    assert plain.steps.min() <= 5 and plain.steps.max() >= 2900
+    ref = oracle.track(domain, field, algorithm, *args, precision=64, threads=oracle_threads)
+    fast = tg.tracking.track(algorithm, domain, field, *args, fast_runtime)
+    rf = deviation.report(fast.lengths, fast.steps, fast.points, ref)
+    rp = deviation.report(plain.lengths, plain.steps, plain.points, ref)
+    print(f"\n[dti145] fast {rf}\n[dti145] plain {rp}")
+    assert rf["max_deviation"] <= TOLERANCE
+    assert rf["compared_fraction"] > 0.9 * rp["compared_fraction"]
+    assert np.all(fast.lengths >= 1) and np.all(fast.lengths <= 1200)
+
+
 @pytest.mark.parametrize("name", ["transport", "diffusion"])
 def test_fast_sh45_tracks_fp64_oracle(fast_runtime, runtime, brain145, name, oracle_threads):
     cfg, field, domain = brain145
@@ -54,6 +90,9 @@ def test_fast_sh45_tracks_fp64_oracle(fast_runtime, runtime, brain145, name, ora
     ref = oracle.track(domain, field, algorithm, *args, precision=64, threads=oracle_threads)
     fast = tg.tracking.track(algorithm, domain, field, *args, fast_runtime)
     plain = tg.tracking.track(algorithm, domain, field, *args, runtime)
+    # The plain path's contract on the benchmark field: every streamline's
+    # saved-point count, step count and points equal the FP32 restatement's.
+    _compare_tracks(plain, oracle.track(domain, field, algorithm, *args, precision=32, threads=oracle_threads))
     rf = deviation.report(fast.lengths, fast.steps, fast.points, ref)
     rp = deviation.report(plain.lengths, plain.steps, plain.points, ref)
     print(f"\n[{name}] fast {rf}\n[{name}] plain {rp}")
@@ -62,10 +101,6 @@ def test_fast_sh45_tracks_fp64_oracle(fast_runtime, runtime, brain145, name, ora
     # the FP64 one; over 300-voxel streamlines that leaves about half of the
     # points, as for the reference's FP32 arithmetic.
     assert rf["compared_fraction"] > max(0.3, 0.9 * rp["compared_fraction"])
-    # The plain path's contract is bit-exactness with the FP32 restatement;
-    # its own FP32 rounding (the reference's basis formulas) is reported,
-    # and only bounded loosely here.
-    assert rp["max_deviation"] <= 10 * TOLERANCE
     # Every streamline starts at its seed and saves at most n_points points.
     assert np.all(fast.lengths >= 1) and np.all(fast.lengths <= 1200)
 

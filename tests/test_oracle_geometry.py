@@ -222,3 +222,127 @@ def test_fod_seeds_stay_in_their_voxels(oracle_threads):
     assert np.all(seeds[:, :3] >= -0.5) and np.all(seeds[:, :3] <= 9.5)
     np.testing.assert_allclose(np.linalg.norm(seeds[:, 4:7], axis=1), 1, atol=1e-5)
     assert np.all(mask.get_fdata()[tuple(np.round(seeds[:, :3]).astype(int).T)] == 1)
+
+
+def _stream_uniforms(rng_seed, n_seeds, per_seed):
+    """The first `per_seed` randu draws of each boundary seed's stream
+    rng_init(rng_seed, g) (core.cl:196-214), as float64."""
+    out = np.zeros((n_seeds, per_seed))
+    for g in range(n_seeds):
+        draws, _ = oracle.rng_draws(oracle.rng_init(rng_seed, g), 1, per_seed)
+        out[g] = draws
+    return out
+
+
+def test_boundary_seeds_follow_the_noise_stream_in_float64():
+    # domain.py:131-166 restated in float64 from each seed's first three
+    # draws: the triangle whose cumulative area first exceeds the first
+    # draw, the point (1 - sqrt(u), sqrt(u)(1 - v), sqrt(u) v) on (a, b, c),
+    # the inward normal, the 1e-4 nudge. Swapped barycentric roles (sqrt of
+    # v instead of u, weights on the wrong vertices) or a wrong normal fail.
+    mask = np.zeros((6, 5, 4), np.uint8)
+    mask[1:5, 1:4, 1:3] = 1
+    mask[2, 2, 3] = 1
+    affine = np.array([[0.9, 0.2, 0.0, 3.0], [-0.1, 1.1, 0.3, -2.0], [0.05, 0.0, 1.3, 0.5], [0, 0, 0, 1]])
+    domain = tg.domain.Image(tg.Nifti1Image(mask, affine))
+    data = domain._ensure_boundary_data()
+    vertices = data["vertices"][:, :3].astype(np.float64)
+    triangles = data["indices"][:, :3].astype(np.int64)
+    cumulative = data["cumulative"].astype(np.float64)
+    n = 3000
+    seeds = oracle.domain_seed(domain, 99, n)
+    u = _stream_uniforms(99, n, 3)
+    tri = np.minimum(np.searchsorted(cumulative, u[:, 0], side="right"), len(triangles) - 1)
+    a, b, c = (vertices[triangles[tri, k]] for k in range(3))
+    root = np.sqrt(u[:, 1])
+    sampled = (1 - root)[:, None] * a + (root * (1 - u[:, 2]))[:, None] * b + (root * u[:, 2])[:, None] * c
+    outward = np.cross(b - a, c - a)
+    inward = -outward / np.linalg.norm(outward, axis=1, keepdims=True)
+    np.testing.assert_allclose(seeds[:, 4:7], inward, atol=2e-6)
+    np.testing.assert_allclose(seeds[:, :3], sampled + 1e-4 * inward, atol=2e-5)
+
+
+def test_boundary_points_are_uniform_on_each_face():
+    # Points of a one-voxel cube's boundary (two triangles per face) must be
+    # uniform on each face square: both in-face coordinates uniform on
+    # [-0.5, 0.5] (Kolmogorov-Smirnov at 1e-3). sqrt taken of the wrong
+    # draw concentrates points near a vertex and fails.
+    from scipy import stats
+
+    domain = tg.domain.Image(tg.Nifti1Image(np.ones((1, 1, 1), np.uint8), np.eye(4)))
+    seeds = oracle.domain_seed(domain, 5, 60000)
+    normal_axis = np.argmax(np.abs(seeds[:, 4:7]), axis=1)
+    for axis in range(3):
+        on_face = seeds[normal_axis == axis, :3]
+        for k in range(3):
+            if k == axis:
+                continue
+            p = stats.kstest(on_face[:, k], stats.uniform(loc=-0.5, scale=1.0).cdf).pvalue
+            assert p > 1e-3, (axis, k, p)
+
+
+def test_fod_seed_positions_under_a_general_affine(oracle_threads):
+    # seeds.cl:68-93: voxel randi(nnz), three jitters randu - 0.5, then the
+    # forward affine, restated in float64 from the states' draws. The affine
+    # rotates, shears and translates, so a transposed matrix, a dropped
+    # translation or jitters on the wrong axes fail.
+    fod, _, _ = fixtures.cross(shape=(7, 6, 3), n_coefficients=15)
+    affine = np.array([[0.0, 1.2, 0.1, -4.0], [0.9, 0.0, -0.2, 7.5], [0.1, 0.3, 1.1, 2.25], [0, 0, 0, 1]])
+    image = tg.Nifti1Image(fod.get_fdata().astype(np.float32), affine)
+    inline, voxels, affine32 = tg.seeds.sparse_fod(image)
+    states = np.random.default_rng(11).integers(0, 2**32 - 1, (4000, 2), dtype=np.uint32)
+    seeds, _ = oracle.seeds_from_fod(0, 15, inline, voxels, affine32, states.copy(), threads=oracle_threads)
+    nnz = len(inline)
+    expected = np.zeros((len(states), 3))
+    for g, state in enumerate(states):
+        draws, _ = oracle.rng_draws(state, 1, 4)
+        index = int(np.float32(draws[0]) * np.float32(nnz))
+        voxel = voxels[index, :3].astype(np.float64) + (draws[1:4].astype(np.float64) - 0.5)
+        expected[g] = affine[:3, :3] @ voxel + affine[:3, 3]
+    np.testing.assert_allclose(seeds[:, :3], expected, atol=5e-6 * np.abs(expected).max())
+    np.testing.assert_array_equal(seeds[:, 3], 1.0)
+
+
+def test_exact_t_ties_are_rare_on_tracking_segments():
+    # The BVH orders triangles with equal centroid coordinates by a stable
+    # sort, the reference by numpy's default argsort (domain.py:99), whose
+    # order among equal keys is not specified. The two can only return
+    # different triangles when two triangles are hit at the same t, i.e. a
+    # segment crosses the surface within rounding of a shared edge. On
+    # tracking-like segments (0.1 voxel, random positions and directions)
+    # across a voxel-boundary surface, count the hits whose second-closest
+    # triangle is within a relative 1e-6 of the closest (a superset of
+    # float32 exact ties).
+    mask = np.zeros((16, 16, 16), bool)
+    idx = np.indices(mask.shape).reshape(3, -1).T
+    mask.reshape(-1)[np.linalg.norm(idx - 7.5, axis=1) < 6.5] = True
+    domain = tg.domain.Image(tg.Nifti1Image(mask.astype(np.uint8), np.eye(4)))
+    verts = domain.boundary_vertices[:, :3].astype(np.float64)
+    tris = verts[domain.boundary_indices[:, :3].astype(int)]
+    rng = np.random.default_rng(8)
+    n = 400_000
+    starts = rng.uniform(0.5, 14.5, (n, 3))
+    d = rng.standard_normal((n, 3))
+    d *= 0.1 / np.linalg.norm(d, axis=1, keepdims=True)
+    found, _, _ = oracle.bvh_hit(domain, starts.astype(np.float32), (starts + d).astype(np.float32))
+    hits = np.flatnonzero(found)
+    assert len(hits) > 5000
+    a, e1, e2 = tris[:, 0], tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0]
+    near_ties = 0
+    for i in hits:
+        s, dd = starts[i], d[i]
+        p = np.cross(dd, e2)
+        det = np.einsum("ij,ij->i", e1, p)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            inv = 1 / det
+            off = s - a
+            u = np.einsum("ij,ij->i", off, p) * inv
+            q = np.cross(off, e1)
+            v = (q @ dd) * inv
+            t = np.einsum("ij,ij->i", e2, q) * inv
+        ok = (np.abs(det) > 1e-10) & (u >= -1e-6) & (v >= -1e-6) & (u + v <= 1 + 1e-6) & (t > 1e-6) & (t <= 1)
+        ts = np.sort(t[ok])
+        if len(ts) >= 2 and ts[1] - ts[0] <= 1e-6 * ts[0]:
+            near_ties += 1
+    print(f"\n[ties] {near_ties} near-ties in {len(hits)} boundary hits of {n} segments, {len(tris)} triangles")
+    assert near_ties <= 2
