@@ -202,6 +202,20 @@ int tgk_track_device(tgk_runtime *rt, const tgk_domain *domain, const tgk_field 
 int tgk_compact_device(tgk_runtime *rt, int32_t path, const float *d_dense, int64_t n_seeds, uint32_t n_points,
                        const uint32_t *d_lengths, int64_t *d_offsets, float *d_points);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): tgk_compact_device with the offsets scan's workspace
+ * in a fixed 64 KiB device array of the library instead of the runtime's
+ * growable buffer, so the call makes no allocation of any kind. Same
+ * arguments and results as tgk_compact_device; n_seeds above
+ * TGK_COMPACT_NOALLOC_MAX_SEEDS fails with TGK_ERR_INVALID. The array is
+ * one per GPU, shared by every runtime on it: calls that overlap on one GPU
+ * (on different streams or runtimes) race and must be serialised by the
+ * caller. */
+#define TGK_COMPACT_NOALLOC_MAX_SEEDS 16775168LL
+int tgk_compact_device_noalloc(tgk_runtime *rt, int32_t path, const float *d_dense, int64_t n_seeds,
+                               uint32_t n_points, const uint32_t *d_lengths, int64_t *d_offsets, float *d_points);
+
 /* Storage precision of the coefficients the FAST path reads from `field`.
  * Only TGK_PRECISION_FP32 (the default) is served: a FAST-path call on a
  * field set to a 16-bit storage fails with TGK_ERR_UNSUPPORTED, because

@@ -33,6 +33,8 @@ SEEDS_VALU = 0
 SEEDS_MFMA = 1
 
 NO_HIT = 0xFFFFFFFF
+# Largest n_seeds tgk_compact_device_noalloc takes (TGK_COMPACT_NOALLOC_MAX_SEEDS).
+COMPACT_NOALLOC_MAX_SEEDS = 16775168
 
 
 class tgk_algorithm(ctypes.Structure):
@@ -87,6 +89,7 @@ _SIGNATURES = {
          _P, _P, _P, _P],
     ),
     "tgk_compact_device": (c_int, [_P, c_int32, _P, c_int64, c_uint32, _P, _P, _P]),
+    "tgk_compact_device_noalloc": (c_int, [_P, c_int32, _P, c_int64, c_uint32, _P, _P, _P]),
     "tgk_domain_seed": (c_int, [_P, _P, c_uint32, c_int64, _P]),
     "tgk_connectome": (c_int, [_P, _P, _P, _P, POINTER(tgk_algorithm), c_int32, c_uint32, c_int64, c_uint32, c_float,
                                _P, _P, _P]),

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <deque>
@@ -140,6 +141,48 @@ struct tgk_tracks {
 };
 
 namespace {
+
+// ---------------------------------------- allocation-free compaction ---
+//
+// tgk_compact_device_noalloc keeps the offsets scan's per-tile sums (one
+// int64 per 2048 seeds, plus the total) in this file-scope device array
+// instead of the runtime's growable workspace. Each device holds its own
+// copy at its own address until the process ends. In host code `scratch`
+// names a host-side stand-in, not device memory: kernels and copies get the
+// device address, looked up once per device when the first runtime on that
+// device is created (scratch_setup) and kept in g_scratch_address.
+constexpr int64_t NOALLOC_SCRATCH_WORDS = 8192;  // 64 KiB: up to 16,775,168 seeds per call
+__device__ __align__(256) int64_t scratch[NOALLOC_SCRATCH_WORDS];
+
+constexpr int MAX_SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_address[MAX_SCRATCH_DEVICES] = {};
+
+// With `device` current: the device address of `scratch` on it, looked up
+// on the first call for that device.
+int scratch_setup(int32_t device)
+{
+    if (device < 0 || device >= MAX_SCRATCH_DEVICES)
+        return fail(TGK_ERR_UNSUPPORTED, "device %d: the allocation-free scratch serves devices 0..%d", device,
+                    MAX_SCRATCH_DEVICES - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_address[device])
+        return TGK_OK;
+    void *ptr = nullptr;
+    TGK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr)
+        return fail(TGK_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_address[device] = ptr;
+    return TGK_OK;
+}
+
+void *scratch_address(int32_t device)
+{
+    if (device < 0 || device >= MAX_SCRATCH_DEVICES)
+        return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_address[device];
+}
 
 void copy_rows3(const float m[16], float out[12])
 {
@@ -296,6 +339,12 @@ int tgk_runtime_create(int32_t device, tgk_runtime **out)
     // The seed counter of the persistent kernels, allocated up front so that
     // no launch allocates (e.g. while a graph is being recorded).
     TGK_HIP(rt->queue.allocate(256), "queue");
+    // The device address of the allocation-free entry point's scratch array,
+    // once per device (a device past the table's size only loses that entry
+    // point, which then fails with TGK_ERR_UNSUPPORTED).
+    if (device < MAX_SCRATCH_DEVICES)
+        if (int r = scratch_setup(device))
+            return r;
     *out = rt.release();
     return TGK_OK;
 }
@@ -735,6 +784,36 @@ int tgk_compact_device(tgk_runtime *rt, int32_t path, const float *d_dense, int6
     if (int r = select_device(rt))
         return r;
     return run_compact(rt, path, d_dense, n_seeds, n_points, d_lengths, d_offsets, d_points);
+}
+
+int tgk_compact_device_noalloc(tgk_runtime *rt, int32_t path, const float *d_dense, int64_t n_seeds,
+                               uint32_t n_points, const uint32_t *d_lengths, int64_t *d_offsets, float *d_points)
+{
+    if (!rt)
+        return fail(TGK_ERR_INVALID, "runtime is NULL");
+    if (path != TGK_PATH_PLAIN && path != TGK_PATH_FAST)
+        return fail(TGK_ERR_INVALID, "unknown path %d", path);
+    if (n_seeds < 0 || (n_seeds > 0 && (!d_dense || !d_lengths || !d_offsets || !d_points)))
+        return fail(TGK_ERR_INVALID, "bad compaction arguments");
+    const size_t scan_bytes = tgk::offsets_workspace_bytes(n_seeds);
+    if (scan_bytes > sizeof(scratch))
+        return fail(TGK_ERR_INVALID,
+                    "%lld seeds exceed the allocation-free compaction's maximum of %lld per call (use "
+                    "tgk_compact_device)",
+                    (long long)n_seeds, (long long)TGK_COMPACT_NOALLOC_MAX_SEEDS);
+    if (n_seeds == 0)
+        return TGK_OK;
+    void *sums = scratch_address(rt->device);
+    if (!sums)
+        return fail(TGK_ERR_UNSUPPORTED, "no allocation-free scratch on device %d", rt->device);
+    if (int r = select_device(rt))
+        return r;
+    // The scan writes every per-tile sum before reading it, so the scratch
+    // needs no clearing between calls.
+    TGK_HIP(tgk::launch_offsets(d_lengths, n_seeds, d_offsets, sums, scan_bytes, rt->stream), "offsets scan");
+    TGK_HIP(tgk::launch_compact_seed_major(d_dense, n_seeds, n_points, d_lengths, d_offsets, d_points, rt->stream),
+            "compaction");
+    return TGK_OK;
 }
 
 int tgk_track(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field, const tgk_algorithm *algorithm,
