@@ -95,6 +95,29 @@ int tgk_runtime_stream(tgk_runtime *rt, void **stream);
  * reference, whose kernel is launched one work-item per seed; set by
  * tractography_amd.autotune. */
 int tgk_runtime_set_fast_grid(tgk_runtime *rt, int32_t blocks);
+/* No counterpart in the reference. The FAST path runs a tracking call as up
+ * to six launches ("generations"): once a generation's streamlines are all
+ * started and fewer than half of them are still in flight, it saves those
+ * and the next generation resumes them packed into full wavefronts. A
+ * generation is cut short only while half its streamlines number at least
+ * min_streamlines (default 16384); a large value runs every call in one
+ * launch. Results do not depend on it. */
+int tgk_runtime_set_fast_repack(tgk_runtime *rt, int32_t min_streamlines);
+/* No counterpart in the reference (its kernel reads a voxel row every step):
+ * how many coefficient rows and mask values the last FAST-path tracking
+ * launch on this runtime read from memory, i.e. how often a lane's cached
+ * cell changed. Synchronises the runtime's stream. Used to price the
+ * kernel's memory traffic. */
+int tgk_runtime_fast_reads(tgk_runtime *rt, uint64_t *rows, uint64_t *mask_values);
+/* No counterpart in the reference. Kernel-only timing for launch tuning:
+ * uploads `seeds` (n, 8) once, tracks them once untimed and then `repeats`
+ * times on the runtime stream (one batch, noise-stream seed 0), and returns
+ * the fastest launch in milliseconds (HIP events around the tracking
+ * launches only) and its orientation-update count. Allocates, so not while
+ * a graph is being recorded. Used by tractography_amd.autotune. */
+int tgk_track_timed(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field, const tgk_algorithm *algorithm,
+                    int32_t path, const float *seeds, int64_t n_seeds, uint32_t n_points, float save_at,
+                    int32_t repeats, float *milliseconds, uint64_t *steps);
 
 /* ------------------------------------------------------ field, domain --- */
 
@@ -218,10 +241,11 @@ int tgk_compact_device_noalloc(tgk_runtime *rt, int32_t path, const float *d_den
 
 /* Storage precision of the coefficients the FAST path reads from `field`.
  * Only TGK_PRECISION_FP32 (the default) is served: a FAST-path call on a
- * field set to a 16-bit storage fails with TGK_ERR_UNSUPPORTED, because
- * rounding the coefficients to FP16 or BF16 moves tracks by 3e-2 to 5e-2
- * voxel from the FP64 oracle (the budget is 1e-3), and the fast kernel is
- * not bound by coefficient bytes, so narrower rows would buy no time. */
+ * field set to a 16-bit storage fails with TGK_ERR_UNSUPPORTED. On the
+ * 145x174x145 benchmark field, coefficients rounded to FP16 or BF16 move
+ * FP64 trajectories by 1.2e-2 and 2.4e-2 voxel (the budget is 1e-3); split
+ * hi + lo BF16 moves them by 9.7e-4, leaving no room for the kernel's own
+ * rounding, while storing as many bytes as FP32. */
 int tgk_field_set_fast_precision(tgk_field *field, int32_t precision);
 
 /* -------------------------------------------------------------- seeds --- */

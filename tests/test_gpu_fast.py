@@ -339,3 +339,71 @@ def test_fast_path_refuses_16_bit_coefficient_storage(scene, precision):
     got = tg.tracking.tractogram(algorithm, domain, field, seeds, save_at=0.25, maximum_streamline_length=5.0,
                                  runtime=tg.HipRuntime(path="fast"))
     assert len(got.streamlines) == len(seeds)
+
+
+@pytest.mark.parametrize("case", ["sh45-diffusion", "sh45-probabilistic", "mesh", "dti", "sh91-diffusion"])
+def test_fast_generations_change_no_result(scene, case):
+    # A fast-path call runs as generations of the persistent kernel: when a
+    # generation's work is all claimed and half of it has ended, the
+    # streamlines in flight are saved and resumed by the next generation in
+    # fuller wavefronts. A threshold of 8 makes every call here run all six
+    # generations; the results must equal a single run to completion bit
+    # for bit (noise streams, saved-point counters and arc-length clocks
+    # carried across).
+    cfg, field, domain = scene
+    seeds = cfg["seeds"]
+    sh45 = tg.local.RealSymmetricSphericalHarmonics(45)
+    max_length = 80.0
+    if case == "sh45-diffusion":
+        algorithm = tg.algorithm.Diffusion(sh45, step_size=0.1, gamma=1.0, noise_variance=0.1, seed=51)
+    elif case == "sh45-probabilistic":
+        algorithm = tg.algorithm.Probabilistic(sh45, step_size=0.1, seed=52)
+        max_length = 20.0
+    elif case == "mesh":
+        domain, seeds, _, _ = fixtures.shell_scene(n_seeds=2000, seed=12)
+        algorithm = tg.algorithm.Diffusion(sh45, step_size=0.1, gamma=1.0, noise_variance=0.1, seed=53)
+    elif case == "dti":
+        tensor, wm, _ = fixtures.tensor_cross()
+        field = tg.field.Image(tensor)
+        domain = tg.domain.Image(wm)
+        algorithm = tg.algorithm.Diffusion(tg.local.DiffusionTensorImaging(), step_size=0.1, gamma=20.0,
+                                           noise_variance=0.1, seed=54)
+        seeds = tg.seeds.from_mask(wm, 2000)
+    else:
+        field, domain, seeds = _order12_scene(n_seeds=2000, seed=56)
+        algorithm = tg.algorithm.Diffusion(tg.local.RealSymmetricSphericalHarmonics(91), step_size=0.1,
+                                           gamma=1.0, noise_variance=0.1, seed=57)
+    rng = [algorithm.get_rng_seed()]
+    whole = tg.HipRuntime(path="fast")
+    whole.set_fast_repack(1 << 30)
+    repacked = tg.HipRuntime(path="fast")
+    repacked.set_fast_repack(8)
+    a = tg.tracking.track(algorithm, domain, field, seeds, rng, len(seeds), 0.25, max_length, whole)
+    b = tg.tracking.track(algorithm, domain, field, seeds, rng, len(seeds), 0.25, max_length, repacked)
+    print(f"\n[{case}] {int(a.steps.sum())} steps, {len(a.points)} points")
+    np.testing.assert_array_equal(b.lengths, a.lengths)
+    np.testing.assert_array_equal(b.steps, a.steps)
+    np.testing.assert_array_equal(b.hits, a.hits)
+    np.testing.assert_array_equal(b.points.view(np.uint32), a.points.view(np.uint32))
+
+
+def test_tune_fast_caches_its_choice_and_changes_no_result(scene, tmp_path):
+    cfg, field, domain = scene
+    algorithm = tg.algorithm.Diffusion(tg.local.RealSymmetricSphericalHarmonics(45), step_size=0.1, gamma=1.0,
+                                       noise_variance=0.1, seed=61)
+    rng = [algorithm.get_rng_seed()]
+    default = tg.tracking.track(algorithm, domain, field, cfg["seeds"], rng, 2000, 0.25, 80.0,
+                                tg.HipRuntime(path="fast"))
+    runtime = tg.HipRuntime(path="fast")
+    cache = tmp_path / "tune.json"
+    kwargs = dict(runtime=runtime, save_at=0.25, maximum_streamline_length=80.0, grid_factors=(1.0,),
+                  repacks=(64, 1 << 30), repeats=1, cache=cache)
+    tuned = tg.autotune.tune_fast(algorithm, domain, field, cfg["seeds"][:1000], **kwargs)
+    print(f"\n[tune] {tuned}")
+    assert not tuned["cached"] and len(tuned["timings"]) == 4 and tuned["milliseconds"] > 0
+    again = tg.autotune.tune_fast(algorithm, domain, field, cfg["seeds"][:1000], **kwargs)
+    assert again["cached"] and (again["blocks"], again["repack"]) == (tuned["blocks"], tuned["repack"])
+    got = tg.tracking.track(algorithm, domain, field, cfg["seeds"], rng, 2000, 0.25, 80.0, runtime)
+    np.testing.assert_array_equal(got.lengths, default.lengths)
+    np.testing.assert_array_equal(got.steps, default.steps)
+    np.testing.assert_array_equal(got.points.view(np.uint32), default.points.view(np.uint32))

@@ -60,6 +60,10 @@ _SIGNATURES = {
     "tgk_version": (c_int, []),
     "tgk_runtime_create": (c_int, [c_int32, POINTER(c_void_p)]),
     "tgk_runtime_set_fast_grid": (c_int, [_P, c_int32]),
+    "tgk_runtime_set_fast_repack": (c_int, [_P, c_int32]),
+    "tgk_runtime_fast_reads": (c_int, [_P, POINTER(c_uint64), POINTER(c_uint64)]),
+    "tgk_track_timed": (c_int, [_P, _P, _P, POINTER(tgk_algorithm), c_int32, _P, c_int64, c_uint32, c_float, c_int32,
+                                POINTER(c_float), POINTER(c_uint64)]),
     "tgk_runtime_destroy": (c_int, [_P]),
     "tgk_runtime_synchronize": (c_int, [_P]),
     "tgk_runtime_info": (c_int, [_P, c_char_p, c_int32, POINTER(c_int32), POINTER(c_int64)]),

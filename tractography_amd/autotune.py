@@ -1,58 +1,159 @@
-"""Launch-shape tuning of the fast tracking path.
+"""Launch tuning of the fast tracking path.
 
 The persistent fast kernel (one lane per streamline in flight, lanes refilled
-from a device queue) needs a grid: enough blocks to fill every compute unit,
-but more blocks than can be resident only add scheduling. ``tune_fast_grid``
-times the real workload on a sample of its seeds for a few grid sizes around
-the occupancy-derived one and keeps the fastest on the runtime. It has no
+from a device queue, run as generations that repack the streamlines still in
+flight) has two launch settings that change its speed and never its results:
+
+* the grid: enough blocks to fill every compute unit once, or a multiple;
+* the repack threshold: how many streamlines a generation must still hold
+  for it to be cut short and its streamlines resumed in fuller wavefronts.
+
+``tune_fast`` times the real workload on a sample of its seeds for each
+candidate pair, kernel only (``tgk_track_timed``: HIP events around the
+tracking launches, seeds uploaded beforehand, noise-stream seed 0, so the
+caller's algorithm draws nothing), sets the fastest on the runtime, and keeps
+it in a JSON file keyed by device, model, coefficient count, algorithm and
+domain kind, so later calls with the same key reuse it without timing.
+
+Block size and the coefficients a lane keeps in LDS are compile-time shapes:
+256-lane blocks holding 40 of the 45 order-8 coefficients in LDS use exactly
+the 160 KiB of a compute unit at four waves per SIMD, the most the kernel's
+registers allow, so they are fixed rather than tuned. None of this has a
 counterpart in the reference, whose OpenCL kernel runs one work-item per seed.
 """
 
 from __future__ import annotations
 
-import time
+import ctypes
+import json
+import os
+from pathlib import Path
 
 import numpy as np
 
-from . import tracking
+from . import _native
 from .runtime import HipRuntime
+from .tracking import point_capacity
 
-DEFAULT_FACTORS = (0.5, 1.0, 1.5, 2.0, 4.0)
+DEFAULT_GRID_FACTORS = (0.5, 1.0, 2.0)
+DEFAULT_REPACKS = (4096, 16384, 65536, 1 << 30)
 
 
-def tune_fast_grid(
-    algorithm,
-    domain,
-    field,
-    seeds,
-    *,
-    runtime: HipRuntime,
-    save_at: float,
-    maximum_streamline_length: float,
-    factors=DEFAULT_FACTORS,
-    repeats: int = 2,
-) -> dict:
+def cache_path() -> Path:
+    """Where tuned settings are kept: $TGK_AUTOTUNE_CACHE, else
+    ~/.cache/tractography_amd/autotune.json."""
+    env = os.environ.get("TGK_AUTOTUNE_CACHE")
+    if env:
+        return Path(env)
+    return Path.home() / ".cache" / "tractography_amd" / "autotune.json"
+
+
+def cache_key(device_name: str, algorithm, domain) -> str:
+    descriptor = algorithm.descriptor()
+    kind = "mesh" if getattr(domain, "is_mesh", False) or type(domain).__name__ == "Mesh" else "mask"
+    return (f"{device_name}|model={descriptor.model}|K={descriptor.n_parameters}|"
+            f"algorithm={descriptor.algorithm}|domain={kind}")
+
+
+def load_cache(path: Path | None = None) -> dict:
+    path = cache_path() if path is None else Path(path)
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    return data if isinstance(data, dict) else {}
+
+
+def store_cache(key: str, entry: dict, path: Path | None = None) -> None:
+    """Adds `entry` under `key`, rewriting the file atomically."""
+    path = cache_path() if path is None else Path(path)
+    data = load_cache(path)
+    data[key] = entry
+    path.parent.mkdir(parents=True, exist_ok=True)
+    tmp = path.with_suffix(path.suffix + f".{os.getpid()}.tmp")
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def apply(runtime: HipRuntime, entry: dict) -> None:
+    runtime.set_fast_grid(int(entry["blocks"]))
+    runtime.set_fast_repack(int(entry["repack"]))
+
+
+def time_fast(runtime: HipRuntime, algorithm, domain, field, seeds, *, save_at: float,
+              maximum_streamline_length: float, repeats: int = 2) -> tuple[float, int]:
+    """(milliseconds, orientation updates) of the fastest of `repeats`
+    device tracking launches of `seeds` with the runtime's current settings."""
+    seeds = np.ascontiguousarray(seeds, dtype=np.float32)
+    field_device = field.upload(runtime)
+    domain_device = domain.upload(runtime)
+    ms = ctypes.c_float()
+    steps = ctypes.c_uint64()
+    _native.check(_native.lib().tgk_track_timed(
+        runtime.handle, domain_device.handle, field_device.handle, algorithm.descriptor(), _native.PATH_FAST,
+        _native.ptr(seeds), len(seeds), point_capacity(maximum_streamline_length, save_at),
+        float(np.float32(save_at)), int(repeats), ctypes.byref(ms), ctypes.byref(steps)))
+    return ms.value, steps.value
+
+
+def tune_fast(algorithm, domain, field, seeds, *, runtime: HipRuntime, save_at: float,
+              maximum_streamline_length: float, grid_factors=DEFAULT_GRID_FACTORS, repacks=DEFAULT_REPACKS,
+              repeats: int = 2, cache: bool | str | Path = True, force: bool = False) -> dict:
     """Times `seeds` (a sample of the workload, e.g. 100k rows of (N, 8)) on
-    the fast path for grids of factor x (compute units x 3 blocks) and sets
-    the fastest on `runtime`. Returns {"blocks": best, "timings": {blocks: s}}
-    with blocks 0 meaning the occupancy-sized default."""
+    the fast path for every grid (0 = occupancy-sized, or factor x compute
+    units x 4 blocks) and repack threshold, and sets the fastest on
+    `runtime`. With `cache`, a stored entry for the same key is applied
+    without timing unless `force`, and a new result is stored.
+
+    Returns {"blocks", "repack", "milliseconds", "steps_per_s", "timings":
+    {"blocks/repack": ms}, "cached": bool}."""
+    if runtime.path != "fast":
+        raise ValueError("tune_fast needs a runtime created with path='fast'")
+    path = None if cache is True else (Path(cache) if cache else None)
+    info = runtime.info()
+    key = cache_key(info["name"], algorithm, domain)
+    if cache and not force:
+        entry = load_cache(path).get(key)
+        if entry is not None:
+            apply(runtime, entry)
+            return {**entry, "cached": True}
+    units = int(info["compute_units"])
+    grids = [0] + sorted({max(1, int(round(f * units * 4))) for f in grid_factors})
+    timings = {}
+    best = None
+    for blocks in grids:
+        for repack in repacks:
+            runtime.set_fast_grid(blocks)
+            runtime.set_fast_repack(repack)
+            ms, steps = time_fast(runtime, algorithm, domain, field, seeds, save_at=save_at,
+                                  maximum_streamline_length=maximum_streamline_length, repeats=repeats)
+            timings[f"{blocks}/{repack}"] = ms
+            if best is None or ms < best[0]:
+                best = (ms, steps, blocks, repack)
+    ms, steps, blocks, repack = best
+    entry = {"blocks": blocks, "repack": repack, "milliseconds": ms,
+             "steps_per_s": steps / (ms * 1e-3) if ms > 0 else 0.0, "timings": timings}
+    apply(runtime, entry)
+    if cache:
+        store_cache(key, entry, path)
+    return {**entry, "cached": False}
+
+
+def tune_fast_grid(algorithm, domain, field, seeds, *, runtime: HipRuntime, save_at: float,
+                   maximum_streamline_length: float, factors=DEFAULT_GRID_FACTORS, repeats: int = 2) -> dict:
+    """The grid alone (repack threshold left as set), kernel-only timing, no
+    cache. Returns {"blocks": best, "timings": {blocks: ms}}."""
     if runtime.path != "fast":
         raise ValueError("tune_fast_grid needs a runtime created with path='fast'")
-    seeds = np.ascontiguousarray(seeds, dtype=np.float32)
-    rng_seeds = [algorithm.get_rng_seed()]
     units = int(runtime.info()["compute_units"])
-    candidates = [0] + sorted({max(1, int(round(f * units * 3))) for f in factors})
+    candidates = [0] + sorted({max(1, int(round(f * units * 4))) for f in factors})
     timings = {}
     for blocks in candidates:
         runtime.set_fast_grid(blocks)
-        best = float("inf")
-        for _ in range(max(1, repeats)):
-            runtime.finish()
-            t0 = time.perf_counter()
-            tracking.track(algorithm, domain, field, seeds, rng_seeds, len(seeds), save_at,
-                           maximum_streamline_length, runtime)
-            best = min(best, time.perf_counter() - t0)
-        timings[blocks] = best
+        timings[blocks], _ = time_fast(runtime, algorithm, domain, field, seeds, save_at=save_at,
+                                       maximum_streamline_length=maximum_streamline_length, repeats=repeats)
     chosen = min(timings, key=timings.get)
     runtime.set_fast_grid(chosen)
     return {"blocks": chosen, "timings": timings}

@@ -80,15 +80,25 @@ struct DeviceBuffer {
 
 }  // namespace
 
+// Byte offsets in the runtime's 256-byte queue buffer (offset 0: the seed
+// counter of the connectome and TOD kernels): the fast kernel's last-launch
+// read totals (two u64: coefficient rows, mask values) and its two
+// saved-streamline counts (u32).
+constexpr size_t FAST_STATS_OFFSET = 64;
+constexpr size_t FAST_CONT_COUNT_OFFSET = 136;
+
 struct tgk_runtime {
     int32_t device = 0;
     hipStream_t stream = nullptr;
     DeviceBuffer workspace;
-    DeviceBuffer queue;        // work counter of the persistent fast kernel
+    DeviceBuffer queue;        // counters of the persistent kernels (layout: FAST_*_OFFSET)
+    DeviceBuffer continuations[2];  // streamlines saved between generations of the fast kernel
+    DeviceBuffer fast_counters;     // claim counter and ended-streamline shards of the fast kernel
     DeviceBuffer seeds_table;  // prepared FOD-seeding table for seeds_key
     int32_t seeds_key[3] = {-1, -1, -1};  // model, K, method
     bool capturing = false;    // the stream is being recorded into a graph
     int32_t fast_grid = 0;     // blocks of the fast kernel; <= 0: sized by occupancy
+    uint32_t fast_min_repack = tgk::FAST_MIN_REPACK;  // see tgk_runtime_set_fast_repack
 };
 
 struct tgk_graph {
@@ -339,6 +349,7 @@ int tgk_runtime_create(int32_t device, tgk_runtime **out)
     // The seed counter of the persistent kernels, allocated up front so that
     // no launch allocates (e.g. while a graph is being recorded).
     TGK_HIP(rt->queue.allocate(256), "queue");
+    TGK_HIP(rt->fast_counters.allocate(tgk::FAST_COUNTER_BYTES), "fast counters");
     // The device address of the allocation-free entry point's scratch array,
     // once per device (a device past the table's size only loses that entry
     // point, which then fails with TGK_ERR_UNSUPPORTED).
@@ -359,6 +370,22 @@ int tgk_runtime_destroy(tgk_runtime *rt)
         (void)hipStreamDestroy(rt->stream);
     }
     delete rt;
+    return TGK_OK;
+}
+
+int tgk_runtime_fast_reads(tgk_runtime *rt, uint64_t *rows, uint64_t *mask_values)
+{
+    if (!rt || !rows || !mask_values)
+        return fail(TGK_ERR_INVALID, "NULL argument");
+    if (int r = select_device(rt))
+        return r;
+    uint64_t totals[2] = {0, 0};
+    TGK_HIP(hipMemcpyAsync(totals, rt->queue.as<char>() + FAST_STATS_OFFSET, sizeof totals, hipMemcpyDeviceToHost,
+                           rt->stream),
+            "read totals");
+    TGK_HIP(hipStreamSynchronize(rt->stream), "hipStreamSynchronize");
+    *rows = totals[0];
+    *mask_values = totals[1];
     return TGK_OK;
 }
 
@@ -391,6 +418,16 @@ int tgk_runtime_set_fast_grid(tgk_runtime *rt, int32_t blocks)
     if (!rt)
         return fail(TGK_ERR_INVALID, "runtime is NULL");
     rt->fast_grid = blocks > 0 ? blocks : 0;
+    return TGK_OK;
+}
+
+int tgk_runtime_set_fast_repack(tgk_runtime *rt, int32_t min_streamlines)
+{
+    if (!rt)
+        return fail(TGK_ERR_INVALID, "runtime is NULL");
+    if (min_streamlines < 1)
+        return fail(TGK_ERR_INVALID, "min_streamlines must be positive, got %d", min_streamlines);
+    rt->fast_min_repack = (uint32_t)min_streamlines;
     return TGK_OK;
 }
 
@@ -626,6 +663,18 @@ namespace {
 // counter that may run past n by one claim per wavefront.
 constexpr int64_t MAX_SEEDS = INT32_MAX;
 
+// A field or domain holds device memory of the runtime it was made on and
+// was uploaded on that runtime's stream (as was the fast path's field
+// transform): a call on another runtime could read it before its stream has
+// written it, so calls take only handles of their own runtime.
+int check_owner(const tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field,
+                const tgk_domain *boundary = nullptr)
+{
+    if ((domain && domain->rt != rt) || (field && field->rt != rt) || (boundary && boundary->rt != rt))
+        return fail(TGK_ERR_INVALID, "field and domain handles must belong to the runtime of the call");
+    return TGK_OK;
+}
+
 int check_track_args(const tgk_domain *domain, const tgk_field *field, const tgk_algorithm *algorithm, int32_t path,
                      int64_t n_seeds, int64_t batch_size, uint32_t n_points, float save_at)
 {
@@ -719,12 +768,27 @@ int fast_launch(tgk_runtime *rt, tgk_field *field, const tgk_algorithm *algorith
         return r;
     if (int r = reserve(rt, rt->queue, 256, "queue"))
         return r;
+    memset(f, 0, sizeof *f);
     f->track = p;
-    f->queue = rt->queue.as<uint32_t>();
+    char *words = rt->queue.as<char>();
+    f->claims = rt->fast_counters.as<uint32_t>();
+    f->ended = rt->fast_counters.as<uint32_t>() + 32;
+    f->stats = reinterpret_cast<unsigned long long *>(words + FAST_STATS_OFFSET);
     f->precision = field->fast_precision;
     f->coeffs = field->fast.ptr;
     f->coeff_stride = field->fast_stride;
     f->sh_order = sh_order;
+    f->min_repack = rt->fast_min_repack;
+    // Room for one generation's saved streamlines: one per lane of the grid.
+    int blocks = 0;
+    TGK_HIP(tgk::prepare_track_fast(*f, rt->fast_grid, &blocks), "fast kernel occupancy");
+    const size_t bytes = (size_t)blocks * 256 * tgk::FAST_CONTINUATION_VEC4 * sizeof(float4);
+    for (int i = 0; i < 2; i++) {
+        if (int r = reserve(rt, rt->continuations[i], bytes, "saved streamlines"))
+            return r;
+        f->cont[i] = rt->continuations[i].as<float4>();
+        f->cont_count[i] = reinterpret_cast<uint32_t *>(words + FAST_CONT_COUNT_OFFSET + 4 * i);
+    }
     return TGK_OK;
 }
 
@@ -759,6 +823,8 @@ int tgk_track_device(tgk_runtime *rt, const tgk_domain *domain, const tgk_field 
     if (!rt)
         return fail(TGK_ERR_INVALID, "runtime is NULL");
     if (int r = check_track_args(domain, field, algorithm, path, n_seeds, batch_size, n_points, save_at))
+        return r;
+    if (int r = check_owner(rt, domain, field))
         return r;
     if (n_seeds > 0 && (!d_seeds || !d_rng_seeds || !d_lengths || !d_steps || !d_hits || !d_dense))
         return fail(TGK_ERR_INVALID, "NULL device buffer");
@@ -824,6 +890,8 @@ int tgk_track(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field,
         return fail(TGK_ERR_INVALID, "NULL argument");
     *out = nullptr;
     if (int r = check_track_args(domain, field, algorithm, path, n_seeds, batch_size, n_points, save_at))
+        return r;
+    if (int r = check_owner(rt, domain, field))
         return r;
     if (n_seeds > 0 && (!seeds || !rng_seeds))
         return fail(TGK_ERR_INVALID, "NULL seeds");
@@ -1025,6 +1093,81 @@ template <class T> int fetch(T *host, const T *dev, size_t count, hipStream_t s)
 
 extern "C" {
 
+int tgk_track_timed(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field, const tgk_algorithm *algorithm,
+                    int32_t path, const float *seeds, int64_t n_seeds, uint32_t n_points, float save_at,
+                    int32_t repeats, float *milliseconds, uint64_t *steps)
+{
+    if (!rt || !milliseconds || !steps)
+        return fail(TGK_ERR_INVALID, "NULL argument");
+    if (int r = check_track_args(domain, field, algorithm, path, n_seeds, 1, n_points, save_at))
+        return r;
+    if (int r = check_owner(rt, domain, field))
+        return r;
+    if (n_seeds <= 0 || !seeds || repeats < 1)
+        return fail(TGK_ERR_INVALID, "timing needs seeds and at least one repeat");
+    if (rt->capturing)
+        return fail(TGK_ERR_INVALID, "timing allocates; not while a graph is being recorded");
+    if (int r = select_device(rt))
+        return r;
+    Staging st;
+    const float *d_seeds;
+    uint32_t *d_rng, *d_lengths, *d_steps, *d_hits;
+    float *d_dense;
+    const uint32_t rng_seed = 0;
+    if (int r = st.in(seeds, (size_t)n_seeds * 8, rt->stream, const_cast<float **>(&d_seeds)))
+        return r;
+    if (int r = st.in(&rng_seed, 1, rt->stream, &d_rng))
+        return r;
+    if (int r = st.out((size_t)n_seeds, &d_lengths))
+        return r;
+    if (int r = st.out((size_t)n_seeds, &d_steps))
+        return r;
+    if (int r = st.out((size_t)n_seeds, &d_hits))
+        return r;
+    if (int r = st.out((size_t)n_seeds * n_points * 3, &d_dense))
+        return r;
+    hipEvent_t start = nullptr, stop = nullptr;
+    TGK_HIP(hipEventCreate(&start), "hipEventCreate");
+    int status = TGK_OK;
+    float best = -1.0f;
+    if (hipEventCreate(&stop) != hipSuccess) {
+        (void)hipEventDestroy(start);
+        return fail(TGK_ERR_DEVICE, "hipEventCreate");
+    }
+    // One untimed launch (field transform, first-use setup), then the timed ones.
+    for (int i = 0; i <= repeats && status == TGK_OK; i++) {
+        if (hipEventRecord(start, rt->stream) != hipSuccess) {
+            status = fail(TGK_ERR_DEVICE, "hipEventRecord");
+            break;
+        }
+        status = run_track(rt, domain, const_cast<tgk_field *>(field), algorithm, path, d_seeds, 0, n_seeds, d_rng,
+                           n_seeds, n_points, save_at, d_lengths, d_steps, d_hits, d_dense);
+        if (status != TGK_OK)
+            break;
+        float ms = 0.0f;
+        if (hipEventRecord(stop, rt->stream) != hipSuccess || hipEventSynchronize(stop) != hipSuccess ||
+            hipEventElapsedTime(&ms, start, stop) != hipSuccess) {
+            status = fail(TGK_ERR_DEVICE, "timing events");
+            break;
+        }
+        if (i > 0 && (best < 0.0f || ms < best))
+            best = ms;
+    }
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+    if (status != TGK_OK)
+        return status;
+    std::vector<uint32_t> counts((size_t)n_seeds);
+    if (int r = fetch(counts.data(), d_steps, counts.size(), rt->stream))
+        return r;
+    uint64_t total = 0;
+    for (uint32_t c : counts)
+        total += c;
+    *milliseconds = best;
+    *steps = total;
+    return TGK_OK;
+}
+
 int tgk_evaluate(tgk_runtime *rt, int32_t model, int32_t n_parameters, const float *params, const float *orientations,
                  int64_t n, float *out)
 {
@@ -1078,6 +1221,8 @@ int tgk_field_evaluate(tgk_runtime *rt, const tgk_field *field, const float *poi
 {
     if (!rt || !field || n < 0 || (n > 0 && (!points || !out)))
         return fail(TGK_ERR_INVALID, "bad arguments");
+    if (int r = check_owner(rt, nullptr, field))
+        return r;
     if (n == 0)
         return TGK_OK;
     if (int r = select_device(rt))
@@ -1097,6 +1242,8 @@ int tgk_domain_contains(tgk_runtime *rt, const tgk_domain *domain, const float *
 {
     if (!rt || !domain || n < 0 || (n > 0 && (!points || !out)))
         return fail(TGK_ERR_INVALID, "bad arguments");
+    if (int r = check_owner(rt, domain, nullptr))
+        return r;
     if (domain->kind != tgk::DOMAIN_MASK)
         return fail(TGK_ERR_UNSUPPORTED, "point membership is defined for mask domains");
     if (n == 0)
@@ -1123,6 +1270,8 @@ int segment_hit(tgk_runtime *rt, const tgk_domain *domain, int method, const flo
 {
     if (!rt || !domain || n < 0 || (n > 0 && (!starts || !ends || !found || !hit || !t)))
         return fail(TGK_ERR_INVALID, "bad arguments");
+    if (int r = check_owner(rt, domain, nullptr))
+        return r;
     if (domain->n_nodes == 0)
         return fail(TGK_ERR_INVALID, "domain has no BVH");
     if (method == 1 && !domain->cell_start.ptr)
@@ -1204,6 +1353,8 @@ int tgk_connectome(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *
         return fail(TGK_ERR_INVALID, "%lld seeds: at most %lld per call", (long long)n_seeds, (long long)MAX_SEEDS);
     if (int r = check_algorithm(algorithm, field, path == TGK_PATH_FAST ? 91 : 45))
         return r;
+    if (int r = check_owner(rt, domain, field, boundary))
+        return r;
     if (domain->kind != tgk::DOMAIN_MASK)
         return fail(TGK_ERR_INVALID, "the connectome's domain must be a mask domain");
     if (boundary->kind != tgk::DOMAIN_MESH || boundary->n_nodes == 0 || !boundary->cumulative.ptr ||
@@ -1277,6 +1428,8 @@ int tod_launch(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boun
     if (n_seeds > MAX_SEEDS)
         return fail(TGK_ERR_INVALID, "%lld seeds: at most %lld per call", (long long)n_seeds, (long long)MAX_SEEDS);
     if (int r = check_algorithm(algorithm, field))
+        return r;
+    if (int r = check_owner(rt, domain, field, boundary))
         return r;
     if (domain->kind != tgk::DOMAIN_MASK)
         return fail(TGK_ERR_INVALID, "the TOD's domain must be a mask domain");
@@ -1392,6 +1545,8 @@ int tgk_domain_seed(tgk_runtime *rt, const tgk_domain *domain, uint32_t rng_seed
 {
     if (!rt || !domain || n < 0 || (n > 0 && !out))
         return fail(TGK_ERR_INVALID, "bad arguments");
+    if (int r = check_owner(rt, domain, nullptr))
+        return r;
     if (domain->n_triangles == 0 || !domain->cumulative.ptr)
         return fail(TGK_ERR_INVALID, "domain has no boundary surface");
     if (n == 0)
@@ -1532,6 +1687,8 @@ int tgk_track_prepare(tgk_runtime *rt, const tgk_domain *domain, const tgk_field
         return fail(TGK_ERR_INVALID, "prepare before recording a graph");
     if (int r = check_track_args(domain, field, algorithm, path, max_seeds, 1, 1, 1.0f))
         return r;
+    if (int r = check_owner(rt, domain, field))
+        return r;
     if (int r = select_device(rt))
         return r;
     if (int r = ensure_workspace(rt, tgk::offsets_workspace_bytes(max_seeds)))
@@ -1542,7 +1699,6 @@ int tgk_track_prepare(tgk_runtime *rt, const tgk_domain *domain, const tgk_field
         tgk::FastLaunch f;
         if (int r = fast_launch(rt, const_cast<tgk_field *>(field), algorithm, p, &f))
             return r;
-        TGK_HIP(tgk::prepare_track_fast(f), "fast kernel occupancy");
     }
     TGK_HIP(hipStreamSynchronize(rt->stream), "track preparation");
     return TGK_OK;

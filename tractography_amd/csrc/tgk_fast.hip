@@ -60,11 +60,40 @@ struct Lane {
     uint2 rng;
     double time;
     uint32_t n, steps;
-    int64_t field_cell; // cached cell ids (-2: none)
+    int64_t field_cell; // cell ids of `point` (-2: none yet); their data is requested by request_cells
     int64_t mask_cell;
-    bool mask_inside;
-    bool first;
+    uint32_t mask_value; // the mask at mask_cell (0 outside the mask image)
+    bool field_ok;       // `point` is inside the field image
+    uint32_t row_loads;  // coefficient rows and mask values this lane read from memory
+    uint32_t mask_loads;
 };
+
+// A streamline in flight when a generation of the persistent launch ends
+// (see track_fast_kernel): everything its lane needs to resume it, in four
+// float4. The cell data are requested again on resumption.
+__device__ __forceinline__ void save_lane(float4 *slot, const Lane &L)
+{
+    const uint64_t tb = (uint64_t)__double_as_longlong(L.time);
+    slot[0] = make_float4(L.point.x, L.point.y, L.point.z, L.o.x);
+    slot[1] = make_float4(L.o.y, L.o.z, L.previous.x, L.previous.y);
+    slot[2] = make_float4(L.previous.z, __uint_as_float((uint32_t)tb), __uint_as_float((uint32_t)(tb >> 32)),
+                          __uint_as_float((uint32_t)L.g));
+    slot[3] = make_float4(__uint_as_float(L.n), __uint_as_float(L.steps), __uint_as_float(L.rng.x),
+                          __uint_as_float(L.rng.y));
+}
+
+__device__ __forceinline__ void resume_lane(const float4 *slot, Lane &L)
+{
+    const float4 a = slot[0], b = slot[1], c = slot[2], d = slot[3];
+    L.point = v3(a.x, a.y, a.z);
+    L.o = v3(a.w, b.x, b.y);
+    L.previous = v3(b.z, b.w, c.x);
+    L.time = __longlong_as_double((long long)(((uint64_t)__float_as_uint(c.z) << 32) | __float_as_uint(c.y)));
+    L.g = (int64_t)__float_as_uint(c.w);
+    L.n = __float_as_uint(d.x);
+    L.steps = __float_as_uint(d.y);
+    L.rng = make_uint2(__float_as_uint(d.z), __float_as_uint(d.w));
+}
 
 // One iteration of the reference loop (tracking.py:121-147) for a lane with
 // a streamline in flight; on termination the lane's results are written and
@@ -95,9 +124,17 @@ __device__ __forceinline__ void load_row(const float *__restrict__ coeffs, int64
     constexpr int LDS_VEC = lds_vectors<MODEL, LMAX>();
     const float4 *row = reinterpret_cast<const float4 *>(coeffs + cell * (int64_t)cstride);
     if constexpr (MODEL == MODEL_SH) {
+        // Straight from memory into LDS (no registers held while the loads
+        // are in flight): each lane's 16 bytes land at the instruction's
+        // wave-uniform LDS base + 16 * lane, i.e. column j of the lane's
+        // slot. Lanes that do not reload are masked off and keep theirs.
+        float4 *wave_rows = lds_rows + (threadIdx.x & ~63u);
 #pragma unroll
         for (int j = 0; j < LDS_VEC; j++)
-            lds_rows[j * fast_block<LMAX>() + threadIdx.x] = row[j];
+            __builtin_amdgcn_global_load_lds(static_cast<const void *>(row + j),
+                                             (__attribute__((address_space(3))) void *)(wave_rows +
+                                                                                        j * fast_block<LMAX>()),
+                                             16, 0, 0);
 #pragma unroll
         for (int j = LDS_VEC; j < VEC; j++) {
             const float4 v = row[j];
@@ -128,6 +165,10 @@ __device__ __forceinline__ void read_row(float *q, const float4 *__restrict__ ld
     constexpr int TERMS = Order<LMAX>::TERMS;
     constexpr int LDS_VEC = lds_vectors<MODEL, LMAX>();
     if constexpr (rows_in_lds<MODEL, LMAX>()) {
+        // Loads into LDS (load_row) complete under vmcnt, which the compiler
+        // does not wait on before these reads of the same LDS: wait here
+        // (s_waitcnt vmcnt(0); expcnt and lgkmcnt left at their maxima).
+        __builtin_amdgcn_s_waitcnt(0x0F70);
 #pragma unroll
         for (int j = 0; j < LDS_VEC; j++) {
             const float4 v = lds_rows[j * fast_block<LMAX>() + threadIdx.x];
@@ -143,37 +184,72 @@ __device__ __forceinline__ void read_row(float *q, const float4 *__restrict__ ld
     }
 }
 
-// One orientation update on the fast model.
+// One orientation update on the fast model. Transport and Diffusion take
+// their normal draws (fast::draw_noise) from the caller, already drawn;
+// Probabilistic draws from `rng` as it goes.
 template <int MODEL, int ALGO, int LMAX>
-__device__ __forceinline__ V3 update_orientation(const AlgoParams &a, const float *q, V3 o, uint2 &rng)
+__device__ __forceinline__ V3 update_orientation(const AlgoParams &a, const float *q, V3 o, uint2 &rng,
+                                                 const fast::Noise &nz)
 {
     if constexpr (ALGO == ALGO_PROBABILISTIC) {
         return fast::probabilistic<MODEL, LMAX>(a, q, o, rng);
     } else if constexpr (ALGO == ALGO_DETERMINISTIC) {
         return fast::deterministic<MODEL, LMAX>(a, q, o);
     } else if constexpr (MODEL == MODEL_SH) {
-        return fast::sh_update<ALGO, LMAX>(a, q, o, rng);
+        return fast::sh_update<ALGO, LMAX>(a, q, o, nz);
     } else {
         const fast::Direction dir = fast::direction_of(o);
-        return fast::update<ALGO>(a, fast::dti_model(q, o, dir), dir, o, rng);
+        return fast::update<ALGO>(a, fast::dti_model(q, o, dir), dir, o, nz);
     }
 }
 
+template <int MODEL, int ALGO, int LMAX>
+__device__ __forceinline__ V3 update_orientation(const AlgoParams &a, const float *q, V3 o, uint2 &rng)
+{
+    const fast::Noise nz = fast::draw_noise<ALGO>(a, rng);
+    return update_orientation<MODEL, ALGO, LMAX>(a, q, o, rng, nz);
+}
+
+// Requests the data of L.point's cells one iteration before the loop needs
+// it: the mask value when the mask cell changes, the coefficient row when
+// the field cell changes (into LDS or registers, see load_row). Their
+// memory latency then overlaps the rest of the step, and of the next one
+// up to the model evaluation, instead of stalling the wavefront, which
+// nearly every step holds at least one lane entering a new cell.
+template <int MODEL, int DOMAIN, int LMAX>
+__device__ __forceinline__ void request_cells(const TrackLaunch &p, const float *__restrict__ coeffs, uint32_t cstride,
+                                              Lane &L, float *q, float4 *__restrict__ lds_rows)
+{
+    if constexpr (DOMAIN == DOMAIN_MASK) {
+        const int64_t cell = mask_cell(p.mask, L.point);
+        if (cell != L.mask_cell) {
+            L.mask_cell = cell;
+            L.mask_value = cell >= 0 ? (uint32_t)p.mask.mask[cell] : 0u;
+            L.mask_loads += cell >= 0 ? 1u : 0u;
+        }
+    }
+    const int64_t cell = field_voxel(p.field, L.point);
+    L.field_ok = cell >= 0;
+    if (cell >= 0 && cell != L.field_cell) {
+        L.field_cell = cell;
+        load_row<MODEL, LMAX>(coeffs, cell, cstride, q, lds_rows);
+        L.row_loads++;
+    }
+}
+
+// nz: this step's Diffusion draws, made by the caller before anything that
+// waits on memory, so they cover the latency of the cell data
+// request_cells asked for.
 template <int MODEL, int ALGO, int DOMAIN, int LMAX>
 __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__restrict__ coeffs, uint32_t cstride,
-                                        Lane &L, float *q, float4 *__restrict__ lds_rows)
+                                        Lane &L, float *q, float4 *__restrict__ lds_rows, const fast::Noise &nz)
 {
     bool stop = !(L.n < p.n_points && L.steps < p.max_steps);
     uint32_t hit = NO_HIT;
     if (!stop) {
         if constexpr (DOMAIN == DOMAIN_MASK) {
-            const int64_t cell = mask_cell(p.mask, L.point);
-            if (cell != L.mask_cell) {
-                L.mask_cell = cell;
-                L.mask_inside = cell >= 0 && p.mask.mask[cell] != 0;
-            }
-            stop = !L.mask_inside;
-        } else if (!L.first) {
+            stop = L.mask_value == 0u;
+        } else if (L.steps > 0) {
             // The start cell's candidate list answers segments up to `reach`
             // long with a few independent loads; longer segments, or starts
             // off the grid, traverse the BVH.
@@ -194,11 +270,8 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
             }
         }
     }
-    int64_t cell = -1;
-    if (!stop) {
-        cell = field_voxel(p.field, L.point);
-        stop = cell < 0;
-    }
+    if (!stop)
+        stop = !L.field_ok;
     if (stop) {
         p.lengths[L.g] = L.n;
         p.steps[L.g] = L.steps;
@@ -206,17 +279,12 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
         L.g = -1;
         return;
     }
-    if (cell != L.field_cell) {
-        L.field_cell = cell;
-        load_row<MODEL, LMAX>(coeffs, cell, cstride, q, lds_rows);
-    }
     read_row<MODEL, LMAX>(q, lds_rows);
-    L.o = update_orientation<MODEL, ALGO, LMAX>(p.algo, q, L.o, L.rng);
+    L.o = update_orientation<MODEL, ALGO, LMAX>(p.algo, q, L.o, L.rng, nz);
     L.steps++;
     L.previous = L.point;
     const float step = p.algo.step_size;
     L.point = v3(L.point.x + step * L.o.x, L.point.y + step * L.o.y, L.point.z + step * L.o.z);
-    L.first = false;
     L.time += (double)step;
     if (L.time >= (double)p.save_at) {
         L.time -= (double)p.save_at;
@@ -224,6 +292,7 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
         store_row(p.points, L.g, p.n_points, L.n, v3(L.point.x - t * L.o.x, L.point.y - t * L.o.y, L.point.z - t * L.o.z));
         L.n++;
     }
+    request_cells<MODEL, DOMAIN, LMAX>(p, coeffs, cstride, L, q, lds_rows);
 }
 
 // Waves per SIMD the compiler must fit: the order-8 SH Deterministic kernel
@@ -238,6 +307,17 @@ template <int MODEL, int ALGO, int LMAX> constexpr int fast_min_waves()
     return MODEL == MODEL_SH && LMAX == 8 && ALGO == ALGO_DETERMINISTIC ? 4 : 1;
 }
 
+// The persistent launch runs in generations (launch_fast): generation 0
+// claims seeds; each later one claims the streamlines the previous one left
+// in flight. Once all of a generation's work is claimed, lanes go idle as
+// their streamlines end, while a wavefront keeps issuing for its longest
+// one: on the 1M-seed benchmark workload half the launch would run with
+// its lanes ever sparser. So when the generation's live lanes fall below
+// `stop_below` (half its work), every wavefront saves its live lanes and
+// ends, and the next generation resumes them packed into full wavefronts,
+// a quarter or half as many per SIMD, each running correspondingly faster.
+// The last generation, or one with little work, runs to completion. Which
+// lane tracks which streamline, and when, changes no result.
 template <int MODEL, int ALGO, int DOMAIN, int LMAX>
 __global__ __launch_bounds__(fast_block<LMAX>())
 __attribute__((amdgpu_waves_per_eu(fast_min_waves<MODEL, ALGO, LMAX>())))
@@ -249,50 +329,151 @@ void track_fast_kernel(FastLaunch f)
     const float *coeffs = static_cast<const float *>(f.coeffs);
     const uint32_t cstride = f.coeff_stride;
 
+    // This generation's work: seeds, or the previous generation's saved
+    // streamlines, claimed through f.claims. Streamlines that end or are
+    // saved are counted in f.ended: FAST_END_SHARDS counters a cache line
+    // apart, each wavefront adding to one of them in batches, so that no
+    // single word takes every wavefront's updates.
+    const bool resume = f.cont_in != nullptr;
+    const uint32_t n_work = resume ? *f.cont_in_count : (uint32_t)p.n_seeds;
+    uint32_t stop_below = 0;
+    if (!f.last) {
+        const uint64_t lanes = (uint64_t)gridDim.x * blockDim.x;
+        const uint32_t half = (uint32_t)((resume ? (uint64_t)n_work : min((uint64_t)n_work, lanes)) / 2);
+        stop_below = half >= f.min_repack ? half : 0u;
+    }
+    uint32_t *const my_shard =
+        f.ended + ((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % FAST_END_SHARDS) * FAST_END_STRIDE;
+
     Lane L;
     L.g = -1;
+    L.row_loads = 0;
+    L.mask_loads = 0;
     float q[MODEL == MODEL_SH ? Order<LMAX>::TERMS : 6];
     bool drained = false;
+    uint32_t ended_pending = 0;     // ended here and not yet added to the shard
+    uint32_t iteration = 0;
+    uint32_t live_seen = 0xffffffffu;  // the generation's streamlines in flight at the last count
+    bool counting = false;          // shard reads issued, to be summed next iteration
+    uint32_t shard_value = 0;
 
     while (true) {
-        // Refill idle lanes from the queue, one atomic per wavefront.
+        // Refill idle lanes, one atomic per wavefront.
         const uint64_t need = __ballot(L.g < 0);
         if (need != 0 && !drained) {
             const int leader = __ffsll((unsigned long long)need) - 1;
             const uint32_t count = (uint32_t)__popcll(need);
             uint32_t base = 0;
             if (lane_id == leader)
-                base = atomicAdd(f.queue, count);
+                base = atomicAdd(f.claims, count);
             base = __shfl(base, leader);
-            if (base + count >= (uint64_t)p.n_seeds)
+            if ((uint64_t)base + count >= (uint64_t)n_work)
                 drained = true;
             if (L.g < 0) {
                 const uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane_id) - 1ull));
-                const uint64_t g = (uint64_t)base + rank;
-                if (g < (uint64_t)p.n_seeds) {
-                    L.g = (int64_t)g;
-                    const float4 s0 = reinterpret_cast<const float4 *>(p.seeds)[2 * g];
-                    const float4 s1 = reinterpret_cast<const float4 *>(p.seeds)[2 * g + 1];
-                    L.point = v3(s0.x, s0.y, s0.z);
-                    L.o = v3(s1.x, s1.y, s1.z);
-                    L.previous = L.point;
-                    if constexpr (ALGO == ALGO_DIFFUSION || ALGO == ALGO_PROBABILISTIC)
-                        L.rng = seed_stream(p, p.first_seed + L.g);
-                    L.time = 0.0;
-                    L.n = 1;
-                    L.steps = 0;
+                const uint64_t j = (uint64_t)base + rank;
+                if (j < (uint64_t)n_work) {
+                    if (resume) {
+                        resume_lane(f.cont_in + FAST_CONTINUATION_VEC4 * j, L);
+                    } else {
+                        const uint64_t g = j;
+                        L.g = (int64_t)g;
+                        const float4 s0 = reinterpret_cast<const float4 *>(p.seeds)[2 * g];
+                        const float4 s1 = reinterpret_cast<const float4 *>(p.seeds)[2 * g + 1];
+                        L.point = v3(s0.x, s0.y, s0.z);
+                        L.o = v3(s1.x, s1.y, s1.z);
+                        L.previous = L.point;
+                        if constexpr (ALGO == ALGO_DIFFUSION || ALGO == ALGO_PROBABILISTIC)
+                            L.rng = seed_stream(p, p.first_seed + L.g);
+                        L.time = 0.0;
+                        L.n = 1;
+                        L.steps = 0;
+                        store_row(p.points, L.g, p.n_points, 0, L.point);
+                    }
                     L.field_cell = -2;
                     L.mask_cell = -2;
-                    L.mask_inside = false;
-                    L.first = true;
-                    store_row(p.points, L.g, p.n_points, 0, L.point);
+                    L.mask_value = 0;
+                    request_cells<MODEL, DOMAIN, LMAX>(p, coeffs, cstride, L, q, lds_rows);
                 }
             }
         }
-        if (__ballot(L.g >= 0) == 0)
+        const uint64_t live = __ballot(L.g >= 0);
+        if (live == 0)
             break;
+        // The draws need no memory: made first, they cover the latency of
+        // the cell data and shard counts requested last iteration. The
+        // stream before them is what a saved streamline resumes from.
+        // Only live lanes draw: an idle lane's stream is not initialised (an
+        // all-zero MWC64X state never leaves zero, and randn would spin).
+        const uint2 rng_before = L.rng;
+        fast::Noise nz{0.0f, 0.0f};
         if (L.g >= 0)
-            advance<MODEL, ALGO, DOMAIN, LMAX>(p, coeffs, cstride, L, q, lds_rows);
+            nz = fast::draw_noise<ALGO>(p.algo, L.rng);
+        if (counting) {
+            // Every claim was made before the shards were read, so the
+            // streamlines in flight are the work minus those that left.
+            uint32_t left = shard_value;
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1)
+                left += __shfl_xor(left, d);
+            live_seen = n_work - min(left, n_work);
+            counting = false;
+        }
+        if (live_seen < stop_below) {
+            // Few streamlines left in flight: end the generation, handing
+            // this wavefront's to the next one.
+            const int leader = __ffsll((unsigned long long)live) - 1;
+            const uint32_t count = (uint32_t)__popcll(live);
+            uint32_t base = 0;
+            if (lane_id == leader) {
+                base = atomicAdd(f.cont_out_count, count);
+                atomicAdd(my_shard, ended_pending + count);
+            }
+            base = __shfl(base, leader);
+            if (L.g >= 0) {
+                L.rng = rng_before;
+                save_lane(f.cont_out +
+                              FAST_CONTINUATION_VEC4 *
+                                  ((uint64_t)base + (uint32_t)__popcll(live & ((1ull << lane_id) - 1ull))),
+                          L);
+            }
+            ended_pending = 0;
+            break;
+        }
+        if (L.g >= 0)
+            advance<MODEL, ALGO, DOMAIN, LMAX>(p, coeffs, cstride, L, q, lds_rows, nz);
+        ended_pending += (uint32_t)__popcll(live & ~__ballot(L.g >= 0));
+        // Once every claim is made, every eighth iteration: add this
+        // wavefront's ended streamlines to its shard and read all shards
+        // (summed next iteration, after the draws).
+        if (stop_below != 0 && drained && (++iteration & 7u) == 0u) {
+            if (ended_pending != 0 && lane_id == 0)
+                atomicAdd(my_shard, ended_pending);
+            ended_pending = 0;
+            shard_value = lane_id < FAST_END_SHARDS
+                              ? __hip_atomic_load(f.ended + lane_id * FAST_END_STRIDE, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : 0u;
+            counting = true;
+        }
+    }
+    if (stop_below != 0 && ended_pending != 0 && lane_id == 0)
+        atomicAdd(my_shard, ended_pending);
+    // A lane that stopped may have requested a row for its last point; the
+    // wavefront waits for every load into LDS before it ends, so none lands
+    // in LDS another workgroup has been given.
+    __builtin_amdgcn_s_waitcnt(0);
+    // The wavefront's memory reads of rows and mask values, one atomic each
+    // (the launch's totals price the bytes the cell cache had to fetch).
+    uint64_t rows = L.row_loads, masks = L.mask_loads;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        rows += __shfl_xor(rows, d);
+        masks += __shfl_xor(masks, d);
+    }
+    if (lane_id == 0) {
+        atomicAdd(&f.stats[0], (unsigned long long)rows);
+        atomicAdd(&f.stats[1], (unsigned long long)masks);
     }
 }
 
@@ -408,6 +589,24 @@ __global__ __launch_bounds__(fast_block<LMAX>()) void connectome_fast_kernel(Fas
 }
 
 __global__ void zero_counter(uint32_t *counter) { *counter = 0u; }
+
+// A generation's claim counter, ended-streamline shards and saved-streamline
+// count; before generation 0 also the launch's row and mask read totals.
+// One block of 64 threads.
+__global__ void zero_generation(uint32_t *claims, uint32_t *ended, uint32_t *cont_count, unsigned long long *stats)
+{
+    const int t = threadIdx.x;
+    if (t < FAST_END_SHARDS)
+        ended[t * FAST_END_STRIDE] = 0u;
+    if (t == 0) {
+        *claims = 0u;
+        *cont_count = 0u;
+        if (stats) {
+            stats[0] = 0ull;
+            stats[1] = 0ull;
+        }
+    }
+}
 
 // Dense seed-major rows -> ragged points. One wavefront per streamline copies
 // its lengths[g] * 3 floats, coalesced on both sides, eight loads in flight
@@ -578,55 +777,69 @@ template <int MODEL, int ALGO, int DOMAIN, int LMAX> hipError_t occupancy_grid(i
     return hipSuccess;
 }
 
+// query: only report the grid (*blocks) a launch would use.
 template <int MODEL, int ALGO, int DOMAIN, int LMAX>
-hipError_t launch_fast(const FastLaunch &f, int grid_blocks, bool query, hipStream_t stream)
+hipError_t launch_fast(const FastLaunch &f, int grid_blocks, bool query, int *blocks, hipStream_t stream)
 {
-    if (grid_blocks <= 0 || query) {
-        int blocks = 0;
-        const hipError_t e = occupancy_grid<MODEL, ALGO, DOMAIN, LMAX>(&blocks);
-        if (e != hipSuccess || query)
+    if (grid_blocks <= 0) {
+        const hipError_t e = occupancy_grid<MODEL, ALGO, DOMAIN, LMAX>(&grid_blocks);
+        if (e != hipSuccess)
             return e;
-        grid_blocks = blocks;
     }
-    zero_counter<<<1, 1, 0, stream>>>(f.queue);
-    track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX><<<grid_blocks, fast_block<LMAX>(), 0, stream>>>(f);
+    if (blocks)
+        *blocks = grid_blocks;
+    if (query)
+        return hipSuccess;
+    // Generation g resumes what g - 1 saved in cont[(g - 1) % 2] and saves
+    // into cont[g % 2], whose count (generation g - 2's, consumed by g - 1)
+    // it clears first.
+    FastLaunch gen = f;
+    for (int g = 0; g < FAST_GENERATIONS; g++) {
+        gen.cont_in = g == 0 ? nullptr : f.cont[(g - 1) % 2];
+        gen.cont_in_count = g == 0 ? nullptr : f.cont_count[(g - 1) % 2];
+        gen.cont_out = f.cont[g % 2];
+        gen.cont_out_count = f.cont_count[g % 2];
+        gen.last = g == FAST_GENERATIONS - 1;
+        zero_generation<<<1, 64, 0, stream>>>(f.claims, f.ended, gen.cont_out_count, g == 0 ? f.stats : nullptr);
+        track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX><<<grid_blocks, fast_block<LMAX>(), 0, stream>>>(gen);
+    }
     return hipGetLastError();
 }
 
 template <int MODEL, int ALGO, int DOMAIN>
-hipError_t launch_fast_order(const FastLaunch &f, int grid, bool query, hipStream_t stream)
+hipError_t launch_fast_order(const FastLaunch &f, int grid, bool query, int *blocks, hipStream_t stream)
 {
     if constexpr (MODEL == MODEL_SH) {
         if (f.sh_order == 12)
-            return launch_fast<MODEL, ALGO, DOMAIN, 12>(f, grid, query, stream);
+            return launch_fast<MODEL, ALGO, DOMAIN, 12>(f, grid, query, blocks, stream);
     }
-    return launch_fast<MODEL, ALGO, DOMAIN, 8>(f, grid, query, stream);
+    return launch_fast<MODEL, ALGO, DOMAIN, 8>(f, grid, query, blocks, stream);
 }
 
 template <int MODEL, int ALGO>
-hipError_t launch_fast_domain(const FastLaunch &f, int grid, bool query, hipStream_t stream)
+hipError_t launch_fast_domain(const FastLaunch &f, int grid, bool query, int *blocks, hipStream_t stream)
 {
     if (f.track.domain_kind == DOMAIN_MESH)
-        return launch_fast_order<MODEL, ALGO, DOMAIN_MESH>(f, grid, query, stream);
-    return launch_fast_order<MODEL, ALGO, DOMAIN_MASK>(f, grid, query, stream);
+        return launch_fast_order<MODEL, ALGO, DOMAIN_MESH>(f, grid, query, blocks, stream);
+    return launch_fast_order<MODEL, ALGO, DOMAIN_MASK>(f, grid, query, blocks, stream);
 }
 
 template <int MODEL>
-hipError_t launch_fast_algo(const FastLaunch &f, int grid, bool query, hipStream_t stream)
+hipError_t launch_fast_algo(const FastLaunch &f, int grid, bool query, int *blocks, hipStream_t stream)
 {
     switch (f.track.algo.algorithm) {
-    case ALGO_DIFFUSION: return launch_fast_domain<MODEL, ALGO_DIFFUSION>(f, grid, query, stream);
-    case ALGO_PROBABILISTIC: return launch_fast_domain<MODEL, ALGO_PROBABILISTIC>(f, grid, query, stream);
-    case ALGO_DETERMINISTIC: return launch_fast_domain<MODEL, ALGO_DETERMINISTIC>(f, grid, query, stream);
-    default: return launch_fast_domain<MODEL, ALGO_TRANSPORT>(f, grid, query, stream);
+    case ALGO_DIFFUSION: return launch_fast_domain<MODEL, ALGO_DIFFUSION>(f, grid, query, blocks, stream);
+    case ALGO_PROBABILISTIC: return launch_fast_domain<MODEL, ALGO_PROBABILISTIC>(f, grid, query, blocks, stream);
+    case ALGO_DETERMINISTIC: return launch_fast_domain<MODEL, ALGO_DETERMINISTIC>(f, grid, query, blocks, stream);
+    default: return launch_fast_domain<MODEL, ALGO_TRANSPORT>(f, grid, query, blocks, stream);
     }
 }
 
-hipError_t dispatch_fast(const FastLaunch &f, int grid, bool query, hipStream_t stream)
+hipError_t dispatch_fast(const FastLaunch &f, int grid, bool query, int *blocks, hipStream_t stream)
 {
     if (f.track.algo.model == MODEL_DTI)
-        return launch_fast_algo<MODEL_DTI>(f, grid, query, stream);
-    return launch_fast_algo<MODEL_SH>(f, grid, query, stream);
+        return launch_fast_algo<MODEL_DTI>(f, grid, query, blocks, stream);
+    return launch_fast_algo<MODEL_SH>(f, grid, query, blocks, stream);
 }
 
 // Connectome launches: the persistent grid of the kernel's occupancy
@@ -691,10 +904,15 @@ hipError_t launch_track_fast(const FastLaunch &f, int grid_blocks, hipStream_t s
 {
     if (f.track.n_seeds == 0)
         return hipSuccess;
-    return dispatch_fast(f, grid_blocks, false, stream);
+    if (f.track.n_seeds >= (int64_t)1 << 31)
+        return hipErrorInvalidValue;
+    return dispatch_fast(f, grid_blocks, false, nullptr, stream);
 }
 
-hipError_t prepare_track_fast(const FastLaunch &f) { return dispatch_fast(f, 0, true, nullptr); }
+hipError_t prepare_track_fast(const FastLaunch &f, int grid_blocks, int *blocks)
+{
+    return dispatch_fast(f, grid_blocks, true, blocks, nullptr);
+}
 
 hipError_t launch_connectome_fast(const FastConnectomeLaunch &f, hipStream_t stream)
 {

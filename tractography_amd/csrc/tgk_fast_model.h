@@ -309,8 +309,31 @@ __device__ __forceinline__ V3 exps2(V3 p, V3 x)
     return v3(c * p.x + sn * x.x, c * p.y + sn * x.y, c * p.z + sn * x.z);
 }
 
+// The two normal draws of a Diffusion update (algorithm.py:217-220), in the
+// reference's order; none for Transport or a zero noise variance, which
+// consume no random numbers. Drawn apart from the update so that a step can
+// draw them before its memory-dependent work: the draws depend only on the
+// lane's stream, and a lane whose streamline stops in that step discards
+// its stream.
+struct Noise {
+    float r1, r2;
+};
+
+template <int ALGO> __device__ __forceinline__ Noise draw_noise(const AlgoParams &a, uint2 &rng)
+{
+    Noise n{0.0f, 0.0f};
+    if constexpr (ALGO == ALGO_DIFFUSION) {
+        if (a.noise_variance > 0.0f) {
+            n.r1 = randn(rng);
+            n.r2 = randn(rng);
+        }
+    }
+    return n;
+}
+
 template <int ALGO>
-__device__ __forceinline__ V3 update(const AlgoParams &a, const ModelOut &mv, const Direction &dir, V3 o, uint2 &rng)
+__device__ __forceinline__ V3 update(const AlgoParams &a, const ModelOut &mv, const Direction &dir, V3 o,
+                                     const Noise &nz)
 {
     const float etx = dir.z * dir.c1, ety = dir.z * dir.s1, etz = -dir.s;
     const float epx = -dir.s1, epy = dir.c1;
@@ -321,8 +344,7 @@ __device__ __forceinline__ V3 update(const AlgoParams &a, const ModelOut &mv, co
     float tz = scale * (mv.dtheta * etz);
     if constexpr (ALGO == ALGO_DIFFUSION) {
         if (a.noise_variance > 0.0f) {
-            const float r1 = randn(rng);
-            const float r2 = randn(rng);
+            const float r1 = nz.r1, r2 = nz.r2;
             const float sn = __builtin_sqrtf(a.noise_variance * a.gamma * a.step_size);
             tx += sn * (r1 * etx + r2 * epx);
             ty += sn * (r1 * ety + r2 * epy);
@@ -337,7 +359,7 @@ __device__ __forceinline__ V3 update(const AlgoParams &a, const ModelOut &mv, co
 // (e_theta, e_phi) is formed only for the Diffusion noise, which is drawn
 // in it.
 template <int ALGO, int L>
-__device__ __forceinline__ V3 sh_update(const AlgoParams &a, const float *q, V3 o, uint2 &rng)
+__device__ __forceinline__ V3 sh_update(const AlgoParams &a, const float *q, V3 o, const Noise &nz)
 {
     const V3 n = unit_direction(o);
     const Cartesian g = sh_cartesian<L>(q, n);
@@ -360,8 +382,7 @@ __device__ __forceinline__ V3 sh_update(const AlgoParams &a, const float *q, V3 
                 sin_theta = rho2 * inv_rho;
             }
             const float z = fminf(fmaxf(n.z, -1.0f), 1.0f);
-            const float r1 = randn(rng);
-            const float r2 = randn(rng);
+            const float r1 = nz.r1, r2 = nz.r2;
             const float sn = __builtin_sqrtf(a.noise_variance * a.gamma * a.step_size);
             tx += sn * (r1 * (z * c1) - r2 * s1);
             ty += sn * (r1 * (z * s1) + r2 * c1);

@@ -74,22 +74,49 @@ hipError_t launch_pad_rows(const float *src, int64_t n_rows, uint32_t K, float *
 // reference utils.py:6-22 (normalize_odf) on n_cells rows of 45, in place.
 hipError_t launch_normalize_odf(float *coefficients, int64_t n_cells, hipStream_t stream);
 
-// Fast path (tgk_fast.hip): persistent lanes with refill, seed-major output.
+// Fast path (tgk_fast.hip): persistent lanes with refill, seed-major output,
+// run as FAST_GENERATIONS launches (generation 0 claims seeds, each later
+// one resumes the streamlines the previous one saved; see track_fast_kernel).
+constexpr int FAST_GENERATIONS = 6;
+// By default a generation stops early only while it has at least twice
+// this many streamlines; below that, repacking them gains too little.
+constexpr uint32_t FAST_MIN_REPACK = 16384;
+// float4s per saved streamline.
+constexpr int FAST_CONTINUATION_VEC4 = 4;
+// Counters of streamlines that left a generation: FAST_END_SHARDS u32, one
+// per FAST_END_STRIDE u32 (128 bytes), after a first 128-byte line holding
+// the claim counter.
+constexpr int FAST_END_SHARDS = 32;
+constexpr int FAST_END_STRIDE = 32;
+constexpr size_t FAST_COUNTER_BYTES = 128 + (size_t)FAST_END_SHARDS * FAST_END_STRIDE * 4;
 struct FastLaunch {
     TrackLaunch track;
-    uint32_t *queue;        // one u32 work counter, zeroed before launch
-    int32_t precision;      // TGK_PRECISION_*
-    const void *coeffs;     // transformed field (layout per precision)
-    uint32_t coeff_stride;  // elements per voxel in `coeffs`
-    int32_t sh_order;       // 8 (K <= 45) or 12 (K <= 91): polynomial layout of `coeffs`
+    uint32_t *claims;           // a generation's claim counter (zeroed per generation)
+    uint32_t *ended;            // its ended-streamline shards (zeroed per generation)
+    unsigned long long *stats;  // [0] coefficient rows, [1] mask values read from memory (zeroed before generation 0)
+    float4 *cont[2];            // saved streamlines, FAST_CONTINUATION_VEC4 float4 each; room for the grid's lanes
+    uint32_t *cont_count[2];    // entries in cont[i]
+    // Set per generation by launch_track_fast:
+    const float4 *cont_in;      // the previous generation's saved streamlines (nullptr: claim seeds)
+    const uint32_t *cont_in_count;
+    float4 *cont_out;
+    uint32_t *cont_out_count;
+    int32_t last;               // the last generation runs every streamline to its end
+    uint32_t min_repack;        // a generation stops early only while half its work is at least this
+    int32_t precision;          // TGK_PRECISION_*
+    const void *coeffs;         // transformed field (layout per precision)
+    uint32_t coeff_stride;      // elements per voxel in `coeffs`
+    int32_t sh_order;           // 8 (K <= 45) or 12 (K <= 91): polynomial layout of `coeffs`
 };
 // grid_blocks <= 0 sizes the persistent grid to the device's occupancy
-// (computed once per kernel and device, then cached).
+// (computed once per kernel and device, then cached). n_seeds must be below
+// 2^31.
 hipError_t launch_track_fast(const FastLaunch &p, int grid_blocks, hipStream_t stream);
 // Computes and caches that occupancy for the kernel `p` selects, launching
 // nothing (so later launches make no device queries, e.g. while a graph is
-// being recorded).
-hipError_t prepare_track_fast(const FastLaunch &p);
+// being recorded); *blocks receives the grid a launch with grid_blocks
+// would use (its lanes bound the saved streamlines of a generation).
+hipError_t prepare_track_fast(const FastLaunch &p, int grid_blocks, int *blocks);
 // Dense seed-major rows (both paths) -> ragged points.
 hipError_t launch_compact_seed_major(const float *dense, int64_t n_seeds, uint32_t n_points,
                                      const uint32_t *lengths, const int64_t *offsets, float *out,

@@ -29,8 +29,9 @@ class HipRuntime:
         precision: coefficient storage read by the fast path. Only
             ``"fp32"`` is served; ``"bf16x2"``, ``"fp16"`` and ``"bf16"`` are
             accepted names, but a fast-path call then raises
-            NotImplementedError (16-bit coefficients move tracks by 3e-2 to
-            5e-2 voxel, past the 1e-3 budget).
+            NotImplementedError (FP16 or BF16 coefficients move tracks by
+            1.2e-2 to 2.4e-2 voxel, past the 1e-3 budget; split BF16 uses 97%
+            of the budget by itself and stores as many bytes as FP32).
     """
 
     def __init__(self, device: int = 0, *, path: str = "plain", precision: str = "fp32"):
@@ -77,6 +78,19 @@ class HipRuntime:
         """Blocks of the fast path's persistent kernel (<= 0: occupancy)."""
         _native.check(_native.lib().tgk_runtime_set_fast_grid(self._handle, int(blocks)))
         self.fast_grid = int(blocks) if blocks > 0 else 0
+
+    def set_fast_repack(self, min_streamlines: int) -> None:
+        """Streamlines (half a generation's work) below which the fast path
+        no longer cuts a generation short to repack its streamlines."""
+        _native.check(_native.lib().tgk_runtime_set_fast_repack(self._handle, int(min_streamlines)))
+
+    def fast_reads(self) -> dict:
+        """Coefficient rows and mask values the last fast-path tracking launch
+        read from memory (each lane reads them only when its cell changes)."""
+        rows = ctypes.c_uint64()
+        masks = ctypes.c_uint64()
+        _native.check(_native.lib().tgk_runtime_fast_reads(self._handle, ctypes.byref(rows), ctypes.byref(masks)))
+        return {"rows": rows.value, "mask_values": masks.value}
 
     def stream(self) -> int:
         s = ctypes.c_void_p()
