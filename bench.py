@@ -415,6 +415,14 @@ def bytes_per_step(K: int, step: float, save_at: float, mask: bool = True) -> fl
     return 4.0 * K + (4.0 if mask else 0.0) + 16.0 * step / save_at
 
 
+def unique_bytes(K: int, reads: dict, saved_points: int, n_seeds: int) -> float:
+    """Bytes a tracking launch with a per-lane cell cache must move at
+    least: K f32 coefficients per row read and one mask byte per mask read
+    (both counted by the kernel), 12 B per saved point (f32 x, y, z), the
+    32-B seed rows and 12 B of per-seed counts (length, steps, hit)."""
+    return 4.0 * K * reads["rows"] + reads["mask_values"] + 12.0 * saved_points + 44.0 * n_seeds
+
+
 def make_algorithm(tg, cfg, seed: int = 0):
     if cfg.get("model") == "dti":
         model = tg.local.DiffusionTensorImaging()
@@ -570,9 +578,10 @@ class GraphRun:
         self.stream = torch.cuda.ExternalStream(runtime.stream(), device=dev)
         self.graph = None
 
-    def enqueue(self, events=None):
+    def enqueue(self, events=None, reads=None):
         """The chunk loop on the runtime stream; with `events`, a (start, stop)
-        pair is recorded around each tracking launch."""
+        pair is recorded around each tracking launch; with `reads`, each
+        launch's row and mask read counts are appended (synchronising)."""
         check, lib, h = self.native.check, self.lib, self.runtime.handle
         first = 0
         for count in self.chunks:
@@ -591,6 +600,8 @@ class GraphRun:
                                        self.d_dense.data_ptr()))
             if events is not None:
                 events[-1][1].record(self.stream)
+            if reads is not None:
+                reads.append(self.runtime.fast_reads())
             check(lib.tgk_compact_device(h, self.native.PATH_FAST, self.d_dense.data_ptr(), count, self.n_points,
                                          self.d_lengths.data_ptr() + first * 4, self.d_offsets.data_ptr(),
                                          self.d_points.data_ptr()))
@@ -617,9 +628,11 @@ class GraphRun:
         pass, orientation updates per replay)."""
         torch = self.torch
         events = []
-        self.enqueue(events)  # eager pass: tracking kernel time per chunk
+        self.reads = []
+        self.enqueue(events, self.reads)  # eager pass: tracking kernel time and reads per chunk
         self.runtime.finish()
         kernel = float(sum(a.elapsed_time(b) for a, b in events))
+        self.saved_points = int(self.d_lengths.to(torch.int64).sum().item())
         self.record()
         for _ in range(warmup):
             self.native.check(self.lib.tgk_graph_launch(self.graph))
@@ -744,11 +757,17 @@ def seeds_leg(args, tg, torch, cfg, workload: str):
             "config": {"workload": workload, "product": "seeds", "seeds": n, "K": K, "nnz": len(voxels)},
             "methods": results, "value": results["mfma"]["seeds_per_s"], "ms_per_step": results["mfma"]["ms"],
             "dtype": "bf16 (hi+lo split) MFMA, f32 accumulate"}
-    flops = n * 2 * 1000 * 2 * K * 3
+    # Algorithmic work: two passes over 1000 directions of a K-term dot
+    # product per seed, 4000 K flop (the three split-bf16 products the
+    # kernel issues per product are its cost, not the algorithm's). The
+    # second pass ends early once every lane has chosen, so this is an
+    # upper bound of the work the algorithm needs.
+    flops = n * 4000 * K
     achieved = flops / (results["mfma"]["ms"] * 1e-3) / 1e12
     line["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": MI355X_BF16_DENSE_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / MI355X_BF16_DENSE_TFLOPS, "kernel": "seeds_mfma_kernel",
-                        "flops_per_seed_upper_bound": 2 * 1000 * 2 * K * 3}
+                        "algorithmic_flops_per_seed": 4000 * K,
+                        "issued_mfma_flops_per_seed_upper_bound": 2 * 1000 * 2 * K * 3}
     print(json.dumps(line), flush=True)
 
 
@@ -888,7 +907,11 @@ def main():
     parser.add_argument("--fast-grid", type=int, default=0,
                         help="blocks of the fast kernel (default: occupancy-sized)")
     parser.add_argument("--autotune", action="store_true",
-                        help="choose the fast kernel's grid on 100k seeds of the workload before timing")
+                        help="choose the fast kernel's grid and repack threshold on 100k seeds of the workload "
+                             "before timing")
+    parser.add_argument("--repack", type=int, default=0,
+                        help="fast path: streamlines below which a generation is no longer cut short "
+                             "(default: the library's, 16384)")
     parser.add_argument("--no-api", dest="api", action="store_false",
                         help="skip timing tg.tractogram on host arrays (transfers and uploads included)")
     parser.add_argument("--api", dest="api", action="store_true", help=argparse.SUPPRESS)
@@ -944,16 +967,22 @@ def main():
     if args.calibrate_copy > 0:
         calibration_copy(torch, args.calibrate_copy)
     runtime = tg.HipRuntime()
+    tuned = None
     if args.autotune:
+        # Kernel-only timing of a 100k-seed sample for each grid and repack
+        # threshold (not cached: the bench measures what it tunes).
         tuner = tg.HipRuntime(path="fast")
         sample = cfg["seeds"][:100_000]
-        tuned = tg.autotune.tune_fast_grid(make_algorithm(tg, cfg), make_domain(tg, cfg),
-                                           tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4))), sample,
-                                           runtime=tuner, save_at=cfg["save_at"],
-                                           maximum_streamline_length=cfg["max_length"])
+        tuned = tg.autotune.tune_fast(make_algorithm(tg, cfg), make_domain(tg, cfg),
+                                      tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4))), sample,
+                                      runtime=tuner, save_at=cfg["save_at"],
+                                      maximum_streamline_length=cfg["max_length"], cache=False)
         args.fast_grid = tuned["blocks"]
-        print(f"[bench] autotuned fast grid: {tuned}", file=sys.stderr, flush=True)
+        args.repack = tuned["repack"]
+        print(f"[bench] autotuned: {tuned}", file=sys.stderr, flush=True)
     runtime.set_fast_grid(args.fast_grid)
+    if args.repack:
+        runtime.set_fast_repack(args.repack)
     run = DeviceRun(tg, torch, runtime, cfg)
     paths = [p for p in args.paths.split(",") if p]
     results = {}
@@ -966,6 +995,9 @@ def main():
             print(f"[bench] {name} path unavailable: {e}", file=sys.stderr, flush=True)
             continue
         results[name] = {"ms_per_step": wall, "kernel_ms": kernel, "steps": total}
+        if path == _native.PATH_FAST:
+            results[name]["reads"] = runtime.fast_reads()
+            results[name]["saved_points"] = int(run.d_lengths.to(torch.int64).sum().item())
         gpu_results[name] = run.results(min(400, run.n))
         print(f"[bench] {name}: {wall:.1f} ms/step, kernel {kernel:.1f} ms, {total} orientation updates, "
               f"{total / wall * 1e3:.3e} steps/s", file=sys.stderr, flush=True)
@@ -973,7 +1005,16 @@ def main():
     h = results[headline]
     K = cfg["fod"].shape[-1]
     bps = bytes_per_step(K, cfg["step_size"], cfg["save_at"], mask=cfg.get("mesh") is None)
-    achieved = bps * h["steps"] / (h["kernel_ms"] * 1e-3) / 1e9
+    if "reads" in h:
+        # The bytes the cell cache has to move: a row of K f32 coefficients
+        # and a mask byte each time a lane's cell changes (counted in the
+        # kernel), 12 B per saved point, the seed rows in and the per-seed
+        # counts out. Every byte is needed by the kernel at least once, so
+        # this is a lower bound of its memory traffic.
+        unique = unique_bytes(K, h["reads"], h["saved_points"], len(cfg["seeds"]))
+    else:
+        unique = bps * h["steps"]
+    achieved = unique / (h["kernel_ms"] * 1e-3) / 1e9
     line = {
         "metric": "streamline steps/s on 1 MI355X, mixed vs FP32 path, at <=1e-3 voxel max deviation",
         "value": h["steps"] / (h["ms_per_step"] * 1e-3),
@@ -988,16 +1029,31 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded numpy generators)",
         "config": {"workload": workload, "path": headline, "seeds": len(cfg["seeds"]),
-                   "orientation_updates_per_step": h["steps"]},
+                   "orientation_updates_per_step": h["steps"],
+                   "fast_launch": {"grid_blocks": args.fast_grid or "occupancy",
+                                   "repack": args.repack or "default",
+                                   "autotuned": tuned is not None}},
         "paths": {k: {"steps_per_s": v["steps"] / (v["ms_per_step"] * 1e-3), "ms_per_step": v["ms_per_step"],
                       "kernel_ms": v["kernel_ms"]} for k, v in results.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / MI355X_HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic and headline == "fast" else None,
-                     "kernel": f"track_{headline}", "bytes_per_step": bps},
+                     "kernel": f"track_{headline}", "algorithmic_bytes_per_launch": unique,
+                     "basis": ("rows and mask values read on cell changes (kernel counters) x (4K, 1) B + 12 B "
+                               "per saved point + 32 B per seed in + 12 B per seed out" if "reads" in h else
+                               "survey bytes per step x steps"),
+                     "survey_bytes_per_step": bps,
+                     "survey_model_gbs": bps * h["steps"] / (h["kernel_ms"] * 1e-3) / 1e9},
     }
+    if "reads" in h:
+        line["roofline"]["reads"] = h["reads"]
+        line["roofline"]["steps_per_row_read"] = h["steps"] / max(1, h["reads"]["rows"])
     if traffic and headline == "fast":
-        line["roofline"]["traffic_detail"] = {k: v for k, v in traffic.items() if k != "valu_instructions"}
+        line["roofline"]["traffic_detail"] = {k: v for k, v in traffic.items() if k not in ("valu_instructions", "l2")}
+        if traffic.get("l2"):
+            l2 = traffic["l2"]
+            line["roofline"]["l2"] = {**l2, "hit_rate": l2["hits"] / max(1.0, l2["hits"] + l2["misses"]),
+                                      "traffic_gbs": traffic["bytes_per_launch"] / (h["kernel_ms"] * 1e-3) / 1e9}
         # The binding resource: wavefront VALU instructions issued per second
         # of the kernel against the issue rate of all SIMDs.
         issued = traffic["valu_instructions"] / (h["kernel_ms"] * 1e-3)
@@ -1075,14 +1131,18 @@ def traffic_leg(args):
         return None
     measured = {}
     with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
-        for counter in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU"):
-            out = Path(tmp) / counter
-            cmd = [profiler, "--pmc", counter, "--output-format", "csv", "-d", str(out), "-o", "run", "--",
+        for counter in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU", "TCC_HIT_sum TCC_MISS_sum"):
+            out = Path(tmp) / counter.split()[0]
+            cmd = [profiler, "--pmc", *counter.split(), "--output-format", "csv", "-d", str(out), "-o", "run", "--",
                    sys.executable, str(ROOT / "bench.py"), "--workload", args.workload, "--paths", "fast",
                    "--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--no-traffic", "--no-api",
                    "--calibrate-copy", str(TRAFFIC_CALIBRATION_BYTES)]
             if args.seeds:
                 cmd += ["--seeds", str(args.seeds)]
+            if args.fast_grid:
+                cmd += ["--fast-grid", str(args.fast_grid)]
+            if args.repack:
+                cmd += ["--repack", str(args.repack)]
             print(f"[bench] counter run {counter}", file=sys.stderr, flush=True)
             try:
                 subprocess.run(cmd, cwd="/tmp", stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
@@ -1093,23 +1153,33 @@ def traffic_leg(args):
             files = sorted(out.rglob("*counter_collection.csv"))
             if not files:
                 return None
+            if counter.startswith("TCC_HIT"):
+                # L2 hits and misses of the kernel's requests (gathers of
+                # coefficient rows, mask values, point stores).
+                hits = [v for k, v in _counter_values(files[0], "TCC_HIT_sum").items() if "track_fast_kernel" in k]
+                misses = [v for k, v in _counter_values(files[0], "TCC_MISS_sum").items() if "track_fast_kernel" in k]
+                if hits and misses:
+                    measured["l2"] = {"hits": float(np.sum(hits[0])), "misses": float(np.sum(misses[0]))}
+                continue
             values = _counter_values(files[0], counter)
             kernel = [v for k, v in values.items() if "track_fast_kernel" in k]
+            # One tracking launch is several dispatches of the kernel (its
+            # generations): per-launch figures are their sums.
             if counter == "SQ_INSTS_VALU":
                 if not kernel:
                     return None
-                measured[counter] = float(np.mean(kernel[0]))
+                measured[counter] = float(np.sum(kernel[0]))
                 continue
             copy = [max(v) for k, v in values.items() if "CUDAFunctorOnSelf_add" in k]
             if not kernel or not copy or copy[0] <= 0:
                 return None
             scale = TRAFFIC_CALIBRATION_BYTES / (copy[0] * 1024.0)
-            measured[counter] = {"kib_per_dispatch": float(np.mean(kernel[0])), "scale": scale,
-                                 "bytes": float(np.mean(kernel[0])) * 1024.0 * scale}
+            measured[counter] = {"kib_per_launch": float(np.sum(kernel[0])), "dispatches": len(kernel[0]),
+                                 "scale": scale, "bytes": float(np.sum(kernel[0])) * 1024.0 * scale}
     return {"bytes_per_launch": measured["FETCH_SIZE"]["bytes"] + measured["WRITE_SIZE"]["bytes"],
             "fetch_bytes": measured["FETCH_SIZE"]["bytes"], "write_bytes": measured["WRITE_SIZE"]["bytes"],
             "fetch_scale": measured["FETCH_SIZE"]["scale"], "write_scale": measured["WRITE_SIZE"]["scale"],
-            "valu_instructions": measured["SQ_INSTS_VALU"]}
+            "valu_instructions": measured["SQ_INSTS_VALU"], "l2": measured.get("l2")}
 
 
 def calibration_copy(torch, n_bytes: int):
@@ -1138,7 +1208,9 @@ def run_graph_workload(args, cfg, workload):
           f"{updates / ms * 1e3:.3e} steps/s", file=sys.stderr, flush=True)
     K = cfg["fod"].shape[-1]
     bps = bytes_per_step(K, cfg["step_size"], cfg["save_at"])
-    achieved = bps * updates / (kernel * 1e-3) / 1e9
+    reads = {"rows": sum(r["rows"] for r in run.reads), "mask_values": sum(r["mask_values"] for r in run.reads)}
+    unique = unique_bytes(K, reads, run.saved_points, run.n)
+    achieved = unique / (kernel * 1e-3) / 1e9
     line = {
         "metric": "streamline steps/s on 1 MI355X, mixed vs FP32 path, at <=1e-3 voxel max deviation",
         "value": updates / (ms * 1e-3),
@@ -1156,7 +1228,11 @@ def run_graph_workload(args, cfg, workload):
                    "orientation_updates_per_step": updates},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": MI355X_HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / MI355X_HBM_PEAK_GBS, "traffic": None, "kernel": "track_fast",
-                     "bytes_per_step": bps},
+                     "algorithmic_bytes_per_launch": unique, "reads": reads,
+                     "basis": "rows and mask values read on cell changes (kernel counters, summed over the "
+                              "chunks) x (4K, 1) B + 12 B per saved point + 44 B per seed",
+                     "survey_bytes_per_step": bps,
+                     "survey_model_gbs": bps * updates / (kernel * 1e-3) / 1e9},
     }
     print(json.dumps(line), flush=True)
 

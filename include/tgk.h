@@ -197,7 +197,10 @@ int tgk_domain_destroy(tgk_domain *domain);
  * rng_init(rng_seeds[g / batch_size], g % batch_size) like the reference's
  * per-batch get_rng_seed(); n_points = floor(max_length / save_at) is the
  * point capacity including the seed. Results stay in the returned handle:
- * the per-seed counts in host memory, the ragged points in device memory. */
+ * the per-seed counts and the ragged points in host memory (page-locked,
+ * filled by DMA as each chunk is compacted; device memory only when
+ * page-locked memory cannot be had). The runtime keeps its working device
+ * buffers for later calls. */
 int tgk_track(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field, const tgk_algorithm *algorithm,
               int32_t path, const float *seeds, int64_t n_seeds, const uint32_t *rng_seeds, int64_t batch_size,
               uint32_t n_points, float save_at, tgk_tracks **out);
@@ -209,6 +212,11 @@ int tgk_tracks_size(const tgk_tracks *tracks, int64_t *n_seeds, int64_t *total_p
  * points come straight from device memory: the call blocks until they are
  * in `points`. */
 int tgk_tracks_copy(const tgk_tracks *tracks, uint32_t *lengths, uint32_t *steps, uint32_t *hits, float *points);
+/* No counterpart in the reference (its points come back in a dense array):
+ * the ragged points (total_points, 3) in place, when the handle holds them
+ * in one host buffer (a call tracked in one chunk), else NULL (use
+ * tgk_tracks_copy). Valid until tgk_tracks_destroy. */
+int tgk_tracks_points(const tgk_tracks *tracks, const float **points);
 int tgk_tracks_destroy(tgk_tracks *tracks);
 
 /* Device-resident tracking of one chunk, asynchronous on the runtime stream.

@@ -86,6 +86,7 @@ _SIGNATURES = {
     ),
     "tgk_tracks_size": (c_int, [_P, POINTER(c_int64), POINTER(c_int64)]),
     "tgk_tracks_copy": (c_int, [_P, _P, _P, _P, _P]),
+    "tgk_tracks_points": (c_int, [_P, POINTER(c_void_p)]),
     "tgk_tracks_destroy": (c_int, [_P]),
     "tgk_track_device": (
         c_int,

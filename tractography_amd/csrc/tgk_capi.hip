@@ -87,6 +87,11 @@ struct DeviceBuffer {
 constexpr size_t FAST_STATS_OFFSET = 64;
 constexpr size_t FAST_CONT_COUNT_OFFSET = 136;
 
+// tgk_track's device buffers for one chunk of seeds, kept by the runtime.
+struct TrackBuffers {
+    DeviceBuffer seeds, rng, lengths, steps, hits, dense, offsets, points;
+};
+
 struct tgk_runtime {
     int32_t device = 0;
     hipStream_t stream = nullptr;
@@ -94,6 +99,7 @@ struct tgk_runtime {
     DeviceBuffer queue;        // counters of the persistent kernels (layout: FAST_*_OFFSET)
     DeviceBuffer continuations[2];  // streamlines saved between generations of the fast kernel
     DeviceBuffer fast_counters;     // claim counter and ended-streamline shards of the fast kernel
+    TrackBuffers track_buffers;     // tgk_track's working buffers
     DeviceBuffer seeds_table;  // prepared FOD-seeding table for seeds_key
     int32_t seeds_key[3] = {-1, -1, -1};  // model, K, method
     bool capturing = false;    // the stream is being recorded into a graph
@@ -135,13 +141,40 @@ struct tgk_domain {
     float reach = 0.0f;
 };
 
+// Page-locked host memory (hipHostMalloc): the device writes it by DMA at
+// the link's full rate, and the host reads it as ordinary cached memory.
+struct PinnedBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer &) = delete;
+    PinnedBuffer &operator=(const PinnedBuffer &) = delete;
+    ~PinnedBuffer()
+    {
+        if (ptr)
+            (void)hipHostFree(ptr);
+    }
+    hipError_t allocate(size_t n)
+    {
+        hipError_t e = hipHostMalloc(&ptr, n, hipHostMallocDefault);
+        if (e == hipSuccess)
+            bytes = n;
+        else
+            ptr = nullptr;
+        return e;
+    }
+};
+
 struct tgk_tracks {
     int device = 0;
     std::vector<uint32_t> lengths, steps, hits;
-    // The ragged points of each chunk, left in device memory (exactly sized)
-    // until tgk_tracks_copy moves them once into the caller's array; in host
-    // memory instead when the device has no room left for them.
+    // The ragged points of each chunk: copied by DMA into exactly sized
+    // page-locked host memory as soon as the chunk is compacted (a single
+    // chunk's buffer is then the result itself, see tgk_tracks_points);
+    // left in device memory instead when no page-locked memory is to be had,
+    // and in pageable host memory when neither has room.
     struct Piece {
+        std::unique_ptr<PinnedBuffer> pinned;
         std::unique_ptr<DeviceBuffer> device;
         std::vector<float> host;
         int64_t n_points = 0;
@@ -779,6 +812,12 @@ int fast_launch(tgk_runtime *rt, tgk_field *field, const tgk_algorithm *algorith
     f->coeff_stride = field->fast_stride;
     f->sh_order = sh_order;
     f->min_repack = rt->fast_min_repack;
+    // The tracking kernel's cell ids are 32-bit.
+    const uint64_t field_voxels = (uint64_t)p.field.dims[0] * p.field.dims[1] * p.field.dims[2];
+    const uint64_t mask_voxels = (uint64_t)p.mask.dims[0] * p.mask.dims[1] * p.mask.dims[2];
+    if (field_voxels >= 0xfffffffeull || (p.domain_kind == tgk::DOMAIN_MASK && mask_voxels >= 0xfffffffeull))
+        return fail(TGK_ERR_UNSUPPORTED, "the fast path takes grids of fewer than 2^32 - 2 voxels");
+    f->shared_affine = p.domain_kind == tgk::DOMAIN_MASK && memcmp(p.mask.inv, p.field.inv, sizeof p.mask.inv) == 0;
     // Room for one generation's saved streamlines: one per lane of the grid.
     int blocks = 0;
     TGK_HIP(tgk::prepare_track_fast(*f, rt->fast_grid, &blocks), "fast kernel occupancy");
@@ -921,16 +960,31 @@ int tgk_track(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field,
     const int64_t n_batches = (n_seeds + batch_size - 1) / batch_size;
     tracks->device = rt->device;
 
-    DeviceBuffer d_seeds, d_rng, d_lengths, d_steps, d_hits, d_dense, d_offsets, d_points;
-    TGK_HIP(d_seeds.allocate((size_t)chunk * 8 * sizeof(float)), "seed buffer");
-    if (int r = upload(d_rng, rng_seeds, (size_t)n_batches, rt->stream, "rng seeds"))
+    // Working buffers are the runtime's, kept between calls (grown, never
+    // shrunk until the runtime is destroyed): a 1M-seed call needs about
+    // 29 GB of them, whose allocation would otherwise recur per call.
+    TrackBuffers &w = rt->track_buffers;
+    if (int r = reserve(rt, w.seeds, (size_t)chunk * 8 * sizeof(float), "seed buffer"))
         return r;
-    TGK_HIP(d_lengths.allocate((size_t)chunk * 4), "lengths buffer");
-    TGK_HIP(d_steps.allocate((size_t)chunk * 4), "steps buffer");
-    TGK_HIP(d_hits.allocate((size_t)chunk * 4), "hits buffer");
-    TGK_HIP(d_dense.allocate((size_t)chunk * per_seed), "point buffer");
-    TGK_HIP(d_offsets.allocate((size_t)(chunk + 1) * 8), "offsets buffer");
-    TGK_HIP(d_points.allocate((size_t)chunk * per_seed), "point buffer");
+    if (int r = reserve(rt, w.rng, (size_t)n_batches * sizeof(uint32_t), "rng seeds"))
+        return r;
+    TGK_HIP(hipMemcpyAsync(w.rng.ptr, rng_seeds, (size_t)n_batches * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           rt->stream),
+            "rng seeds");
+    if (int r = reserve(rt, w.lengths, (size_t)chunk * 4, "lengths buffer"))
+        return r;
+    if (int r = reserve(rt, w.steps, (size_t)chunk * 4, "steps buffer"))
+        return r;
+    if (int r = reserve(rt, w.hits, (size_t)chunk * 4, "hits buffer"))
+        return r;
+    if (int r = reserve(rt, w.dense, (size_t)chunk * per_seed, "point buffer"))
+        return r;
+    if (int r = reserve(rt, w.offsets, (size_t)(chunk + 1) * 8, "offsets buffer"))
+        return r;
+    if (int r = reserve(rt, w.points, (size_t)chunk * per_seed, "point buffer"))
+        return r;
+    DeviceBuffer &d_seeds = w.seeds, &d_rng = w.rng, &d_lengths = w.lengths, &d_steps = w.steps, &d_hits = w.hits,
+                 &d_dense = w.dense, &d_offsets = w.offsets, &d_points = w.points;
 
     for (int64_t start = 0; start < n_seeds; start += chunk) {
         const int64_t count = std::min(chunk, n_seeds - start);
@@ -962,6 +1016,20 @@ int tgk_track(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field,
         tgk_tracks::Piece piece;
         piece.n_points = total;
         const size_t piece_bytes = (size_t)total * 3 * sizeof(float);
+        piece.pinned.reset(new (std::nothrow) PinnedBuffer());
+        if (!piece.pinned)
+            return fail(TGK_ERR_NO_MEMORY, "tracks allocation");
+        if (piece_bytes && piece.pinned->allocate(piece_bytes) == hipSuccess) {
+            // In stream order: the next chunk's compaction, which reuses
+            // d_points, comes after this copy.
+            TGK_HIP(hipMemcpyAsync(piece.pinned->ptr, d_points.ptr, piece_bytes, hipMemcpyDeviceToHost, rt->stream),
+                    "points copy");
+            tracks->pieces.push_back(std::move(piece));
+            tracks->total_points += total;
+            continue;
+        }
+        (void)hipGetLastError();  // clear a failed page-locked allocation's error
+        piece.pinned.reset();
         piece.device.reset(new (std::nothrow) DeviceBuffer());
         if (!piece.device)
             return fail(TGK_ERR_NO_MEMORY, "tracks allocation");
@@ -1019,13 +1087,25 @@ int tgk_tracks_copy(const tgk_tracks *tracks, uint32_t *lengths, uint32_t *steps
         float *dst = points;
         for (const tgk_tracks::Piece &piece : tracks->pieces) {
             const size_t bytes = (size_t)piece.n_points * 3 * sizeof(float);
-            if (bytes && piece.device)
+            if (bytes && piece.pinned)
+                memcpy(dst, piece.pinned->ptr, bytes);
+            else if (bytes && piece.device)
                 TGK_HIP(hipMemcpy(dst, piece.device->ptr, bytes, hipMemcpyDeviceToHost), "points copy");
             else if (bytes)
                 memcpy(dst, piece.host.data(), bytes);
             dst += (size_t)piece.n_points * 3;
         }
     }
+    return TGK_OK;
+}
+
+int tgk_tracks_points(const tgk_tracks *tracks, const float **points)
+{
+    if (!tracks || !points)
+        return fail(TGK_ERR_INVALID, "NULL argument");
+    *points = nullptr;
+    if (tracks->pieces.size() == 1 && tracks->pieces[0].pinned)
+        *points = static_cast<const float *>(tracks->pieces[0].pinned->ptr);
     return TGK_OK;
 }
 

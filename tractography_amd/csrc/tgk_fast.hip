@@ -18,6 +18,7 @@
 // Compiled with -ffp-contract=fast.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <mutex>
@@ -54,45 +55,138 @@ __device__ __forceinline__ int64_t mask_cell(const MaskView &m, V3 p)
     return ((int64_t)(uint32_t)rx * m.dims[1] + (uint32_t)ry) * m.dims[2] + (uint32_t)rz;
 }
 
-struct Lane {
-    int64_t g;          // global seed index being tracked (-1: idle)
-    V3 point, o, previous;
+// Cell ids of the tracking kernel are 32-bit: the fast path takes mask and
+// field grids of fewer than 2^32 - 2 voxels.
+constexpr uint32_t NO_CELL = 0xffffffffu;     // outside the grid
+constexpr uint32_t UNSET_CELL = 0xfffffffeu;  // nothing cached yet
+
+// The cell of rounded voxel coordinates (rx, ry, rz) on a grid, C order.
+__device__ __forceinline__ uint32_t grid_cell(float rx, float ry, float rz, const uint32_t dims[3])
+{
+    // Bitwise &: all three coordinates are computed anyway, and a branch per
+    // test costs more than the tests.
+    if (!((rx >= 0.0f) & (rx < (float)dims[0]) & (ry >= 0.0f) & (ry < (float)dims[1]) & (rz >= 0.0f) &
+          (rz < (float)dims[2])))
+        return NO_CELL;
+    return ((uint32_t)rx * dims[1] + (uint32_t)ry) * dims[2] + (uint32_t)rz;
+}
+
+// Saved points are stored in groups. A lane appends a 12-byte point about
+// every 2.5 steps to its own row, and the grid's 262,144 lanes keep more
+// partly written lines open than L2 holds, so single-point stores were
+// written back to memory in partial lines several times each (20.9 GB
+// written for 6 GB of points at 1M seeds). A group of four is four
+// consecutive points starting at a multiple of four, 48 bytes, stored as
+// three 16-byte stores when its address is 16-byte aligned (always, for a
+// 16-byte aligned buffer and n_points a multiple of 4): 8.2 GB written, and
+// 24.2 -> 21.1 ms per launch. The order-12 kernels and the Probabilistic
+// and Deterministic ones, whose registers the group would push past a wave
+// boundary, store single points.
+template <int MODEL, int ALGO, int LMAX> constexpr int point_group()
+{
+    return (MODEL == MODEL_SH && LMAX == 12) || ALGO == ALGO_PROBABILISTIC || ALGO == ALGO_DETERMINISTIC ? 1 : 4;
+}
+
+template <int G> struct Lane {
+    int32_t g;          // seed being tracked (-1: idle); calls take fewer than 2^31 seeds
+    V3 point, o;
+    V3 previous;        // the last step's start (mesh domains: the segment tested against the boundary)
     uint2 rng;
     double time;
     uint32_t n, steps;
-    int64_t field_cell; // cell ids of `point` (-2: none yet); their data is requested by request_cells
-    int64_t mask_cell;
+    uint32_t field_cell; // cell ids of `point` (UNSET_CELL: none yet); their data is requested by request_cells
+    uint32_t mask_cell;
     uint32_t mask_value; // the mask at mask_cell (0 outside the mask image)
     bool field_ok;       // `point` is inside the field image
-    uint32_t row_loads;  // coefficient rows and mask values this lane read from memory
-    uint32_t mask_loads;
+    bool row_read;       // request_cells read a row / a mask value this iteration
+    bool mask_read;
+    // The last `held` saved points not yet stored, oldest first, in
+    // group[G - held ..]; their indices are n - held .. n - 1.
+    // `store_due`: n is a multiple of G, so they close a group.
+    V3 group[G];
+    uint32_t held;
+    bool store_due;
 };
+
+template <int G> __device__ __forceinline__ void hold_point(Lane<G> &L, V3 q)
+{
+#pragma unroll
+    for (int i = 0; i + 1 < G; i++)
+        L.group[i] = L.group[i + 1];
+    L.group[G - 1] = q;
+    L.held = min(L.held + 1u, (uint32_t)G);
+    L.n++;
+    L.store_due = L.n % G == 0;
+}
+
+// Stores the held points: a whole aligned group of four as three float4
+// stores, otherwise one point at a time (a single-point group, the first
+// group after a resumption, or what a stopping or saved streamline holds).
+template <int G> __device__ __forceinline__ void store_held(const TrackLaunch &p, Lane<G> &L)
+{
+    float *row = p.points + ((int64_t)L.g * p.n_points + (L.n - L.held)) * 3;
+    if (G == 4 && L.held == G && (reinterpret_cast<uintptr_t>(row) & 15u) == 0) {
+        float4 *dst = reinterpret_cast<float4 *>(row);
+        const V3 *q = L.group;
+        dst[0] = make_float4(q[0].x, q[0].y, q[0].z, q[G > 1 ? 1 : 0].x);
+        dst[1] = make_float4(q[G > 1 ? 1 : 0].y, q[G > 1 ? 1 : 0].z, q[G > 2 ? 2 : 0].x, q[G > 2 ? 2 : 0].y);
+        dst[2] = make_float4(q[G > 2 ? 2 : 0].z, q[G - 1].x, q[G - 1].y, q[G - 1].z);
+    } else {
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            if ((uint32_t)i >= G - L.held) {
+                float *dst = row + 3 * (i - (G - (int)L.held));
+                dst[0] = L.group[i].x;
+                dst[1] = L.group[i].y;
+                dst[2] = L.group[i].z;
+            }
+        }
+    }
+    L.held = 0;
+    L.store_due = false;
+}
+
+// A closed group is stored in the iteration after its last point is saved,
+// right after that iteration's wait for its cell data: every iteration
+// waits (s_waitcnt vmcnt(0)) for the data requested in the previous one,
+// and vector stores count under the same counter, so a store issued at the
+// end of an iteration would hold the next one at that wait.
+template <int G> __device__ __forceinline__ void store_due_group(const TrackLaunch &p, Lane<G> &L)
+{
+    if (L.store_due)
+        store_held(p, L);
+}
 
 // A streamline in flight when a generation of the persistent launch ends
 // (see track_fast_kernel): everything its lane needs to resume it, in four
-// float4. The cell data are requested again on resumption.
-__device__ __forceinline__ void save_lane(float4 *slot, const Lane &L)
+// float4 (the previous point only for mesh domains). Its held points are
+// stored first; the cell data are requested again on resumption.
+template <int DOMAIN, int G> __device__ __forceinline__ void save_lane(float4 *slot, const Lane<G> &L)
 {
     const uint64_t tb = (uint64_t)__double_as_longlong(L.time);
+    const V3 prev = DOMAIN == DOMAIN_MESH ? L.previous : v3(0.0f, 0.0f, 0.0f);
     slot[0] = make_float4(L.point.x, L.point.y, L.point.z, L.o.x);
-    slot[1] = make_float4(L.o.y, L.o.z, L.previous.x, L.previous.y);
-    slot[2] = make_float4(L.previous.z, __uint_as_float((uint32_t)tb), __uint_as_float((uint32_t)(tb >> 32)),
+    slot[1] = make_float4(L.o.y, L.o.z, prev.x, prev.y);
+    slot[2] = make_float4(prev.z, __uint_as_float((uint32_t)tb), __uint_as_float((uint32_t)(tb >> 32)),
                           __uint_as_float((uint32_t)L.g));
     slot[3] = make_float4(__uint_as_float(L.n), __uint_as_float(L.steps), __uint_as_float(L.rng.x),
                           __uint_as_float(L.rng.y));
 }
 
-__device__ __forceinline__ void resume_lane(const float4 *slot, Lane &L)
+template <int DOMAIN, int G> __device__ __forceinline__ void resume_lane(const float4 *slot, Lane<G> &L)
 {
     const float4 a = slot[0], b = slot[1], c = slot[2], d = slot[3];
     L.point = v3(a.x, a.y, a.z);
     L.o = v3(a.w, b.x, b.y);
-    L.previous = v3(b.z, b.w, c.x);
+    if constexpr (DOMAIN == DOMAIN_MESH)
+        L.previous = v3(b.z, b.w, c.x);
     L.time = __longlong_as_double((long long)(((uint64_t)__float_as_uint(c.z) << 32) | __float_as_uint(c.y)));
-    L.g = (int64_t)__float_as_uint(c.w);
+    L.g = (int32_t)__float_as_uint(c.w);
     L.n = __float_as_uint(d.x);
     L.steps = __float_as_uint(d.y);
     L.rng = make_uint2(__float_as_uint(d.z), __float_as_uint(d.w));
+    L.held = 0;
+    L.store_due = false;
 }
 
 // One iteration of the reference loop (tracking.py:121-147) for a lane with
@@ -216,33 +310,48 @@ __device__ __forceinline__ V3 update_orientation(const AlgoParams &a, const floa
 // memory latency then overlaps the rest of the step, and of the next one
 // up to the model evaluation, instead of stalling the wavefront, which
 // nearly every step holds at least one lane entering a new cell.
-template <int MODEL, int DOMAIN, int LMAX>
-__device__ __forceinline__ void request_cells(const TrackLaunch &p, const float *__restrict__ coeffs, uint32_t cstride,
-                                              Lane &L, float *q, float4 *__restrict__ lds_rows)
+// The voxel coordinates come from one evaluation of the field's inverse
+// affine; the mask's, when it is the same matrix (shared_affine: every
+// image-domain workload that tracks in its field's grid), reuses it. The mask
+// rounds floor(v + 0.5) (domain.py:369), the field rint(v) (field.py:97).
+template <int MODEL, int DOMAIN, int LMAX, int G>
+__device__ __forceinline__ void request_cells(const TrackLaunch &p, bool shared_affine,
+                                              const float *__restrict__ coeffs, uint32_t cstride, Lane<G> &L, float *q,
+                                              float4 *__restrict__ lds_rows)
 {
+    const float fx = affine_row(p.field.inv, 0, L.point);
+    const float fy = affine_row(p.field.inv, 1, L.point);
+    const float fz = affine_row(p.field.inv, 2, L.point);
     if constexpr (DOMAIN == DOMAIN_MASK) {
-        const int64_t cell = mask_cell(p.mask, L.point);
+        float mx = fx, my = fy, mz = fz;
+        if (!shared_affine) {
+            mx = affine_row(p.mask.inv, 0, L.point);
+            my = affine_row(p.mask.inv, 1, L.point);
+            mz = affine_row(p.mask.inv, 2, L.point);
+        }
+        const uint32_t cell = grid_cell(floorf(mx + 0.5f), floorf(my + 0.5f), floorf(mz + 0.5f), p.mask.dims);
         if (cell != L.mask_cell) {
             L.mask_cell = cell;
-            L.mask_value = cell >= 0 ? (uint32_t)p.mask.mask[cell] : 0u;
-            L.mask_loads += cell >= 0 ? 1u : 0u;
+            L.mask_value = cell != NO_CELL ? (uint32_t)p.mask.mask[cell] : 0u;
+            L.mask_read = cell != NO_CELL;
         }
     }
-    const int64_t cell = field_voxel(p.field, L.point);
-    L.field_ok = cell >= 0;
-    if (cell >= 0 && cell != L.field_cell) {
+    const uint32_t cell = grid_cell(rintf(fx), rintf(fy), rintf(fz), p.field.dims);
+    L.field_ok = cell != NO_CELL;
+    if (cell != NO_CELL && cell != L.field_cell) {
         L.field_cell = cell;
-        load_row<MODEL, LMAX>(coeffs, cell, cstride, q, lds_rows);
-        L.row_loads++;
+        load_row<MODEL, LMAX>(coeffs, (int64_t)cell, cstride, q, lds_rows);
+        L.row_read = true;
     }
 }
 
 // nz: this step's Diffusion draws, made by the caller before anything that
 // waits on memory, so they cover the latency of the cell data
 // request_cells asked for.
-template <int MODEL, int ALGO, int DOMAIN, int LMAX>
-__device__ __forceinline__ void advance(const TrackLaunch &p, const float *__restrict__ coeffs, uint32_t cstride,
-                                        Lane &L, float *q, float4 *__restrict__ lds_rows, const fast::Noise &nz)
+template <int MODEL, int ALGO, int DOMAIN, int LMAX, int G>
+__device__ __forceinline__ void advance(const TrackLaunch &p, bool shared_affine, const float *__restrict__ coeffs,
+                                        uint32_t cstride, Lane<G> &L, float *q, float4 *__restrict__ lds_rows,
+                                        const fast::Noise &nz)
 {
     bool stop = !(L.n < p.n_points && L.steps < p.max_steps);
     uint32_t hit = NO_HIT;
@@ -273,6 +382,7 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
     if (!stop)
         stop = !L.field_ok;
     if (stop) {
+        store_held(p, L);
         p.lengths[L.g] = L.n;
         p.steps[L.g] = L.steps;
         p.hits[L.g] = hit;
@@ -280,31 +390,42 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, const float *__res
         return;
     }
     read_row<MODEL, LMAX>(q, lds_rows);
+    store_due_group(p, L);
     L.o = update_orientation<MODEL, ALGO, LMAX>(p.algo, q, L.o, L.rng, nz);
     L.steps++;
-    L.previous = L.point;
+    if constexpr (DOMAIN == DOMAIN_MESH)
+        L.previous = L.point;
     const float step = p.algo.step_size;
     L.point = v3(L.point.x + step * L.o.x, L.point.y + step * L.o.y, L.point.z + step * L.o.z);
     L.time += (double)step;
     if (L.time >= (double)p.save_at) {
         L.time -= (double)p.save_at;
         const float t = (float)L.time;
-        store_row(p.points, L.g, p.n_points, L.n, v3(L.point.x - t * L.o.x, L.point.y - t * L.o.y, L.point.z - t * L.o.z));
-        L.n++;
+        hold_point(L, v3(L.point.x - t * L.o.x, L.point.y - t * L.o.y, L.point.z - t * L.o.z));
     }
-    request_cells<MODEL, DOMAIN, LMAX>(p, coeffs, cstride, L, q, lds_rows);
+    request_cells<MODEL, DOMAIN, LMAX, G>(p, shared_affine, coeffs, cstride, L, q, lds_rows);
 }
 
-// Waves per SIMD the compiler must fit: the order-8 SH Deterministic kernel
-// would take 147-150 VGPRs (three waves); held to 128 it spills 4-8 and runs
-// four waves, the most its 40 KiB of LDS per block allows: 684 -> 643 ms at
-// 1M seeds. Elsewhere the compiler's allocation is kept: held to four waves,
-// Probabilistic (158-165 VGPRs) ran 3% slower, and the order-12 kernels
-// (241-268 VGPRs, two waves) held to three spilled 70-107 VGPRs and ran
-// 3.8x slower.
-template <int MODEL, int ALGO, int LMAX> constexpr int fast_min_waves()
+// Waves per SIMD the compiler must fit: the order-8 SH Transport and
+// Diffusion kernels fit 128 VGPRs without spilling when held to four waves,
+// the most their 40 KiB of LDS per block allows (134-140 otherwise, three
+// waves). Held to three, the order-12 Transport and Diffusion kernels take
+// 168 VGPRs (170-185 otherwise, two waves) and the order-8 Probabilistic
+// one 151-155 (168-181, two waves); held to four, Probabilistic ran 3%
+// slower. The order-8 Deterministic kernel, held to four waves, made this
+// compiler emit an odd-aligned 64-bit register spill that gfx950 rejects,
+// so it keeps its 159-162 VGPRs (three waves). The DTI kernels (90-115
+// VGPRs, four or five waves) ran 4% slower held to seven waves and 19%
+// slower held to eight (spills).
+template <int MODEL, int ALGO, int DOMAIN, int LMAX> constexpr int fast_min_waves()
 {
-    return MODEL == MODEL_SH && LMAX == 8 && ALGO == ALGO_DETERMINISTIC ? 4 : 1;
+    if constexpr (MODEL == MODEL_SH && LMAX == 12 && (ALGO == ALGO_TRANSPORT || ALGO == ALGO_DIFFUSION))
+        return 3;
+    if constexpr (MODEL == MODEL_SH && LMAX == 8 && (ALGO == ALGO_TRANSPORT || ALGO == ALGO_DIFFUSION))
+        return 4;
+    if constexpr (MODEL == MODEL_SH && LMAX == 8 && ALGO == ALGO_PROBABILISTIC)
+        return 3;
+    return 1;
 }
 
 // The persistent launch runs in generations (launch_fast): generation 0
@@ -320,7 +441,7 @@ template <int MODEL, int ALGO, int LMAX> constexpr int fast_min_waves()
 // lane tracks which streamline, and when, changes no result.
 template <int MODEL, int ALGO, int DOMAIN, int LMAX>
 __global__ __launch_bounds__(fast_block<LMAX>())
-__attribute__((amdgpu_waves_per_eu(fast_min_waves<MODEL, ALGO, LMAX>())))
+__attribute__((amdgpu_waves_per_eu(fast_min_waves<MODEL, ALGO, DOMAIN, LMAX>())))
 void track_fast_kernel(FastLaunch f)
 {
     __shared__ float4 lds_rows[rows_in_lds<MODEL, LMAX>() ? lds_vectors<MODEL, LMAX>() * fast_block<LMAX>() : 1];
@@ -345,10 +466,15 @@ void track_fast_kernel(FastLaunch f)
     uint32_t *const my_shard =
         f.ended + ((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % FAST_END_SHARDS) * FAST_END_STRIDE;
 
-    Lane L;
+    constexpr int G = point_group<MODEL, ALGO, LMAX>();
+    Lane<G> L;
     L.g = -1;
-    L.row_loads = 0;
-    L.mask_loads = 0;
+    L.held = 0;
+    L.store_due = false;
+    L.row_read = false;
+    L.mask_read = false;
+    // Rows and mask values this wavefront read (wave-uniform sums).
+    uint64_t rows_read = 0, masks_read = 0;
     float q[MODEL == MODEL_SH ? Order<LMAX>::TERMS : 6];
     bool drained = false;
     uint32_t ended_pending = 0;     // ended here and not yet added to the shard
@@ -374,10 +500,10 @@ void track_fast_kernel(FastLaunch f)
                 const uint64_t j = (uint64_t)base + rank;
                 if (j < (uint64_t)n_work) {
                     if (resume) {
-                        resume_lane(f.cont_in + FAST_CONTINUATION_VEC4 * j, L);
+                        resume_lane<DOMAIN, G>(f.cont_in + FAST_CONTINUATION_VEC4 * j, L);
                     } else {
                         const uint64_t g = j;
-                        L.g = (int64_t)g;
+                        L.g = (int32_t)g;
                         const float4 s0 = reinterpret_cast<const float4 *>(p.seeds)[2 * g];
                         const float4 s1 = reinterpret_cast<const float4 *>(p.seeds)[2 * g + 1];
                         L.point = v3(s0.x, s0.y, s0.z);
@@ -386,14 +512,15 @@ void track_fast_kernel(FastLaunch f)
                         if constexpr (ALGO == ALGO_DIFFUSION || ALGO == ALGO_PROBABILISTIC)
                             L.rng = seed_stream(p, p.first_seed + L.g);
                         L.time = 0.0;
-                        L.n = 1;
+                        L.n = 0;
                         L.steps = 0;
-                        store_row(p.points, L.g, p.n_points, 0, L.point);
+                        L.held = 0;
+                        hold_point(L, L.point);  // the seed is the streamline's first point
                     }
-                    L.field_cell = -2;
-                    L.mask_cell = -2;
+                    L.field_cell = UNSET_CELL;
+                    L.mask_cell = UNSET_CELL;
                     L.mask_value = 0;
-                    request_cells<MODEL, DOMAIN, LMAX>(p, coeffs, cstride, L, q, lds_rows);
+                    request_cells<MODEL, DOMAIN, LMAX, G>(p, f.shared_affine != 0, coeffs, cstride, L, q, lds_rows);
                 }
             }
         }
@@ -431,8 +558,9 @@ void track_fast_kernel(FastLaunch f)
             }
             base = __shfl(base, leader);
             if (L.g >= 0) {
+                store_held(p, L);
                 L.rng = rng_before;
-                save_lane(f.cont_out +
+                save_lane<DOMAIN, G>(f.cont_out +
                               FAST_CONTINUATION_VEC4 *
                                   ((uint64_t)base + (uint32_t)__popcll(live & ((1ull << lane_id) - 1ull))),
                           L);
@@ -441,7 +569,11 @@ void track_fast_kernel(FastLaunch f)
             break;
         }
         if (L.g >= 0)
-            advance<MODEL, ALGO, DOMAIN, LMAX>(p, coeffs, cstride, L, q, lds_rows, nz);
+            advance<MODEL, ALGO, DOMAIN, LMAX, G>(p, f.shared_affine != 0, coeffs, cstride, L, q, lds_rows, nz);
+        rows_read += (uint64_t)__popcll(__ballot(L.row_read));
+        masks_read += (uint64_t)__popcll(__ballot(L.mask_read));
+        L.row_read = false;
+        L.mask_read = false;
         ended_pending += (uint32_t)__popcll(live & ~__ballot(L.g >= 0));
         // Once every claim is made, every eighth iteration: add this
         // wavefront's ended streamlines to its shard and read all shards
@@ -465,15 +597,9 @@ void track_fast_kernel(FastLaunch f)
     __builtin_amdgcn_s_waitcnt(0);
     // The wavefront's memory reads of rows and mask values, one atomic each
     // (the launch's totals price the bytes the cell cache had to fetch).
-    uint64_t rows = L.row_loads, masks = L.mask_loads;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        rows += __shfl_xor(rows, d);
-        masks += __shfl_xor(masks, d);
-    }
     if (lane_id == 0) {
-        atomicAdd(&f.stats[0], (unsigned long long)rows);
-        atomicAdd(&f.stats[1], (unsigned long long)masks);
+        atomicAdd(&f.stats[0], (unsigned long long)rows_read);
+        atomicAdd(&f.stats[1], (unsigned long long)masks_read);
     }
 }
 
@@ -801,7 +927,8 @@ hipError_t launch_fast(const FastLaunch &f, int grid_blocks, bool query, int *bl
         gen.cont_out_count = f.cont_count[g % 2];
         gen.last = g == FAST_GENERATIONS - 1;
         zero_generation<<<1, 64, 0, stream>>>(f.claims, f.ended, gen.cont_out_count, g == 0 ? f.stats : nullptr);
-        track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX><<<grid_blocks, fast_block<LMAX>(), 0, stream>>>(gen);
+        const int blocks_g = std::max(1, grid_blocks >> g);
+        track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX><<<blocks_g, fast_block<LMAX>(), 0, stream>>>(gen);
     }
     return hipGetLastError();
 }

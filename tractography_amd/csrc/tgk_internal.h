@@ -77,7 +77,9 @@ hipError_t launch_normalize_odf(float *coefficients, int64_t n_cells, hipStream_
 // Fast path (tgk_fast.hip): persistent lanes with refill, seed-major output,
 // run as FAST_GENERATIONS launches (generation 0 claims seeds, each later
 // one resumes the streamlines the previous one saved; see track_fast_kernel).
-constexpr int FAST_GENERATIONS = 6;
+// Each generation holds at most half the previous one's work, so generation
+// g runs on grid / 2^g blocks, and the last runs its work to completion.
+constexpr int FAST_GENERATIONS = 4;
 // By default a generation stops early only while it has at least twice
 // this many streamlines; below that, repacking them gains too little.
 constexpr uint32_t FAST_MIN_REPACK = 16384;
@@ -103,6 +105,7 @@ struct FastLaunch {
     uint32_t *cont_out_count;
     int32_t last;               // the last generation runs every streamline to its end
     uint32_t min_repack;        // a generation stops early only while half its work is at least this
+    int32_t shared_affine;      // mask domain whose inverse affine equals the field's (rows 0..2)
     int32_t precision;          // TGK_PRECISION_*
     const void *coeffs;         // transformed field (layout per precision)
     uint32_t coeff_stride;      // elements per voxel in `coeffs`

@@ -12,6 +12,7 @@ point array.
 from __future__ import annotations
 
 import ctypes
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -90,20 +91,40 @@ def track(
             ctypes.byref(handle),
         )
     )
-    try:
-        n = ctypes.c_int64()
-        total = ctypes.c_int64()
-        _native.check(lib.tgk_tracks_size(handle, ctypes.byref(n), ctypes.byref(total)))
-        lengths = np.empty(n.value, dtype=np.uint32)
-        steps = np.empty(n.value, dtype=np.uint32)
-        hits = np.empty(n.value, dtype=np.uint32)
+    owner = _TracksHandle(handle)
+    n = ctypes.c_int64()
+    total = ctypes.c_int64()
+    _native.check(lib.tgk_tracks_size(handle, ctypes.byref(n), ctypes.byref(total)))
+    lengths = np.empty(n.value, dtype=np.uint32)
+    steps = np.empty(n.value, dtype=np.uint32)
+    hits = np.empty(n.value, dtype=np.uint32)
+    in_place = ctypes.c_void_p()
+    _native.check(lib.tgk_tracks_points(handle, ctypes.byref(in_place)))
+    if in_place.value and total.value > 0:
+        # The library's page-locked buffer becomes the points array; the
+        # buffer holds a reference to the handle, freed with the last view.
+        buffer = (ctypes.c_float * (3 * total.value)).from_address(in_place.value)
+        buffer._owner = owner
+        points = np.ctypeslib.as_array(buffer).reshape(-1, 3)
+        _native.check(lib.tgk_tracks_copy(handle, _native.ptr(lengths), _native.ptr(steps), _native.ptr(hits), None))
+    else:
         points = np.empty((total.value, 3), dtype=np.float32)
         _native.check(
             lib.tgk_tracks_copy(handle, _native.ptr(lengths), _native.ptr(steps), _native.ptr(hits), _native.ptr(points))
         )
-    finally:
-        lib.tgk_tracks_destroy(handle)
+        owner.release()
     return TrackResult(lengths, steps, hits, points)
+
+
+class _TracksHandle:
+    """Owns a tgk_tracks handle: destroyed on release() or when the last
+    array viewing its points is collected."""
+
+    def __init__(self, handle: ctypes.c_void_p):
+        self._finalizer = weakref.finalize(self, _native.lib().tgk_tracks_destroy, handle)
+
+    def release(self) -> None:
+        self._finalizer()
 
 
 def tractogram(
