@@ -773,10 +773,9 @@ def seeds_leg(args, tg, torch, cfg, workload: str):
 
 def product_leg(args, tg, torch, cfg, workload: str):
     """Time one TOD or connectome pass over boundary seeds of the workload's
-    mask domain (n_seeds tracks of at most max_length / step steps): the TOD
-    on the plain FP32 path, the connectome on each path of --paths; and the
-    FP32 host restatement (one core) on a bounded sample of the same tracks.
-    Prints one JSON line."""
+    mask domain (n_seeds tracks of at most max_length / step steps) on each
+    path of --paths, and the FP32 host restatement (one core) on a bounded
+    sample of the same tracks. Prints one JSON line."""
     from tractography_amd import _native
     from tractography_amd.domain import _mesh_handle_for
 
@@ -796,13 +795,14 @@ def product_leg(args, tg, torch, cfg, workload: str):
     inverse = np.ascontiguousarray(np.eye(4), dtype=np.float32)
     path_ids = {"plain": _native.PATH_PLAIN, "fast": _native.PATH_FAST}
     if args.product == "tod":
-        paths = ["plain"]
+        paths = [p for p in args.paths.split(",") if p in path_ids]
         out = torch.empty(int(np.prod(dims)) * 45, dtype=torch.float32, device=dev)
 
         def launch(path):
             _native.check(lib.tgk_tod_device(runtime.handle, domain_handle.handle, domain_handle.handle,
-                                             field_handle.handle, algorithm.descriptor(), rng_seed, n, n_steps,
-                                             _native.ptr(dims), _native.ptr(inverse), 0, out.data_ptr()))
+                                             field_handle.handle, algorithm.descriptor(), path_ids[path], rng_seed,
+                                             n, n_steps, _native.ptr(dims), _native.ptr(inverse), 0,
+                                             out.data_ptr()))
     else:
         paths = [p for p in args.paths.split(",") if p in path_ids]
         boundary = _mesh_handle_for(domain, runtime)

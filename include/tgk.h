@@ -281,18 +281,21 @@ int tgk_connectome(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *
  * (X, Y, Z) = output_dims grid. coefficients: (X, Y, Z, 45) float32, C order;
  * with normalize != 0 each cell is scaled as utils.normalize_odf
  * (utils.py:6-22) does. The sums are float atomics, so their rounding depends
- * on the order in which tracks reach a cell, as the reference's do. */
+ * on the order in which tracks reach a cell, as the reference's do. `path`
+ * as for tgk_connectome; on TGK_PATH_FAST the basis is evaluated without
+ * angles (a normalised Legendre recurrence in cos(theta) times powers of
+ * x + iy). */
 int tgk_tod(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
-            const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
+            const tgk_algorithm *algorithm, int32_t path, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
             const uint32_t output_dims[3], const float output_inverse_affine[16], int32_t normalize,
             float *coefficients);
 /* The same on the runtime stream, into device memory the caller owns
  * (X * Y * Z * 45 floats from hipMalloc, cleared by the call: the sums are
  * device float atomics). No allocation, so it may be recorded into a graph. */
 int tgk_tod_device(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
-                   const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
-                   const uint32_t output_dims[3], const float output_inverse_affine[16], int32_t normalize,
-                   float *d_coefficients);
+                   const tgk_algorithm *algorithm, int32_t path, uint32_t rng_seed, int64_t n_seeds,
+                   uint32_t n_steps, const uint32_t output_dims[3], const float output_inverse_affine[16],
+                   int32_t normalize, float *d_coefficients);
 /* Replaces utils.normalize_odf (utils.py:6-22) on n_cells rows of 45 SH
  * coefficients, in place: row / sqrt(4 pi) / row[0] (float64 arithmetic,
  * one rounding to float32), rows with row[0] == 0 set to 0. */

@@ -28,13 +28,14 @@ torch = pytest.importorskip("torch")
 TINY = 2.0**-126
 
 
-def _device_tod(runtime, domain, field, algorithm, rng_seed, n_seeds, n_steps, dims, inverse, normalize=0):
+def _device_tod(runtime, domain, field, algorithm, rng_seed, n_seeds, n_steps, dims, inverse, normalize=0,
+                path=_native.PATH_PLAIN):
     handle = domain.upload(runtime, with_boundary=True)
     dims = np.ascontiguousarray(dims, dtype=np.uint32)
     inverse = np.ascontiguousarray(inverse, dtype=np.float32).reshape(16)
     out = np.full(tuple(int(d) for d in dims) + (45,), np.nan, dtype=np.float32)
     _native.check(_native.lib().tgk_tod(
-        runtime.handle, handle.handle, handle.handle, field.upload(runtime).handle, algorithm.descriptor(),
+        runtime.handle, handle.handle, handle.handle, field.upload(runtime).handle, algorithm.descriptor(), path,
         rng_seed, n_seeds, n_steps, _native.ptr(dims), _native.ptr(inverse), normalize, _native.ptr(out)))
     return out
 
@@ -62,6 +63,17 @@ def assert_same_sums(got, raw, abs_sum, flushes, label):
 
 
 def _scene(case):
+    if case == "dti-volume":
+        # A 24x28x24 volume of the DTI workload's tensors (smooth principal
+        # axes in an ellipsoid): tracks leave the boundary at once, unlike
+        # on the one-voxel-thick slabs below.
+        import bench
+
+        cfg = bench.brain145_dti_transport(seed=3, n_seeds=10, shape=(24, 28, 24))
+        field = tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4)))
+        domain = tg.domain.Image(tg.Nifti1Image(cfg["mask"], np.eye(4)))
+        algorithm = tg.algorithm.Transport(tg.local.DiffusionTensorImaging(), step_size=0.1, gamma=1.0)
+        return domain, field, algorithm, cfg["mask"].shape
     if case.startswith("dti"):
         fod, wm, _ = fixtures.tensor_cross()
         model = tg.local.DiffusionTensorImaging()
@@ -75,6 +87,10 @@ def _scene(case):
     field = tg.field.Image(fod)
     if case.endswith("transport"):
         algorithm = tg.algorithm.Transport(model, step_size=step, gamma=gamma)
+    elif case.endswith("probabilistic"):
+        algorithm = tg.algorithm.Probabilistic(model, step_size=step, seed=9)
+    elif case.endswith("deterministic"):
+        algorithm = tg.algorithm.Deterministic(model, step_size=step)
     else:
         algorithm = tg.algorithm.Diffusion(model, step_size=step, gamma=gamma, noise_variance=0.1, seed=9)
     return domain, field, algorithm, wm.shape
@@ -149,8 +165,8 @@ def test_tod_device_entry_point_and_graph_replays(runtime):
 
     def enqueue():
         return lib.tgk_tod_device(runtime.handle, handle.handle, handle.handle, field_handle.handle,
-                                  algorithm.descriptor(), seed, 2000, 200, _native.ptr(dims), _native.ptr(inverse),
-                                  0, out.data_ptr())
+                                  algorithm.descriptor(), _native.PATH_PLAIN, seed, 2000, 200, _native.ptr(dims),
+                                  _native.ptr(inverse), 0, out.data_ptr())
 
     _native.check(enqueue())
     runtime.finish()
@@ -234,8 +250,8 @@ def test_tod_rejects_invalid_grids_and_domains(runtime):
         dims = np.ascontiguousarray(dims, dtype=np.uint32)
         inverse = np.ascontiguousarray(inverse, dtype=np.float32).reshape(16)
         return lib.tgk_tod(runtime.handle, dom.handle, dom.handle, field.upload(runtime).handle,
-                           algorithm.descriptor(), 1, 10, 10, _native.ptr(dims), _native.ptr(inverse), 0,
-                           _native.ptr(out))
+                           algorithm.descriptor(), _native.PATH_PLAIN, 1, 10, 10, _native.ptr(dims),
+                           _native.ptr(inverse), 0, _native.ptr(out))
 
     assert call([2**31, 2**31, 2**31], np.eye(4)) == _native.ERR_INVALID
     assert b"too large" in lib.tgk_last_error()
@@ -246,3 +262,119 @@ def test_tod_rejects_invalid_grids_and_domains(runtime):
     mesh, _, _, _ = fixtures.shell_scene(n_seeds=10, seed=1)
     assert call([1, 1, 1], np.eye(4), dom=mesh.upload(runtime)) == _native.ERR_INVALID
     assert b"mask domain" in lib.tgk_last_error()
+
+
+Y00 = 1.0 / np.sqrt(4.0 * np.pi)
+
+
+@pytest.mark.parametrize("case", ["sh45-transport", "sh45-diffusion", "sh15-diffusion", "dti-transport",
+                                  "dti-volume", "sh45-probabilistic", "sh45-deterministic"])
+def test_fast_tod_tracks_fp64(runtime, case):
+    # The fast path's TOD against the FP64 restatement, with the plain
+    # path's FP32 arithmetic (the reference's) measured the same way. Tracks
+    # within ~1e-3 voxel of FP64 deposit the same steps in the same cells,
+    # except for steps that close to a cell face or a track's end, so the
+    # grids differ from FP64 by a small fraction of their mass; the fast
+    # path is held to that fraction for the plain path, or 1e-3. A wrong
+    # basis value, sign or index would move a large share of the mass. The
+    # deposited steps (the l = 0 sums over Y_00) are held the same way. On
+    # the one-voxel-thick DTI slab (dti-transport) boundary seeds start at
+    # an unstable equilibrium, where rounding decides whether a track turns
+    # (see the fast connectome test): both FP32 paths are far from FP64
+    # there, and the plain path's distance is the yardstick.
+    domain, field, algorithm, shape = _scene(case)
+    seed = algorithm.get_rng_seed()
+    dims = np.array(shape)
+    inverse = np.eye(4, dtype=np.float32)
+    n, n_steps = 4000, 200
+    fast = _device_tod(runtime, domain, field, algorithm, seed, n, n_steps, dims, inverse, path=_native.PATH_FAST)
+    plain = _device_tod(runtime, domain, field, algorithm, seed, n, n_steps, dims, inverse)
+    ref, _, flushes = oracle.tod(domain, field, algorithm, seed, n, n_steps, dims, inverse, precision=64)
+    mass = float(np.abs(ref).sum())
+    d_fast = float(np.abs(fast.astype(np.float64) - ref).sum()) / mass
+    d_plain = float(np.abs(plain.astype(np.float64) - ref).sum()) / mass
+    steps_ref, steps_fast, steps_plain = (float(x[..., 0].astype(np.float64).sum()) / Y00 for x in (ref, fast, plain))
+    print(f"[fast tod {case}] L1 distance from FP64 over its mass: fast {d_fast:.2e} plain {d_plain:.2e}; "
+          f"deposited steps FP64 {steps_ref:.0f} fast {steps_fast:.0f} plain {steps_plain:.0f}", flush=True)
+    assert flushes.sum() > 1000
+    assert np.isfinite(fast).all()
+    assert d_fast <= max(1e-3, 2.0 * d_plain)
+    assert abs(steps_fast - steps_ref) <= max(1e-3 * steps_ref, 2.0 * abs(steps_plain - steps_ref))
+
+
+def test_fast_tod_basis_of_single_steps(runtime):
+    # One deposit per track: tracks of one step (n_steps 1), so each cell
+    # holds the basis at its tracks' start orientations, the inward normals
+    # of the boundary faces. The fast path's angle-free basis must give the
+    # FP64 basis there, summed over the same tracks (the starts are the same
+    # on both paths), to FP32 rounding.
+    domain, field, algorithm, shape = _scene("sh45-transport")
+    seed = algorithm.get_rng_seed()
+    dims = np.array(shape)
+    inverse = np.eye(4, dtype=np.float32)
+    fast = _device_tod(runtime, domain, field, algorithm, seed, 20000, 1, dims, inverse, path=_native.PATH_FAST)
+    ref, abs_sum, flushes = oracle.tod(domain, field, algorithm, seed, 20000, 1, dims, inverse, precision=64)
+    assert flushes.sum() > 1000
+    # Per cell: 1e-6 of its largest sum of magnitudes for the basis, plus
+    # the float32 rounding of m summed deposits (m ulps). Relative to the
+    # cell, not to each value: on the y axis FP64 gives cos(m pi / 2) =
+    # 6e-17 where the exact value, and the fast basis's, is 0.
+    scale = abs_sum.max(axis=-1, keepdims=True)
+    bound = scale * (1e-6 + flushes[..., None] * 2.0**-23) + 1e-30
+    err = np.abs(fast.astype(np.float64) - ref)
+    print(f"[fast tod single steps] {int(flushes.sum())} deposits; worst error / bound {float(np.max(err / bound)):.3f}",
+          flush=True)
+    assert np.all(err <= bound)
+
+
+def test_fast_tod_at_oblique_orientations(runtime):
+    # A field whose single fibre runs along (1, 2, 3) / sqrt(14): tracks turn
+    # off the axes at once, so every order m of the basis carries weight.
+    shape = (12, 12, 12)
+    rng = np.random.default_rng(5)
+    direction = np.array([1.0, 2.0, 3.0]) / np.sqrt(14.0)
+    import bench
+
+    fod = bench.fod_from_fibres(np.broadcast_to(direction, shape + (3,)).reshape(-1, 1, 3),
+                                np.ones((int(np.prod(shape)), 1)), lmax=8).reshape(*shape, 45).astype(np.float32)
+    mask = np.zeros(shape, np.uint8)
+    mask[1:-1, 1:-1, 1:-1] = 1
+    field = tg.field.Image(tg.Nifti1Image(fod, np.eye(4)))
+    domain = tg.domain.Image(tg.Nifti1Image(mask, np.eye(4)))
+    algorithm = tg.algorithm.Diffusion(tg.local.RealSymmetricSphericalHarmonics(45), step_size=0.1, gamma=1.0,
+                                       noise_variance=0.05, seed=int(rng.integers(1, 1000)))
+    seed = algorithm.get_rng_seed()
+    dims = np.array(shape)
+    inverse = np.eye(4, dtype=np.float32)
+    fast = _device_tod(runtime, domain, field, algorithm, seed, 3000, 150, dims, inverse, path=_native.PATH_FAST)
+    plain = _device_tod(runtime, domain, field, algorithm, seed, 3000, 150, dims, inverse)
+    ref, _, _ = oracle.tod(domain, field, algorithm, seed, 3000, 150, dims, inverse, precision=64)
+    mass = float(np.abs(ref).sum())
+    d_fast = float(np.abs(fast.astype(np.float64) - ref).sum()) / mass
+    d_plain = float(np.abs(plain.astype(np.float64) - ref).sum()) / mass
+    print(f"[fast tod oblique] L1 distance from FP64 over its mass: fast {d_fast:.2e} plain {d_plain:.2e}",
+          flush=True)
+    # Each order m's share of the mass, so that a basis error in any of them shows.
+    for m in range(1, 9):
+        idx = [l * (l + 1) // 2 + s * m for l in range(m + (m & 1), 9, 2) for s in (1, -1)]
+        part = float(np.abs(ref[..., idx]).sum())
+        assert part > 1e-4 * mass, f"order {m} carries no weight"
+        d = float(np.abs(fast[..., idx].astype(np.float64) - ref[..., idx]).sum()) / part
+        dp = float(np.abs(plain[..., idx].astype(np.float64) - ref[..., idx]).sum()) / part
+        assert d <= max(2e-3, 2.0 * dp), f"order {m}: {d:.2e} (plain {dp:.2e})"
+
+
+def test_fast_tod_through_the_public_api():
+    fod, seed_mask, _ = fixtures.cross()
+    algorithm = tg.algorithm.Transport(tg.local.from_shape(fod.shape), step_size=0.1, gamma=1.0)
+    fast = tg.tod(fod, 2000, seed_mask, algorithm, maximum_streamline_length=20.0,
+                  runtime=tg.HipRuntime(path="fast")).get_fdata()
+    plain = tg.tod(fod, 2000, seed_mask, algorithm, maximum_streamline_length=20.0,
+                   runtime=tg.HipRuntime()).get_fdata()
+    populated = fast[..., 0] != 0
+    assert np.any(populated)
+    np.testing.assert_allclose(fast[..., 0][populated], Y00, rtol=1e-6)
+    # Normalised cells: the same cells populated, shapes within tracking noise.
+    assert (populated == (plain[..., 0] != 0)).mean() > 0.99
+    both = populated & (plain[..., 0] != 0)
+    assert float(np.abs(fast[both] - plain[both]).mean()) < 1e-2

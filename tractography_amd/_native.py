@@ -98,9 +98,10 @@ _SIGNATURES = {
     "tgk_domain_seed": (c_int, [_P, _P, c_uint32, c_int64, _P]),
     "tgk_connectome": (c_int, [_P, _P, _P, _P, POINTER(tgk_algorithm), c_int32, c_uint32, c_int64, c_uint32, c_float,
                                _P, _P, _P]),
-    "tgk_tod": (c_int, [_P, _P, _P, _P, POINTER(tgk_algorithm), c_uint32, c_int64, c_uint32, _P, _P, c_int32, _P]),
-    "tgk_tod_device": (c_int, [_P, _P, _P, _P, POINTER(tgk_algorithm), c_uint32, c_int64, c_uint32, _P, _P, c_int32,
-                               _P]),
+    "tgk_tod": (c_int, [_P, _P, _P, _P, POINTER(tgk_algorithm), c_int32, c_uint32, c_int64, c_uint32, _P, _P, c_int32,
+                        _P]),
+    "tgk_tod_device": (c_int, [_P, _P, _P, _P, POINTER(tgk_algorithm), c_int32, c_uint32, c_int64, c_uint32, _P, _P,
+                               c_int32, _P]),
     "tgk_normalize_odf": (c_int, [_P, c_int64, _P]),
     "tgk_seeds_from_fod": (c_int, [_P, c_int32, c_int32, _P, _P, c_uint32, _P, _P, c_int64, c_int32, _P]),
     "tgk_seeds_prepare": (c_int, [_P, c_int32, c_int32, c_int32]),

@@ -1499,15 +1499,17 @@ namespace {
 
 // Checks shared by the TOD entry points; fills the launch on success.
 int tod_launch(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
-               const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
+               const tgk_algorithm *algorithm, int32_t path, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
                const uint32_t output_dims[3], const float output_inverse_affine[16], tgk::TodLaunch *t,
                int64_t *n_values)
 {
     if (!rt || !domain || !boundary || !field || n_seeds < 0 || !output_dims || !output_inverse_affine)
         return fail(TGK_ERR_INVALID, "bad arguments");
+    if (path != TGK_PATH_PLAIN && path != TGK_PATH_FAST)
+        return fail(TGK_ERR_INVALID, "unknown path %d", path);
     if (n_seeds > MAX_SEEDS)
         return fail(TGK_ERR_INVALID, "%lld seeds: at most %lld per call", (long long)n_seeds, (long long)MAX_SEEDS);
-    if (int r = check_algorithm(algorithm, field))
+    if (int r = check_algorithm(algorithm, field, path == TGK_PATH_FAST ? 91 : 45))
         return r;
     if (int r = check_owner(rt, domain, field, boundary))
         return r;
@@ -1547,11 +1549,25 @@ int tod_launch(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boun
 }
 
 // Zero, deposit, normalise: kernels only, on the runtime stream.
-int enqueue_tod(tgk_runtime *rt, tgk::TodLaunch &t, int64_t n_values, int32_t normalize, float *d_coefficients)
+int enqueue_tod(tgk_runtime *rt, tgk_field *field, const tgk_algorithm *algorithm, int32_t path, tgk::TodLaunch &t,
+                int64_t n_values, int32_t normalize, float *d_coefficients)
 {
     t.coefficients = d_coefficients;
+    tgk::FastTodLaunch f;
+    if (path == TGK_PATH_FAST) {
+        memset(&f, 0, sizeof f);
+        if (int r = ensure_fast_field(rt, field, algorithm, &f.sh_order))
+            return r;
+        f.coeffs = field->fast.ptr;
+        f.coeff_stride = field->fast_stride;
+    }
     TGK_HIP(tgk::launch_zero(d_coefficients, n_values, rt->stream), "TOD clear");
-    TGK_HIP(tgk::launch_tod(t, rt->stream), "TOD kernel");
+    if (path == TGK_PATH_FAST) {
+        f.tod = t;
+        TGK_HIP(tgk::launch_tod_fast(f, rt->stream), "fast TOD kernel");
+    } else {
+        TGK_HIP(tgk::launch_tod(t, rt->stream), "TOD kernel");
+    }
     if (normalize)
         TGK_HIP(tgk::launch_normalize_odf(d_coefficients, n_values / 45, rt->stream), "TOD normalisation");
     return TGK_OK;
@@ -1562,13 +1578,13 @@ int enqueue_tod(tgk_runtime *rt, tgk::TodLaunch &t, int64_t n_values, int32_t no
 extern "C" {
 
 int tgk_tod(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
-            const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
+            const tgk_algorithm *algorithm, int32_t path, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
             const uint32_t output_dims[3], const float output_inverse_affine[16], int32_t normalize,
             float *coefficients)
 {
     tgk::TodLaunch t;
     int64_t n_values = 0;
-    if (int r = tod_launch(rt, domain, boundary, field, algorithm, rng_seed, n_seeds, n_steps, output_dims,
+    if (int r = tod_launch(rt, domain, boundary, field, algorithm, path, rng_seed, n_seeds, n_steps, output_dims,
                            output_inverse_affine, &t, &n_values))
         return r;
     if (n_values > 0 && !coefficients)
@@ -1581,19 +1597,19 @@ int tgk_tod(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundar
     float *dout;
     if (int r = st.out((size_t)n_values, &dout))
         return r;
-    if (int r = enqueue_tod(rt, t, n_values, normalize, dout))
+    if (int r = enqueue_tod(rt, const_cast<tgk_field *>(field), algorithm, path, t, n_values, normalize, dout))
         return r;
     return fetch(coefficients, dout, (size_t)n_values, rt->stream);
 }
 
 int tgk_tod_device(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *boundary, const tgk_field *field,
-                   const tgk_algorithm *algorithm, uint32_t rng_seed, int64_t n_seeds, uint32_t n_steps,
-                   const uint32_t output_dims[3], const float output_inverse_affine[16], int32_t normalize,
-                   float *d_coefficients)
+                   const tgk_algorithm *algorithm, int32_t path, uint32_t rng_seed, int64_t n_seeds,
+                   uint32_t n_steps, const uint32_t output_dims[3], const float output_inverse_affine[16],
+                   int32_t normalize, float *d_coefficients)
 {
     tgk::TodLaunch t;
     int64_t n_values = 0;
-    if (int r = tod_launch(rt, domain, boundary, field, algorithm, rng_seed, n_seeds, n_steps, output_dims,
+    if (int r = tod_launch(rt, domain, boundary, field, algorithm, path, rng_seed, n_seeds, n_steps, output_dims,
                            output_inverse_affine, &t, &n_values))
         return r;
     if (n_values > 0 && !d_coefficients)
@@ -1602,7 +1618,7 @@ int tgk_tod_device(tgk_runtime *rt, const tgk_domain *domain, const tgk_domain *
         return TGK_OK;
     if (int r = select_device(rt))
         return r;
-    return enqueue_tod(rt, t, n_values, normalize, d_coefficients);
+    return enqueue_tod(rt, const_cast<tgk_field *>(field), algorithm, path, t, n_values, normalize, d_coefficients);
 }
 
 int tgk_normalize_odf(tgk_runtime *rt, int64_t n_cells, float *coefficients)

@@ -208,14 +208,14 @@ template <int MODEL, int LMAX> constexpr int lds_vectors() { return rows_in_lds<
 
 // A cell's row into the lane's cache on a cell change: order-8 SH rows to
 // LDS (the last TERMS - 4 LDS_VEC coefficients to registers), others to
-// registers.
-template <int MODEL, int LMAX>
+// registers; with LDS false, every row to registers.
+template <int MODEL, int LMAX, bool LDS = rows_in_lds<MODEL, LMAX>()>
 __device__ __forceinline__ void load_row(const float *__restrict__ coeffs, int64_t cell, uint32_t cstride, float *q,
                                          float4 *__restrict__ lds_rows)
 {
     constexpr int TERMS = Order<LMAX>::TERMS;
     constexpr int VEC = Order<LMAX>::STRIDE / 4;
-    constexpr int LDS_VEC = lds_vectors<MODEL, LMAX>();
+    constexpr int LDS_VEC = LDS ? lds_vectors<MODEL, LMAX>() : 0;
     const float4 *row = reinterpret_cast<const float4 *>(coeffs + cell * (int64_t)cstride);
     if constexpr (MODEL == MODEL_SH) {
         // Straight from memory into LDS (no registers held while the loads
@@ -714,6 +714,122 @@ __global__ __launch_bounds__(fast_block<LMAX>()) void connectome_fast_kernel(Fas
     }
 }
 
+// tracking.py:262-357 (kernel tod) on the fast model: the plain TOD
+// kernel's loop (tod_kernel: mask, field and output cell, flush on leaving a
+// cell, deposit, update, finite and length test, move, flush at the end)
+// with persistent lanes and the polynomial model. The deposit is the
+// reference's 45-term SH basis at the step's orientation (fast::sh45_basis,
+// from z and powers of x + iy instead of angles), added to the lane's run
+// (acc = Y on entering a cell, acc += Y after); runs are flushed through the
+// LDS stages of flush_runs. The row stays in registers: the LDS holds the
+// stages.
+template <int MODEL, int ALGO, int LMAX>
+__global__ __launch_bounds__(256) void tod_fast_kernel(FastTodLaunch f)
+{
+    __shared__ RunStage stages[4];
+    RunStage &stage = stages[threadIdx.x >> 6];
+    const TodLaunch &t = f.tod;
+    const float *coeffs = static_cast<const float *>(f.coeffs);
+    const uint32_t cstride = f.coeff_stride;
+    const float step = t.algo.step_size;
+    const int lane_id = __lane_id();
+
+    int64_t g = -1;
+    bool drained = false;
+    uint2 rng = make_uint2(0u, 0u);
+    V3 point = v3(0.0f, 0.0f, 0.0f), o = point;
+    uint32_t k = 0;
+    int64_t previous = -1;
+    int64_t field_cell = -2, cached_mask_cell = -2;
+    bool mask_inside = false;
+    float q[MODEL == MODEL_SH ? Order<LMAX>::TERMS : 6];
+    float acc[45];
+#pragma unroll
+    for (int i = 0; i < 45; i++)
+        acc[i] = 0.0f;
+
+    while (true) {
+        // Refill idle lanes from the queue, one atomic per wavefront.
+        const uint64_t need = __ballot(g < 0);
+        if (need != 0 && !drained) {
+            const int leader = __ffsll((unsigned long long)need) - 1;
+            const uint32_t count = (uint32_t)__popcll(need);
+            uint32_t base = 0;
+            if (lane_id == leader)
+                base = atomicAdd(t.queue, count);
+            base = __shfl(base, leader);
+            if ((uint64_t)base + count >= (uint64_t)t.n_seeds)
+                drained = true;
+            if (g < 0) {
+                const uint64_t claimed = (uint64_t)base + (uint64_t)__popcll(need & ((1ull << lane_id) - 1ull));
+                if (claimed < (uint64_t)t.n_seeds) {
+                    g = (int64_t)claimed;
+                    rng = rng_init(t.rng_seed, (uint32_t)g);
+                    const BoundarySample sample =
+                        sample_boundary(t.vertices, t.triangles, t.cumulative, t.n_triangles, rng);
+                    point = sample.point;
+                    o = sample.orientation;
+                    k = 0;
+                    previous = -1;
+                    field_cell = -2;
+                    cached_mask_cell = -2;
+                    mask_inside = false;
+                }
+            }
+        }
+        if (__ballot(g >= 0) == 0)
+            break;
+        const bool live = g >= 0;
+        bool stop = false;
+        int64_t voxel = -1, cell = -1;
+        if (live) {
+            stop = k >= t.n_steps;
+            if (!stop) {
+                const int64_t mc = mask_cell(t.mask, point);
+                if (mc != cached_mask_cell) {
+                    cached_mask_cell = mc;
+                    mask_inside = mc >= 0 && t.mask.mask[mc] != 0;
+                }
+                stop = !mask_inside;
+            }
+            if (!stop) {
+                voxel = field_voxel(t.field, point);
+                stop = voxel < 0;
+            }
+            if (!stop)
+                cell = output_cell(t.grid, point);  // -1 off the grid
+        }
+        // Leaving a cell (for another or for no cell) ends its run.
+        flush_runs(live && !stop && previous >= 0 && cell != previous, previous, acc, t.coefficients, stage);
+        if (live && !stop) {
+            const bool in_grid = cell >= 0;
+            // On the grid: a new run starts from the basis (keep 0) or the
+            // run goes on (keep 1); off it, the run is left as it is.
+            const float keep = in_grid && cell != previous ? 0.0f : 1.0f;
+            previous = cell;
+            fast::sh45_basis(o, in_grid ? 1.0f : 0.0f, [&](int i, float y) { acc[i] = fmaf(acc[i], keep, y); });
+            if (voxel != field_cell) {
+                field_cell = voxel;
+                load_row<MODEL, LMAX, false>(coeffs, voxel, cstride, q, nullptr);
+            }
+            o = update_orientation<MODEL, ALGO, LMAX>(t.algo, q, o, rng);
+            stop = !(__builtin_isfinite(o.x) && __builtin_isfinite(o.y) && __builtin_isfinite(o.z)) ||
+                   length3(o) < 0.5f;
+            if (!stop) {
+                point.x += step * o.x;
+                point.y += step * o.y;
+                point.z += step * o.z;
+                k++;
+            }
+        }
+        // The end of a track ends its run.
+        const bool ended = live && stop;
+        flush_runs(ended && previous >= 0, previous, acc, t.coefficients, stage);
+        if (ended)
+            g = -1;
+    }
+}
+
 __global__ void zero_counter(uint32_t *counter) { *counter = 0u; }
 
 // A generation's claim counter, ended-streamline shards and saved-streamline
@@ -1025,6 +1141,57 @@ hipError_t launch_connectome_fast_algo(const FastConnectomeLaunch &f, hipStream_
     }
 }
 
+// TOD launches: the persistent grid of the kernel's occupancy (cached per
+// device), at most one lane per seed.
+template <int MODEL, int ALGO, int LMAX> hipError_t launch_tod_fast_kernel(const FastTodLaunch &f, hipStream_t stream)
+{
+    static std::atomic<int> cache[64] = {};
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess)
+        return e;
+    if (device < 0 || device >= 64)
+        return hipErrorInvalidDevice;
+    if (cache[device].load(std::memory_order_acquire) == 0) {
+        int per_cu = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(tod_fast_kernel<MODEL, ALGO, LMAX>), 256, 0);
+        if (e != hipSuccess)
+            return e;
+        int cus = 0;
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+        if (e != hipSuccess)
+            return e;
+        cache[device].store((cus > 0 ? cus : 1) * (per_cu > 0 ? per_cu : 1), std::memory_order_release);
+    }
+    const int64_t fill = cache[device].load(std::memory_order_acquire);
+    const int64_t needed = (f.tod.n_seeds + 255) / 256;
+    zero_counter<<<1, 1, 0, stream>>>(f.tod.queue);
+    tod_fast_kernel<MODEL, ALGO, LMAX><<<(unsigned)std::min(fill, needed), 256, 0, stream>>>(f);
+    return hipGetLastError();
+}
+
+template <int MODEL, int ALGO>
+hipError_t launch_tod_fast_order(const FastTodLaunch &f, hipStream_t stream)
+{
+    if constexpr (MODEL == MODEL_SH) {
+        if (f.sh_order == 12)
+            return launch_tod_fast_kernel<MODEL, ALGO, 12>(f, stream);
+    }
+    return launch_tod_fast_kernel<MODEL, ALGO, 8>(f, stream);
+}
+
+template <int MODEL>
+hipError_t launch_tod_fast_algo(const FastTodLaunch &f, hipStream_t stream)
+{
+    switch (f.tod.algo.algorithm) {
+    case ALGO_DIFFUSION: return launch_tod_fast_order<MODEL, ALGO_DIFFUSION>(f, stream);
+    case ALGO_PROBABILISTIC: return launch_tod_fast_order<MODEL, ALGO_PROBABILISTIC>(f, stream);
+    case ALGO_DETERMINISTIC: return launch_tod_fast_order<MODEL, ALGO_DETERMINISTIC>(f, stream);
+    default: return launch_tod_fast_order<MODEL, ALGO_TRANSPORT>(f, stream);
+    }
+}
+
 }  // namespace
 
 hipError_t launch_track_fast(const FastLaunch &f, int grid_blocks, hipStream_t stream)
@@ -1048,6 +1215,15 @@ hipError_t launch_connectome_fast(const FastConnectomeLaunch &f, hipStream_t str
     if (f.connectome.algo.model == MODEL_DTI)
         return launch_connectome_fast_algo<MODEL_DTI>(f, stream);
     return launch_connectome_fast_algo<MODEL_SH>(f, stream);
+}
+
+hipError_t launch_tod_fast(const FastTodLaunch &f, hipStream_t stream)
+{
+    if (f.tod.n_seeds == 0)
+        return hipSuccess;
+    if (f.tod.algo.model == MODEL_DTI)
+        return launch_tod_fast_algo<MODEL_DTI>(f, stream);
+    return launch_tod_fast_algo<MODEL_SH>(f, stream);
 }
 
 hipError_t launch_compact_seed_major(const float *dense, int64_t n_seeds, uint32_t n_points,

@@ -586,5 +586,109 @@ __device__ __forceinline__ V3 deterministic(const AlgoParams &a, const float *q,
     return o;
 }
 
+// The reference's 45-term real symmetric SH basis (order 8, MRtrix order:
+// index l(l+1)/2 + m for even l; spharm.cl:10-207, core.py:36-69) at the
+// direction of o, without angles. With z = cos(theta) and u = (x + iy) / r =
+// sin(theta) e^{i phi}, Y_{l,0} = L_l^0(z), Y_{l,+-m} = sqrt(2) L_l^m(z)
+// (Re, Im) u^m, where L_l^m = N_lm P_l^m / sin^m(theta) (N_lm the
+// normalisation, P_l^m with the Condon-Shortley phase) is a polynomial in z
+// of degree l - m, run up in l for each m by the normalised recurrence
+//   L_m^m = N_mm (-1)^m (2m-1)!!,
+//   L_l^m = alpha_lm z L_{l-1}^m - beta_lm L_{l-2}^m,
+//   alpha_lm = (2l-1)/(l-m) N_lm/N_{l-1,m}, beta_lm = (l+m-1)/(l-m) N_lm/N_{l-2,m},
+// from P_l^m = ((2l-1) z P_{l-1}^m - (l+m-1) P_{l-2}^m) / (l-m). Every value
+// is scaled by w; the zero vector maps to the pole, as cart2sph does.
+// visit(index, value) receives each of the 45 values.
+namespace basis {
+
+constexpr double csqrt(double x)
+{
+    if (!(x > 0.0))
+        return 0.0;
+    double r = x > 1.0 ? x : 1.0;
+    for (int i = 0; i < 200; i++)
+        r = 0.5 * (r + x / r);
+    return r;
+}
+constexpr double factorial(int n)
+{
+    double f = 1.0;
+    for (int i = 2; i <= n; i++)
+        f *= i;
+    return f;
+}
+constexpr double norm(int l, int m)
+{
+    return csqrt((2.0 * l + 1.0) / (4.0 * 3.14159265358979323846) * factorial(l - m) / factorial(l + m));
+}
+// L_m^m, with the sqrt(2) of the m > 0 harmonics folded in.
+constexpr double start(int m)
+{
+    double dfact = 1.0;
+    for (int k = 2 * m - 1; k > 1; k -= 2)
+        dfact *= k;
+    return (m > 0 ? csqrt(2.0) : 1.0) * norm(m, m) * ((m & 1) ? -dfact : dfact);
+}
+constexpr double alpha(int l, int m) { return (2.0 * l - 1.0) / (l - m) * norm(l, m) / norm(l - 1, m); }
+constexpr double beta(int l, int m) { return l - m >= 2 ? (l + m - 1.0) / (l - m) * norm(l, m) / norm(l - 2, m) : 0.0; }
+
+template <int M, int L, class Visit>
+__device__ __forceinline__ void order_step(float z, float ur, float ui, float &prev2, float &prev, Visit &visit)
+{
+    if constexpr (L <= 8) {
+        constexpr float a = (float)alpha(L, M);
+        constexpr float b = (float)beta(L, M);
+        const float cur = L - M >= 2 ? a * z * prev - b * prev2 : a * z * prev;
+        prev2 = prev;
+        prev = cur;
+        if constexpr ((L & 1) == 0) {
+            constexpr int centre = L * (L + 1) / 2;
+            if constexpr (M == 0) {
+                visit(centre, cur);
+            } else {
+                visit(centre + M, cur * ur);
+                visit(centre - M, cur * ui);
+            }
+        }
+        order_step<M, L + 1>(z, ur, ui, prev2, prev, visit);
+    }
+}
+
+template <int M, class Visit>
+__device__ __forceinline__ void orders(float z, float ux, float uy, float w, float ur, float ui, Visit &visit)
+{
+    if constexpr (M <= 8) {
+        constexpr float lmm = (float)start(M);
+        float prev2 = 0.0f, prev = w * lmm;
+        if constexpr ((M & 1) == 0) {
+            constexpr int centre = M * (M + 1) / 2;
+            if constexpr (M == 0) {
+                visit(centre, prev);
+            } else {
+                visit(centre + M, prev * ur);
+                visit(centre - M, prev * ui);
+            }
+        }
+        order_step<M, M + 1>(z, ur, ui, prev2, prev, visit);
+        // u^(M+1) from u^M.
+        orders<M + 1>(z, ux, uy, w, ur * ux - ui * uy, ur * uy + ui * ux, visit);
+    }
+}
+
+}  // namespace basis
+
+template <class Visit> __device__ __forceinline__ void sh45_basis(V3 o, float w, Visit &&visit)
+{
+    const float r2 = o.x * o.x + o.y * o.y + o.z * o.z;
+    float z = 1.0f, ux = 0.0f, uy = 0.0f;
+    if (r2 > 0.0f) {
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        z = fminf(fmaxf(o.z * inv, -1.0f), 1.0f);
+        ux = o.x * inv;
+        uy = o.y * inv;
+    }
+    basis::orders<0>(z, ux, uy, w, 1.0f, 0.0f, visit);
+}
+
 }  // namespace fast
 }  // namespace tgk

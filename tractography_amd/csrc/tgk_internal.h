@@ -66,6 +66,14 @@ struct TodLaunch {
     uint32_t *queue;         // seed counter of the persistent lanes (zeroed by the launch)
 };
 hipError_t launch_tod(const TodLaunch &t, hipStream_t stream);
+// The TOD on the fast path (tgk_fast.hip): `coeffs` as in FastLaunch.
+struct FastTodLaunch {
+    TodLaunch tod;
+    const void *coeffs;
+    uint32_t coeff_stride;
+    int32_t sh_order;
+};
+hipError_t launch_tod_fast(const FastTodLaunch &f, hipStream_t stream);
 // Fill n floats with +0 (a kernel, so recorded graphs hold no memset node).
 hipError_t launch_zero(float *out, int64_t n, hipStream_t stream);
 // Rows of K packed floats (src) -> rows of `stride` floats (dst), pad = +0.

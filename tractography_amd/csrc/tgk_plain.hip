@@ -441,62 +441,6 @@ __global__ __launch_bounds__(256) void connectome_hits_kernel(ConnectomeLaunch c
     }
 }
 
-// Output-grid cell of p (tracking.py:290-300): floor(inverse_affine p + 0.5)
-// per axis, C-order linear index, -1 off the grid.
-__device__ __forceinline__ int64_t grid_cell(const MaskView &grid, V3 p)
-{
-    const float rx = floorf(affine_row(grid.inv, 0, p) + 0.5f);
-    const float ry = floorf(affine_row(grid.inv, 1, p) + 0.5f);
-    const float rz = floorf(affine_row(grid.inv, 2, p) + 0.5f);
-    if (!((rx >= 0.0f) & (rx < (float)grid.dims[0]) & (ry >= 0.0f) & (ry < (float)grid.dims[1]) & (rz >= 0.0f) &
-          (rz < (float)grid.dims[2])))
-        return -1;
-    return ((int64_t)(uint32_t)rx * grid.dims[1] + (uint32_t)ry) * grid.dims[2] + (uint32_t)rz;
-}
-
-// Run sums staged in LDS for one wavefront: up to one record (45 sums and
-// the cell) per lane. Stride 45 (odd) keeps the lanes' writes on distinct
-// banks; the drain reads the records as one linear array.
-struct RunStage {
-    float vals[64 * 45];
-    int64_t cell[64];
-};
-
-// Flushes the runs of the lanes with `need` (wavefront-converged call): each
-// such lane writes its record to the stage, then the whole wavefront adds
-// the records to the grid with hardware float atomics (global_atomic_add_f32,
-// no return), 64 consecutive (record, coefficient) pairs per instruction, so
-// one record's 45 adds touch two or three cache lines in one or two
-// instructions instead of 45 instructions of scattered lanes. The reference
-// adds the same values one at a time (compare-and-swap loop, core.cl:80-99);
-// either way the order in which runs of different tracks land is the
-// device's.
-__device__ __forceinline__ void flush_runs(bool need, int64_t cell, const float acc[45], float *__restrict__ grid,
-                                           RunStage &stage)
-{
-    const uint64_t mask = __ballot(need);
-    if (mask == 0)
-        return;
-    const int lane = __lane_id();
-    if (need) {
-        const int slot = __popcll(mask & ((1ull << lane) - 1ull));
-#pragma unroll
-        for (int i = 0; i < 45; i++)
-            stage.vals[slot * 45 + i] = acc[i];
-        stage.cell[slot] = cell;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const int total = __popcll(mask) * 45;
-    for (int p = lane; p < total; p += 64) {
-        const int r = p / 45;
-        unsafeAtomicAdd(grid + stage.cell[r] * 45 + (p - r * 45), stage.vals[p]);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // tracking.py:262-357 (kernel tod): boundary seed g from rng_init(rng_seed,
 // g), whose state then drives the noise; each step checks the mask and the
@@ -553,7 +497,7 @@ void tod_kernel(TodLaunch t)
                 stop = voxel < 0;
             }
             if (!stop)
-                cell = grid_cell(t.grid, point);  // -1 off the grid
+                cell = output_cell(t.grid, point);  // -1 off the grid
         }
         // Leaving a cell (for another or for no cell) ends its run.
         flush_runs(live && !stop && previous >= 0 && cell != previous, previous, acc, t.coefficients, stage);

@@ -226,7 +226,7 @@ def tod(
     _native.check(
         _native.lib().tgk_tod(
             runtime.handle, domain_handle.handle, domain_handle.handle, field_handle.handle, algorithm.descriptor(),
-            int(algorithm.get_rng_seed()), int(n_seeds), min(n_steps, 2**32 - 1), _native.ptr(dims),
+            runtime.path_id, int(algorithm.get_rng_seed()), int(n_seeds), min(n_steps, 2**32 - 1), _native.ptr(dims),
             _native.ptr(inverse_affine), 1, _native.ptr(out),
         )
     )
