@@ -906,9 +906,12 @@ def main():
     parser.add_argument("--paths", default="fast,plain")
     parser.add_argument("--fast-grid", type=int, default=0,
                         help="blocks of the fast kernel (default: occupancy-sized)")
+    parser.add_argument("--fast-shape", type=int, default=0,
+                        help="launch shape of the order-8 SH Transport / Diffusion fast kernels "
+                             "(tractography_amd.runtime.fast_shapes(); default 0)")
     parser.add_argument("--autotune", action="store_true",
-                        help="choose the fast kernel's grid and repack threshold on 100k seeds of the workload "
-                             "before timing")
+                        help="choose the fast kernel's launch shape, grid and repack threshold on 100k seeds of "
+                             "the workload before timing")
     parser.add_argument("--repack", type=int, default=0,
                         help="fast path: streamlines below which a generation is no longer cut short "
                              "(default: the library's, 16384)")
@@ -979,8 +982,10 @@ def main():
                                       maximum_streamline_length=cfg["max_length"], cache=False)
         args.fast_grid = tuned["blocks"]
         args.repack = tuned["repack"]
+        args.fast_shape = tuned["shape"]
         print(f"[bench] autotuned: {tuned}", file=sys.stderr, flush=True)
     runtime.set_fast_grid(args.fast_grid)
+    runtime.set_fast_shape(args.fast_shape)
     if args.repack:
         runtime.set_fast_repack(args.repack)
     run = DeviceRun(tg, torch, runtime, cfg)
@@ -1030,7 +1035,8 @@ def main():
         "data": "synthetic (seeded numpy generators)",
         "config": {"workload": workload, "path": headline, "seeds": len(cfg["seeds"]),
                    "orientation_updates_per_step": h["steps"],
-                   "fast_launch": {"grid_blocks": args.fast_grid or "occupancy",
+                   "fast_launch": {"shape": tg.runtime.fast_shapes()[args.fast_shape],
+                                   "grid_blocks": args.fast_grid or "occupancy",
                                    "repack": args.repack or "default",
                                    "autotuned": tuned is not None}},
         "paths": {k: {"steps_per_s": v["steps"] / (v["ms_per_step"] * 1e-3), "ms_per_step": v["ms_per_step"],
@@ -1141,6 +1147,8 @@ def traffic_leg(args):
                 cmd += ["--seeds", str(args.seeds)]
             if args.fast_grid:
                 cmd += ["--fast-grid", str(args.fast_grid)]
+            if args.fast_shape:
+                cmd += ["--fast-shape", str(args.fast_shape)]
             if args.repack:
                 cmd += ["--repack", str(args.repack)]
             print(f"[bench] counter run {counter}", file=sys.stderr, flush=True)

@@ -90,13 +90,22 @@ int tgk_runtime_synchronize(tgk_runtime *rt);
 int tgk_runtime_info(tgk_runtime *rt, char *name, int32_t name_len, int32_t *compute_units, int64_t *total_bytes);
 /* The runtime's HIP stream (hipStream_t) for callers sharing device buffers. */
 int tgk_runtime_stream(tgk_runtime *rt, void **stream);
-/* Grid of the FAST path's persistent kernel in blocks of 256 lanes; <= 0
- * (the default) sizes it to the device's occupancy. No counterpart in the
- * reference, whose kernel is launched one work-item per seed; set by
- * tractography_amd.autotune. */
+/* Grid of the FAST path's persistent kernel in blocks (of the launch
+ * shape's lanes, 256 by default); <= 0 (the default) sizes it to the
+ * device's occupancy. No counterpart in the reference, whose kernel is
+ * launched one work-item per seed; set by tractography_amd.autotune. */
 int tgk_runtime_set_fast_grid(tgk_runtime *rt, int32_t blocks);
+/* No counterpart in the reference (one work-item per seed, tracking.py:174).
+ * Launch shape of the FAST path's order-8 SH Transport and Diffusion kernels:
+ * 0..TGK_FAST_SHAPES - 1, each a number of lanes per block (streamlines per
+ * workgroup) and of a row's 45 coefficients kept in LDS between steps (the
+ * rest in registers); tgk_fast_shape_info gives both. Other kernels run
+ * shape 0. Results do not depend on it; set by tractography_amd.autotune. */
+#define TGK_FAST_SHAPES 5
+int tgk_runtime_set_fast_shape(tgk_runtime *rt, int32_t shape);
+int tgk_fast_shape_info(int32_t shape, int32_t *lanes_per_block, int32_t *lds_coefficients);
 /* No counterpart in the reference. The FAST path runs a tracking call as up
- * to six launches ("generations"): once a generation's streamlines are all
+ * to four launches ("generations"): once a generation's streamlines are all
  * started and fewer than half of them are still in flight, it saves those
  * and the next generation resumes them packed into full wavefronts. A
  * generation is cut short only while half its streamlines number at least

@@ -46,3 +46,20 @@ def test_malformed_cache_reads_as_empty(tmp_path):
     assert autotune.load_cache(path) == {}
     path.write_text(json.dumps([1, 2, 3]))
     assert autotune.load_cache(path) == {}
+
+
+def test_launch_shapes_and_grid_candidates():
+    # The shapes the library compiles (lanes per block, coefficients in LDS)
+    # and the grids tried for each: the same lanes per CU whatever the block.
+    shapes = tg.runtime.fast_shapes()
+    assert [(s["lanes_per_block"], s["lds_coefficients"]) for s in shapes] == [
+        (256, 40), (128, 40), (512, 40), (256, 32), (256, 44)]
+    assert autotune.DEFAULT_SHAPES == tuple(range(len(shapes)))
+    from tractography_amd import _native
+
+    assert _native.lib().tgk_fast_shape_info(len(shapes), None, None) == _native.ERR_INVALID
+    assert _native.lib().tgk_fast_shape_info(-1, None, None) == _native.ERR_INVALID
+    for s in shapes:
+        grids = autotune._grids(256, s["lanes_per_block"], (0.5, 1.0, 2.0))
+        assert grids[0] == 0
+        assert [g * s["lanes_per_block"] for g in grids[1:]] == [131072, 262144, 524288]

@@ -341,6 +341,40 @@ def test_fast_path_refuses_16_bit_coefficient_storage(scene, precision):
     assert len(got.streamlines) == len(seeds)
 
 
+@pytest.mark.parametrize("case", ["transport", "diffusion", "mesh"])
+def test_fast_results_do_not_depend_on_the_launch_shape(scene, case):
+    # Lanes per block and the coefficients kept in LDS (32, 40 or 44 of 45)
+    # change where a lane's data waits, not its arithmetic: every shape
+    # gives shape 0's results bit for bit.
+    cfg, field, domain = scene
+    seeds = cfg["seeds"]
+    sh45 = tg.local.RealSymmetricSphericalHarmonics(45)
+    if case == "transport":
+        algorithm = tg.algorithm.Transport(sh45, step_size=0.1, gamma=1.0)
+    else:
+        algorithm = tg.algorithm.Diffusion(sh45, step_size=0.1, gamma=1.0, noise_variance=0.1, seed=58)
+    if case == "mesh":
+        domain, seeds, _, _ = fixtures.shell_scene(n_seeds=2000, seed=13)
+    rng = [algorithm.get_rng_seed()]
+    results = []
+    shapes = tg.runtime.fast_shapes() + [tg.runtime.fast_shapes()[0]]  # shape 0 again: a control
+    for shape in shapes:
+        runtime = tg.HipRuntime(path="fast")
+        runtime.set_fast_shape(shape["shape"])
+        results.append(tg.tracking.track(algorithm, domain, field, seeds, rng, len(seeds), 0.25, 80.0, runtime))
+    first = results[0]
+    assert first.steps.sum() > 10000
+    differing = []
+    for shape, got in zip(shapes[1:], results[1:]):
+        seeds_differing = int(np.sum((got.lengths != first.lengths) | (got.steps != first.steps)))
+        print(f"\n[shape] {case} {shape}: {seeds_differing} seeds differ from shape 0", flush=True)
+        differing.append(seeds_differing)
+    assert differing == [0] * len(differing)
+    for got in results[1:]:
+        np.testing.assert_array_equal(got.hits, first.hits)
+        np.testing.assert_array_equal(got.points.view(np.uint32), first.points.view(np.uint32))
+
+
 @pytest.mark.parametrize("case", ["sh45-diffusion", "sh45-probabilistic", "mesh", "dti", "sh91-diffusion"])
 def test_fast_generations_change_no_result(scene, case):
     # A fast-path call runs as generations of the persistent kernel: when a
@@ -397,12 +431,15 @@ def test_tune_fast_caches_its_choice_and_changes_no_result(scene, tmp_path):
     runtime = tg.HipRuntime(path="fast")
     cache = tmp_path / "tune.json"
     kwargs = dict(runtime=runtime, save_at=0.25, maximum_streamline_length=80.0, grid_factors=(1.0,),
-                  repacks=(64, 1 << 30), repeats=1, cache=cache)
+                  repacks=(64, 1 << 30), shapes=(0, 1, 4), repeats=1, cache=cache)
     tuned = tg.autotune.tune_fast(algorithm, domain, field, cfg["seeds"][:1000], **kwargs)
     print(f"\n[tune] {tuned}")
-    assert not tuned["cached"] and len(tuned["timings"]) == 4 and tuned["milliseconds"] > 0
+    # Three shapes, then two grids x two thresholds with the fastest.
+    assert not tuned["cached"] and len(tuned["timings"]) == 7 and tuned["milliseconds"] > 0
     again = tg.autotune.tune_fast(algorithm, domain, field, cfg["seeds"][:1000], **kwargs)
-    assert again["cached"] and (again["blocks"], again["repack"]) == (tuned["blocks"], tuned["repack"])
+    assert again["cached"] and (again["shape"], again["blocks"], again["repack"]) == (
+        tuned["shape"], tuned["blocks"], tuned["repack"])
+    assert runtime.fast_shape == tuned["shape"]
     got = tg.tracking.track(algorithm, domain, field, cfg["seeds"], rng, 2000, 0.25, 80.0, runtime)
     np.testing.assert_array_equal(got.lengths, default.lengths)
     np.testing.assert_array_equal(got.steps, default.steps)

@@ -28,6 +28,7 @@ ALGO_PROBABILISTIC = 2
 ALGO_DETERMINISTIC = 3
 PATH_PLAIN = 0
 PATH_FAST = 1
+FAST_SHAPES = 5
 PRECISIONS = {"fp32": 0, "bf16x2": 1, "fp16": 2, "bf16": 3}
 SEEDS_VALU = 0
 SEEDS_MFMA = 1
@@ -60,6 +61,8 @@ _SIGNATURES = {
     "tgk_version": (c_int, []),
     "tgk_runtime_create": (c_int, [c_int32, POINTER(c_void_p)]),
     "tgk_runtime_set_fast_grid": (c_int, [_P, c_int32]),
+    "tgk_runtime_set_fast_shape": (c_int, [_P, c_int32]),
+    "tgk_fast_shape_info": (c_int, [c_int32, POINTER(c_int32), POINTER(c_int32)]),
     "tgk_runtime_set_fast_repack": (c_int, [_P, c_int32]),
     "tgk_runtime_fast_reads": (c_int, [_P, POINTER(c_uint64), POINTER(c_uint64)]),
     "tgk_track_timed": (c_int, [_P, _P, _P, POINTER(tgk_algorithm), c_int32, _P, c_int64, c_uint32, c_float, c_int32,

@@ -104,6 +104,7 @@ struct tgk_runtime {
     int32_t seeds_key[3] = {-1, -1, -1};  // model, K, method
     bool capturing = false;    // the stream is being recorded into a graph
     int32_t fast_grid = 0;     // blocks of the fast kernel; <= 0: sized by occupancy
+    int32_t fast_shape = 0;    // launch shape of the order-8 SH Transport / Diffusion kernels
     uint32_t fast_min_repack = tgk::FAST_MIN_REPACK;  // see tgk_runtime_set_fast_repack
 };
 
@@ -451,6 +452,28 @@ int tgk_runtime_set_fast_grid(tgk_runtime *rt, int32_t blocks)
     if (!rt)
         return fail(TGK_ERR_INVALID, "runtime is NULL");
     rt->fast_grid = blocks > 0 ? blocks : 0;
+    return TGK_OK;
+}
+
+int tgk_runtime_set_fast_shape(tgk_runtime *rt, int32_t shape)
+{
+    if (!rt)
+        return fail(TGK_ERR_INVALID, "runtime is NULL");
+    if (shape < 0 || shape >= tgk::FAST_SHAPE_COUNT)
+        return fail(TGK_ERR_INVALID, "fast launch shape %d: shapes are 0..%d", shape, tgk::FAST_SHAPE_COUNT - 1);
+    rt->fast_shape = shape;
+    return TGK_OK;
+}
+
+int tgk_fast_shape_info(int32_t shape, int32_t *lanes_per_block, int32_t *lds_coefficients)
+{
+    static const int32_t table[tgk::FAST_SHAPE_COUNT][2] = {{256, 40}, {128, 40}, {512, 40}, {256, 32}, {256, 44}};
+    if (shape < 0 || shape >= tgk::FAST_SHAPE_COUNT)
+        return fail(TGK_ERR_INVALID, "fast launch shape %d: shapes are 0..%d", shape, tgk::FAST_SHAPE_COUNT - 1);
+    if (lanes_per_block)
+        *lanes_per_block = table[shape][0];
+    if (lds_coefficients)
+        *lds_coefficients = table[shape][1];
     return TGK_OK;
 }
 
@@ -812,6 +835,7 @@ int fast_launch(tgk_runtime *rt, tgk_field *field, const tgk_algorithm *algorith
     f->coeff_stride = field->fast_stride;
     f->sh_order = sh_order;
     f->min_repack = rt->fast_min_repack;
+    f->shape = rt->fast_shape;
     // The tracking kernel's cell ids are 32-bit.
     const uint64_t field_voxels = (uint64_t)p.field.dims[0] * p.field.dims[1] * p.field.dims[2];
     const uint64_t mask_voxels = (uint64_t)p.mask.dims[0] * p.mask.dims[1] * p.mask.dims[2];
@@ -819,9 +843,9 @@ int fast_launch(tgk_runtime *rt, tgk_field *field, const tgk_algorithm *algorith
         return fail(TGK_ERR_UNSUPPORTED, "the fast path takes grids of fewer than 2^32 - 2 voxels");
     f->shared_affine = p.domain_kind == tgk::DOMAIN_MASK && memcmp(p.mask.inv, p.field.inv, sizeof p.mask.inv) == 0;
     // Room for one generation's saved streamlines: one per lane of the grid.
-    int blocks = 0;
-    TGK_HIP(tgk::prepare_track_fast(*f, rt->fast_grid, &blocks), "fast kernel occupancy");
-    const size_t bytes = (size_t)blocks * 256 * tgk::FAST_CONTINUATION_VEC4 * sizeof(float4);
+    int64_t lanes = 0;
+    TGK_HIP(tgk::prepare_track_fast(*f, rt->fast_grid, &lanes), "fast kernel occupancy");
+    const size_t bytes = (size_t)lanes * tgk::FAST_CONTINUATION_VEC4 * sizeof(float4);
     for (int i = 0; i < 2; i++) {
         if (int r = reserve(rt, rt->continuations[i], bytes, "saved streamlines"))
             return r;

@@ -15,7 +15,11 @@
 // * The boundary connectome's tracking loop on the same machinery
 //   (connectome_fast_kernel); its hits pass is the plain path's.
 //
-// Compiled with -ffp-contract=fast.
+// Compiled with -ffp-contract=on: a product and a sum in one expression
+// become an fma as the source writes them. With =fast the compiler also
+// fused across statements, and chose differently in kernel variants that
+// differ only in where a row waits (LDS or registers), which changed a few
+// trajectories in the last bit; with =on every variant rounds alike.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -197,7 +201,7 @@ template <int DOMAIN, int G> __device__ __forceinline__ void resume_lane(const f
 // step), so no large register array is carried around the loop; DTI rows
 // and order-12 rows in registers.
 template <int MODEL, int LMAX> constexpr bool rows_in_lds() { return MODEL == MODEL_SH && LMAX == 8; }
-// Threads per block of the fast kernel.
+// Threads per block of the fast kernels.
 template <int LMAX> constexpr int fast_block() { return 256; }
 // float4 columns of a row kept in LDS. At order 8, 10 of the 12 (the first
 // 40 of 45 coefficients): 40 KiB per 256-lane block, so four blocks share a
@@ -206,16 +210,39 @@ template <int LMAX> constexpr int fast_block() { return 256; }
 // coefficients stay in registers.
 template <int MODEL, int LMAX> constexpr int lds_vectors() { return rows_in_lds<MODEL, LMAX>() ? 10 : 0; }
 
-// A cell's row into the lane's cache on a cell change: order-8 SH rows to
-// LDS (the last TERMS - 4 LDS_VEC coefficients to registers), others to
-// registers; with LDS false, every row to registers.
-template <int MODEL, int LMAX, bool LDS = rows_in_lds<MODEL, LMAX>()>
+// Launch shapes of the order-8 SH Transport and Diffusion tracking kernels,
+// chosen per runtime (tgk_runtime_set_fast_shape, the autotuner): lanes per
+// block and float4 columns of the row in LDS. Shape 0 is the default above;
+// the others trade LDS per block against blocks per CU, or coefficients in
+// LDS against registers. Which lane tracks which streamline, and where its
+// coefficients wait between steps, changes no result.
+struct FastShape {
+    int block;
+    int lds_vectors;
+};
+constexpr FastShape FAST_SHAPES[FAST_SHAPE_COUNT] = {{256, 10}, {128, 10}, {512, 10}, {256, 8}, {256, 11}};
+template <int MODEL, int ALGO, int LMAX> constexpr bool shaped()
+{
+    return MODEL == MODEL_SH && LMAX == 8 && (ALGO == ALGO_TRANSPORT || ALGO == ALGO_DIFFUSION);
+}
+template <int MODEL, int ALGO, int LMAX, int S> constexpr int shape_block()
+{
+    return shaped<MODEL, ALGO, LMAX>() ? FAST_SHAPES[S].block : fast_block<LMAX>();
+}
+template <int MODEL, int ALGO, int LMAX, int S> constexpr int shape_lds_vectors()
+{
+    return shaped<MODEL, ALGO, LMAX>() ? FAST_SHAPES[S].lds_vectors : lds_vectors<MODEL, LMAX>();
+}
+
+// A cell's row into the lane's cache on a cell change: the first LDS_VEC
+// float4 columns of an SH row to LDS (column j of a lane at j * BLOCK +
+// lane), the rest to registers; DTI rows to registers.
+template <int MODEL, int LMAX, int LDS_VEC = lds_vectors<MODEL, LMAX>(), int BLOCK = fast_block<LMAX>()>
 __device__ __forceinline__ void load_row(const float *__restrict__ coeffs, int64_t cell, uint32_t cstride, float *q,
                                          float4 *__restrict__ lds_rows)
 {
     constexpr int TERMS = Order<LMAX>::TERMS;
     constexpr int VEC = Order<LMAX>::STRIDE / 4;
-    constexpr int LDS_VEC = LDS ? lds_vectors<MODEL, LMAX>() : 0;
     const float4 *row = reinterpret_cast<const float4 *>(coeffs + cell * (int64_t)cstride);
     if constexpr (MODEL == MODEL_SH) {
         // Straight from memory into LDS (no registers held while the loads
@@ -226,8 +253,7 @@ __device__ __forceinline__ void load_row(const float *__restrict__ coeffs, int64
 #pragma unroll
         for (int j = 0; j < LDS_VEC; j++)
             __builtin_amdgcn_global_load_lds(static_cast<const void *>(row + j),
-                                             (__attribute__((address_space(3))) void *)(wave_rows +
-                                                                                        j * fast_block<LMAX>()),
+                                             (__attribute__((address_space(3))) void *)(wave_rows + j * BLOCK),
                                              16, 0, 0);
 #pragma unroll
         for (int j = LDS_VEC; j < VEC; j++) {
@@ -253,19 +279,18 @@ __device__ __forceinline__ void load_row(const float *__restrict__ coeffs, int64
 }
 
 // The LDS part of the cached row back into q, every step.
-template <int MODEL, int LMAX>
+template <int MODEL, int LMAX, int LDS_VEC = lds_vectors<MODEL, LMAX>(), int BLOCK = fast_block<LMAX>()>
 __device__ __forceinline__ void read_row(float *q, const float4 *__restrict__ lds_rows)
 {
     constexpr int TERMS = Order<LMAX>::TERMS;
-    constexpr int LDS_VEC = lds_vectors<MODEL, LMAX>();
-    if constexpr (rows_in_lds<MODEL, LMAX>()) {
+    if constexpr (LDS_VEC > 0) {
         // Loads into LDS (load_row) complete under vmcnt, which the compiler
         // does not wait on before these reads of the same LDS: wait here
         // (s_waitcnt vmcnt(0); expcnt and lgkmcnt left at their maxima).
         __builtin_amdgcn_s_waitcnt(0x0F70);
 #pragma unroll
         for (int j = 0; j < LDS_VEC; j++) {
-            const float4 v = lds_rows[j * fast_block<LMAX>() + threadIdx.x];
+            const float4 v = lds_rows[j * BLOCK + threadIdx.x];
             if (4 * j < TERMS)
                 q[4 * j] = v.x;
             if (4 * j + 1 < TERMS)
@@ -314,7 +339,7 @@ __device__ __forceinline__ V3 update_orientation(const AlgoParams &a, const floa
 // affine; the mask's, when it is the same matrix (shared_affine: every
 // image-domain workload that tracks in its field's grid), reuses it. The mask
 // rounds floor(v + 0.5) (domain.py:369), the field rint(v) (field.py:97).
-template <int MODEL, int DOMAIN, int LMAX, int G>
+template <int MODEL, int DOMAIN, int LMAX, int G, int LDS_VEC, int BLOCK>
 __device__ __forceinline__ void request_cells(const TrackLaunch &p, bool shared_affine,
                                               const float *__restrict__ coeffs, uint32_t cstride, Lane<G> &L, float *q,
                                               float4 *__restrict__ lds_rows)
@@ -340,7 +365,7 @@ __device__ __forceinline__ void request_cells(const TrackLaunch &p, bool shared_
     L.field_ok = cell != NO_CELL;
     if (cell != NO_CELL && cell != L.field_cell) {
         L.field_cell = cell;
-        load_row<MODEL, LMAX>(coeffs, (int64_t)cell, cstride, q, lds_rows);
+        load_row<MODEL, LMAX, LDS_VEC, BLOCK>(coeffs, (int64_t)cell, cstride, q, lds_rows);
         L.row_read = true;
     }
 }
@@ -348,7 +373,7 @@ __device__ __forceinline__ void request_cells(const TrackLaunch &p, bool shared_
 // nz: this step's Diffusion draws, made by the caller before anything that
 // waits on memory, so they cover the latency of the cell data
 // request_cells asked for.
-template <int MODEL, int ALGO, int DOMAIN, int LMAX, int G>
+template <int MODEL, int ALGO, int DOMAIN, int LMAX, int G, int LDS_VEC, int BLOCK>
 __device__ __forceinline__ void advance(const TrackLaunch &p, bool shared_affine, const float *__restrict__ coeffs,
                                         uint32_t cstride, Lane<G> &L, float *q, float4 *__restrict__ lds_rows,
                                         const fast::Noise &nz)
@@ -389,7 +414,7 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, bool shared_affine
         L.g = -1;
         return;
     }
-    read_row<MODEL, LMAX>(q, lds_rows);
+    read_row<MODEL, LMAX, LDS_VEC, BLOCK>(q, lds_rows);
     store_due_group(p, L);
     L.o = update_orientation<MODEL, ALGO, LMAX>(p.algo, q, L.o, L.rng, nz);
     L.steps++;
@@ -403,7 +428,7 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, bool shared_affine
         const float t = (float)L.time;
         hold_point(L, v3(L.point.x - t * L.o.x, L.point.y - t * L.o.y, L.point.z - t * L.o.z));
     }
-    request_cells<MODEL, DOMAIN, LMAX, G>(p, shared_affine, coeffs, cstride, L, q, lds_rows);
+    request_cells<MODEL, DOMAIN, LMAX, G, LDS_VEC, BLOCK>(p, shared_affine, coeffs, cstride, L, q, lds_rows);
 }
 
 // Waves per SIMD the compiler must fit: the order-8 SH Transport and
@@ -417,12 +442,18 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, bool shared_affine
 // so it keeps its 159-162 VGPRs (three waves). The DTI kernels (90-115
 // VGPRs, four or five waves) ran 4% slower held to seven waves and 19%
 // slower held to eight (spills).
-template <int MODEL, int ALGO, int DOMAIN, int LMAX> constexpr int fast_min_waves()
+// A launch shape that puts more of the row in LDS allows fewer blocks per
+// CU (160 KiB of LDS each): shape 4 (44 coefficients) three waves per SIMD.
+template <int MODEL, int ALGO, int DOMAIN, int LMAX, int S = 0> constexpr int fast_min_waves()
 {
     if constexpr (MODEL == MODEL_SH && LMAX == 12 && (ALGO == ALGO_TRANSPORT || ALGO == ALGO_DIFFUSION))
         return 3;
-    if constexpr (MODEL == MODEL_SH && LMAX == 8 && (ALGO == ALGO_TRANSPORT || ALGO == ALGO_DIFFUSION))
-        return 4;
+    if constexpr (MODEL == MODEL_SH && LMAX == 8 && (ALGO == ALGO_TRANSPORT || ALGO == ALGO_DIFFUSION)) {
+        constexpr int block = FAST_SHAPES[S].block;
+        constexpr int per_cu = 163840 / (FAST_SHAPES[S].lds_vectors * block * 16);
+        constexpr int waves = per_cu * (block / 64) / 4;
+        return waves < 4 ? waves : 4;
+    }
     if constexpr (MODEL == MODEL_SH && LMAX == 8 && ALGO == ALGO_PROBABILISTIC)
         return 3;
     return 1;
@@ -439,12 +470,14 @@ template <int MODEL, int ALGO, int DOMAIN, int LMAX> constexpr int fast_min_wave
 // a quarter or half as many per SIMD, each running correspondingly faster.
 // The last generation, or one with little work, runs to completion. Which
 // lane tracks which streamline, and when, changes no result.
-template <int MODEL, int ALGO, int DOMAIN, int LMAX>
-__global__ __launch_bounds__(fast_block<LMAX>())
-__attribute__((amdgpu_waves_per_eu(fast_min_waves<MODEL, ALGO, DOMAIN, LMAX>())))
+template <int MODEL, int ALGO, int DOMAIN, int LMAX, int S>
+__global__ __launch_bounds__((shape_block<MODEL, ALGO, LMAX, S>()))
+__attribute__((amdgpu_waves_per_eu(fast_min_waves<MODEL, ALGO, DOMAIN, LMAX, S>())))
 void track_fast_kernel(FastLaunch f)
 {
-    __shared__ float4 lds_rows[rows_in_lds<MODEL, LMAX>() ? lds_vectors<MODEL, LMAX>() * fast_block<LMAX>() : 1];
+    constexpr int BLOCK = shape_block<MODEL, ALGO, LMAX, S>();
+    constexpr int LDS_VEC = shape_lds_vectors<MODEL, ALGO, LMAX, S>();
+    __shared__ float4 lds_rows[LDS_VEC > 0 ? LDS_VEC * BLOCK : 1];
     const TrackLaunch &p = f.track;
     const int lane_id = __lane_id();
     const float *coeffs = static_cast<const float *>(f.coeffs);
@@ -520,7 +553,8 @@ void track_fast_kernel(FastLaunch f)
                     L.field_cell = UNSET_CELL;
                     L.mask_cell = UNSET_CELL;
                     L.mask_value = 0;
-                    request_cells<MODEL, DOMAIN, LMAX, G>(p, f.shared_affine != 0, coeffs, cstride, L, q, lds_rows);
+                    request_cells<MODEL, DOMAIN, LMAX, G, LDS_VEC, BLOCK>(p, f.shared_affine != 0, coeffs, cstride,
+                                                                          L, q, lds_rows);
                 }
             }
         }
@@ -569,7 +603,8 @@ void track_fast_kernel(FastLaunch f)
             break;
         }
         if (L.g >= 0)
-            advance<MODEL, ALGO, DOMAIN, LMAX, G>(p, f.shared_affine != 0, coeffs, cstride, L, q, lds_rows, nz);
+            advance<MODEL, ALGO, DOMAIN, LMAX, G, LDS_VEC, BLOCK>(p, f.shared_affine != 0, coeffs, cstride, L, q,
+                                                                 lds_rows, nz);
         rows_read += (uint64_t)__popcll(__ballot(L.row_read));
         masks_read += (uint64_t)__popcll(__ballot(L.mask_read));
         L.row_read = false;
@@ -810,7 +845,7 @@ __global__ __launch_bounds__(256) void tod_fast_kernel(FastTodLaunch f)
             fast::sh45_basis(o, in_grid ? 1.0f : 0.0f, [&](int i, float y) { acc[i] = fmaf(acc[i], keep, y); });
             if (voxel != field_cell) {
                 field_cell = voxel;
-                load_row<MODEL, LMAX, false>(coeffs, voxel, cstride, q, nullptr);
+                load_row<MODEL, LMAX, 0>(coeffs, voxel, cstride, q, nullptr);
             }
             o = update_orientation<MODEL, ALGO, LMAX>(t.algo, q, o, rng);
             stop = !(__builtin_isfinite(o.x) && __builtin_isfinite(o.y) && __builtin_isfinite(o.z)) ||
@@ -994,7 +1029,7 @@ template <int L> hipError_t device_matrix(const double **out)
 
 // Blocks that fill the current device once, computed on first use per
 // device (atomically published: runtimes on other threads may read it).
-template <int MODEL, int ALGO, int DOMAIN, int LMAX> hipError_t occupancy_grid(int *blocks)
+template <int MODEL, int ALGO, int DOMAIN, int LMAX, int S> hipError_t occupancy_grid(int *blocks)
 {
     static std::atomic<int> cache[64] = {};
     int device = 0;
@@ -1006,7 +1041,8 @@ template <int MODEL, int ALGO, int DOMAIN, int LMAX> hipError_t occupancy_grid(i
     if (cache[device].load(std::memory_order_acquire) == 0) {
         int per_cu = 0;
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void *>(track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX>), fast_block<LMAX>(), 0);
+            &per_cu, reinterpret_cast<const void *>(track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX, S>),
+            shape_block<MODEL, ALGO, LMAX, S>(), 0);
         if (e != hipSuccess)
             return e;
         hipDeviceProp_t prop;
@@ -1019,17 +1055,18 @@ template <int MODEL, int ALGO, int DOMAIN, int LMAX> hipError_t occupancy_grid(i
     return hipSuccess;
 }
 
-// query: only report the grid (*blocks) a launch would use.
-template <int MODEL, int ALGO, int DOMAIN, int LMAX>
-hipError_t launch_fast(const FastLaunch &f, int grid_blocks, bool query, int *blocks, hipStream_t stream)
+// query: only report the lanes of the grid a launch would use.
+template <int MODEL, int ALGO, int DOMAIN, int LMAX, int S>
+hipError_t launch_fast(const FastLaunch &f, int grid_blocks, bool query, int64_t *lanes, hipStream_t stream)
 {
+    constexpr int BLOCK = shape_block<MODEL, ALGO, LMAX, S>();
     if (grid_blocks <= 0) {
-        const hipError_t e = occupancy_grid<MODEL, ALGO, DOMAIN, LMAX>(&grid_blocks);
+        const hipError_t e = occupancy_grid<MODEL, ALGO, DOMAIN, LMAX, S>(&grid_blocks);
         if (e != hipSuccess)
             return e;
     }
-    if (blocks)
-        *blocks = grid_blocks;
+    if (lanes)
+        *lanes = (int64_t)grid_blocks * BLOCK;
     if (query)
         return hipSuccess;
     // Generation g resumes what g - 1 saved in cont[(g - 1) % 2] and saves
@@ -1044,23 +1081,32 @@ hipError_t launch_fast(const FastLaunch &f, int grid_blocks, bool query, int *bl
         gen.last = g == FAST_GENERATIONS - 1;
         zero_generation<<<1, 64, 0, stream>>>(f.claims, f.ended, gen.cont_out_count, g == 0 ? f.stats : nullptr);
         const int blocks_g = std::max(1, grid_blocks >> g);
-        track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX><<<blocks_g, fast_block<LMAX>(), 0, stream>>>(gen);
+        track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX, S><<<blocks_g, BLOCK, 0, stream>>>(gen);
     }
     return hipGetLastError();
 }
 
 template <int MODEL, int ALGO, int DOMAIN>
-hipError_t launch_fast_order(const FastLaunch &f, int grid, bool query, int *blocks, hipStream_t stream)
+hipError_t launch_fast_order(const FastLaunch &f, int grid, bool query, int64_t *lanes, hipStream_t stream)
 {
     if constexpr (MODEL == MODEL_SH) {
         if (f.sh_order == 12)
-            return launch_fast<MODEL, ALGO, DOMAIN, 12>(f, grid, query, blocks, stream);
+            return launch_fast<MODEL, ALGO, DOMAIN, 12, 0>(f, grid, query, lanes, stream);
     }
-    return launch_fast<MODEL, ALGO, DOMAIN, 8>(f, grid, query, blocks, stream);
+    if constexpr (shaped<MODEL, ALGO, 8>()) {
+        switch (f.shape) {
+        case 1: return launch_fast<MODEL, ALGO, DOMAIN, 8, 1>(f, grid, query, lanes, stream);
+        case 2: return launch_fast<MODEL, ALGO, DOMAIN, 8, 2>(f, grid, query, lanes, stream);
+        case 3: return launch_fast<MODEL, ALGO, DOMAIN, 8, 3>(f, grid, query, lanes, stream);
+        case 4: return launch_fast<MODEL, ALGO, DOMAIN, 8, 4>(f, grid, query, lanes, stream);
+        default: break;
+        }
+    }
+    return launch_fast<MODEL, ALGO, DOMAIN, 8, 0>(f, grid, query, lanes, stream);
 }
 
 template <int MODEL, int ALGO>
-hipError_t launch_fast_domain(const FastLaunch &f, int grid, bool query, int *blocks, hipStream_t stream)
+hipError_t launch_fast_domain(const FastLaunch &f, int grid, bool query, int64_t *blocks, hipStream_t stream)
 {
     if (f.track.domain_kind == DOMAIN_MESH)
         return launch_fast_order<MODEL, ALGO, DOMAIN_MESH>(f, grid, query, blocks, stream);
@@ -1068,7 +1114,7 @@ hipError_t launch_fast_domain(const FastLaunch &f, int grid, bool query, int *bl
 }
 
 template <int MODEL>
-hipError_t launch_fast_algo(const FastLaunch &f, int grid, bool query, int *blocks, hipStream_t stream)
+hipError_t launch_fast_algo(const FastLaunch &f, int grid, bool query, int64_t *blocks, hipStream_t stream)
 {
     switch (f.track.algo.algorithm) {
     case ALGO_DIFFUSION: return launch_fast_domain<MODEL, ALGO_DIFFUSION>(f, grid, query, blocks, stream);
@@ -1078,7 +1124,7 @@ hipError_t launch_fast_algo(const FastLaunch &f, int grid, bool query, int *bloc
     }
 }
 
-hipError_t dispatch_fast(const FastLaunch &f, int grid, bool query, int *blocks, hipStream_t stream)
+hipError_t dispatch_fast(const FastLaunch &f, int grid, bool query, int64_t *blocks, hipStream_t stream)
 {
     if (f.track.algo.model == MODEL_DTI)
         return launch_fast_algo<MODEL_DTI>(f, grid, query, blocks, stream);
@@ -1203,9 +1249,9 @@ hipError_t launch_track_fast(const FastLaunch &f, int grid_blocks, hipStream_t s
     return dispatch_fast(f, grid_blocks, false, nullptr, stream);
 }
 
-hipError_t prepare_track_fast(const FastLaunch &f, int grid_blocks, int *blocks)
+hipError_t prepare_track_fast(const FastLaunch &f, int grid_blocks, int64_t *lanes)
 {
-    return dispatch_fast(f, grid_blocks, true, blocks, nullptr);
+    return dispatch_fast(f, grid_blocks, true, lanes, nullptr);
 }
 
 hipError_t launch_connectome_fast(const FastConnectomeLaunch &f, hipStream_t stream)

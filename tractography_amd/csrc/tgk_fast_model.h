@@ -123,6 +123,12 @@ template <int L, int M>
 __device__ __forceinline__ void sh_order_terms(const float *q, float z, float w, float s, float u1r, float u1i,
                                                Powers &pw, Raw &r)
 {
+    // Fused as the compiler finds best, as in the rest of this file before
+    // contraction became per expression: the Deterministic Hessian divides
+    // the rounding of these gradients by 2h (2.7e-3 voxel from FP64 with
+    // per-expression fusion, 1.1e-3 with this). Only the unshaped
+    // Deterministic kernels use the theta/phi form.
+    #pragma clang fp contract(fast)
     if constexpr (M <= L) {
         using O = Order<L>;
         if constexpr (M > 1) {
@@ -146,6 +152,7 @@ __device__ __forceinline__ void sh_order_terms(const float *q, float z, float w,
 template <int L>
 __device__ __forceinline__ Raw sh_evaluate(const float *q, const Direction &dir)
 {
+    #pragma clang fp contract(fast)
     using O = Order<L>;
     const float z = dir.z, s = dir.s, w = z * z;
     Raw r;
@@ -181,6 +188,7 @@ struct ModelOut {
 template <int L>
 __device__ __forceinline__ ModelOut sh_model(const float *q, const Direction &dir)
 {
+    #pragma clang fp contract(fast)
     const Raw raw = sh_evaluate<L>(q, dir);
     ModelOut m;
     float d;
@@ -515,6 +523,7 @@ __device__ __forceinline__ V3 probabilistic(const AlgoParams &a, const float *q,
 template <int MODEL, int LMAX>
 __device__ __forceinline__ V3 sphere_gradient(const float *q, V3 p)
 {
+    #pragma clang fp contract(fast)
     const Direction d = direction_of(p);
     const ModelOut m = model_at<MODEL, LMAX>(q, p, d);
     return v3(m.dtheta * (d.z * d.c1) - m.dphi * d.s1, m.dtheta * (d.z * d.s1) + m.dphi * d.c1, -m.dtheta * d.s);
@@ -525,6 +534,7 @@ __device__ __forceinline__ V3 sphere_gradient(const float *q, V3 p)
 template <int MODEL, int LMAX>
 __device__ __forceinline__ V3 deterministic(const AlgoParams &a, const float *q, V3 o_in)
 {
+    #pragma clang fp contract(fast)
     const float h = 1e-2f, inv_2h = 50.0f;
     const float n2 = o_in.x * o_in.x + o_in.y * o_in.y + o_in.z * o_in.z;
     const float inv = __builtin_amdgcn_rsqf(n2);

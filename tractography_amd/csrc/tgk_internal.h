@@ -99,6 +99,11 @@ constexpr int FAST_CONTINUATION_VEC4 = 4;
 constexpr int FAST_END_SHARDS = 32;
 constexpr int FAST_END_STRIDE = 32;
 constexpr size_t FAST_COUNTER_BYTES = 128 + (size_t)FAST_END_SHARDS * FAST_END_STRIDE * 4;
+// Launch shapes of the order-8 SH Transport and Diffusion tracking kernels
+// (lanes per block, coefficients in LDS), indexed 0..FAST_SHAPE_COUNT - 1:
+// (256, 40), (128, 40), (512, 40), (256, 32), (256, 44). Other kernels run
+// shape 0.
+constexpr int FAST_SHAPE_COUNT = 5;
 struct FastLaunch {
     TrackLaunch track;
     uint32_t *claims;           // a generation's claim counter (zeroed per generation)
@@ -118,6 +123,7 @@ struct FastLaunch {
     const void *coeffs;         // transformed field (layout per precision)
     uint32_t coeff_stride;      // elements per voxel in `coeffs`
     int32_t sh_order;           // 8 (K <= 45) or 12 (K <= 91): polynomial layout of `coeffs`
+    int32_t shape;              // launch shape, 0..FAST_SHAPE_COUNT - 1
 };
 // grid_blocks <= 0 sizes the persistent grid to the device's occupancy
 // (computed once per kernel and device, then cached). n_seeds must be below
@@ -125,9 +131,9 @@ struct FastLaunch {
 hipError_t launch_track_fast(const FastLaunch &p, int grid_blocks, hipStream_t stream);
 // Computes and caches that occupancy for the kernel `p` selects, launching
 // nothing (so later launches make no device queries, e.g. while a graph is
-// being recorded); *blocks receives the grid a launch with grid_blocks
-// would use (its lanes bound the saved streamlines of a generation).
-hipError_t prepare_track_fast(const FastLaunch &p, int grid_blocks, int *blocks);
+// being recorded); *lanes receives the lanes of the grid a launch with
+// grid_blocks would use (they bound the saved streamlines of a generation).
+hipError_t prepare_track_fast(const FastLaunch &p, int grid_blocks, int64_t *lanes);
 // Dense seed-major rows (both paths) -> ragged points.
 hipError_t launch_compact_seed_major(const float *dense, int64_t n_seeds, uint32_t n_points,
                                      const uint32_t *lengths, const int64_t *offsets, float *out,

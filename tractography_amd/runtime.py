@@ -46,6 +46,7 @@ class HipRuntime:
         self.device = int(device)
         self.path = path
         self.precision = precision
+        self.fast_shape = 0
         # Handles made on this runtime: closing the runtime destroys them
         # first, since each one frees its device memory through it.
         self._children: weakref.WeakSet = weakref.WeakSet()
@@ -79,6 +80,12 @@ class HipRuntime:
         _native.check(_native.lib().tgk_runtime_set_fast_grid(self._handle, int(blocks)))
         self.fast_grid = int(blocks) if blocks > 0 else 0
 
+    def set_fast_shape(self, shape: int) -> None:
+        """Launch shape of the fast path's order-8 SH Transport and Diffusion
+        kernels (see fast_shapes()); other kernels run shape 0."""
+        _native.check(_native.lib().tgk_runtime_set_fast_shape(self._handle, int(shape)))
+        self.fast_shape = int(shape)
+
     def set_fast_repack(self, min_streamlines: int) -> None:
         """Streamlines (half a generation's work) below which the fast path
         no longer cuts a generation short to repack its streamlines."""
@@ -103,6 +110,19 @@ class HipRuntime:
         for child in list(self._children):
             child.release()
         self._finalizer()
+
+
+def fast_shapes() -> list[dict]:
+    """The fast path's launch shapes: lanes per block (streamlines per
+    workgroup) and how many of a row's 45 coefficients stay in LDS between
+    steps (the rest in registers). No device is needed."""
+    shapes = []
+    for shape in range(_native.FAST_SHAPES):
+        lanes = ctypes.c_int32()
+        lds = ctypes.c_int32()
+        _native.check(_native.lib().tgk_fast_shape_info(shape, ctypes.byref(lanes), ctypes.byref(lds)))
+        shapes.append({"shape": shape, "lanes_per_block": lanes.value, "lds_coefficients": lds.value})
+    return shapes
 
 
 class DeviceHandle:
