@@ -9,8 +9,10 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <string>
 #include <deque>
 #include <vector>
@@ -165,6 +167,39 @@ struct PinnedBuffer {
         return e;
     }
 };
+
+// Copies spans of host memory with up to 8 threads, in slices of at least
+// 64 MiB (smaller copies stay on the calling thread).
+template <class Span> static void parallel_copy(const Span *spans, size_t n)
+{
+    constexpr size_t SLICE = (size_t)64 << 20;
+    struct Slice {
+        char *dst;
+        const char *src;
+        size_t bytes;
+    };
+    std::vector<Slice> slices;
+    for (size_t i = 0; i < n; i++)
+        for (size_t off = 0; off < spans[i].bytes; off += SLICE)
+            slices.push_back({spans[i].dst + off, spans[i].src + off, std::min(SLICE, spans[i].bytes - off)});
+    const size_t workers = std::min<size_t>(8, slices.size());
+    if (workers <= 1) {
+        for (const Slice &s : slices)
+            memcpy(s.dst, s.src, s.bytes);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t i = next.fetch_add(1); i < slices.size(); i = next.fetch_add(1))
+            memcpy(slices[i].dst, slices[i].src, slices[i].bytes);
+    };
+    std::vector<std::thread> threads;
+    for (size_t t = 1; t < workers; t++)
+        threads.emplace_back(work);
+    work();
+    for (std::thread &t : threads)
+        t.join();
+}
 
 struct tgk_tracks {
     int device = 0;
@@ -972,12 +1007,14 @@ int tgk_track(tgk_runtime *rt, const tgk_domain *domain, const tgk_field *field,
     }
     // Chunk size: the dense per-seed rows of a chunk and its compacted copy
     // take chunk * per_seed bytes each; a chunk may use a quarter of the free
-    // device memory, between 2 and 16 GiB (1.2M seeds of 1200 points).
+    // device memory, between 2 and 48 GiB each (3.5M seeds of 1200 points),
+    // so that most calls are one chunk, whose page-locked points become the
+    // result without a host copy.
     const size_t per_seed = (size_t)n_points * 3 * sizeof(float);
     size_t budget = (size_t)2 << 30;
     size_t free_bytes = 0, total_bytes = 0;
     if (hipMemGetInfo(&free_bytes, &total_bytes) == hipSuccess)
-        budget = std::min<size_t>((size_t)16 << 30, std::max<size_t>(budget, free_bytes / 8));
+        budget = std::min<size_t>((size_t)48 << 30, std::max<size_t>(budget, free_bytes / 8));
     if (const char *env = getenv("TGK_CHUNK_BYTES"))
         budget = std::max<size_t>((size_t)strtoull(env, nullptr, 10), per_seed);
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(n_seeds, (int64_t)(budget / per_seed)));
@@ -1108,17 +1145,28 @@ int tgk_tracks_copy(const tgk_tracks *tracks, uint32_t *lengths, uint32_t *steps
         memcpy(hits, tracks->hits.data(), n * 4);
     if (points && tracks->total_points > 0) {
         TGK_HIP(hipSetDevice(tracks->device), "hipSetDevice");
+        // Host-resident pieces are copied by several threads: a single
+        // thread moves ~3 GB/s into freshly mapped pages (each first touch
+        // a page fault), several fault and copy in parallel.
+        struct Span {
+            char *dst;
+            const char *src;
+            size_t bytes;
+        };
+        std::vector<Span> spans;
         float *dst = points;
         for (const tgk_tracks::Piece &piece : tracks->pieces) {
             const size_t bytes = (size_t)piece.n_points * 3 * sizeof(float);
-            if (bytes && piece.pinned)
-                memcpy(dst, piece.pinned->ptr, bytes);
-            else if (bytes && piece.device)
+            if (bytes && piece.device)
                 TGK_HIP(hipMemcpy(dst, piece.device->ptr, bytes, hipMemcpyDeviceToHost), "points copy");
             else if (bytes)
-                memcpy(dst, piece.host.data(), bytes);
+                spans.push_back({reinterpret_cast<char *>(dst),
+                                 static_cast<const char *>(piece.pinned ? piece.pinned->ptr
+                                                                        : static_cast<const void *>(piece.host.data())),
+                                 bytes});
             dst += (size_t)piece.n_points * 3;
         }
+        parallel_copy(spans.data(), spans.size());
     }
     return TGK_OK;
 }
