@@ -214,6 +214,13 @@ __device__ __forceinline__ uint32_t randi(uint2 &s, uint32_t max) { return (uint
 // host draw; a global seed index maps to it through the batch size.
 __device__ __forceinline__ uint2 seed_stream(const TrackLaunch &p, int64_t global_index)
 {
+    // A 32-bit divide when both operands fit (a 64-bit one expands to about
+    // a hundred instructions, paid by the whole wavefront at each refill).
+    if ((((uint64_t)global_index | (uint64_t)p.batch_size) >> 32) == 0) {
+        const uint32_t index = (uint32_t)global_index, size = (uint32_t)p.batch_size;
+        const uint32_t batch = index / size;
+        return rng_init(p.rng_seeds[batch], index - batch * size);
+    }
     const int64_t batch = global_index / p.batch_size;
     const uint32_t stream = (uint32_t)(global_index - batch * p.batch_size);
     return rng_init(p.rng_seeds[batch], stream);

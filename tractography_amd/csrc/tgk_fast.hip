@@ -249,11 +249,17 @@ __device__ __forceinline__ void load_row(const float *__restrict__ coeffs, int64
         // are in flight): each lane's 16 bytes land at the instruction's
         // wave-uniform LDS base + 16 * lane, i.e. column j of the lane's
         // slot. Lanes that do not reload are masked off and keep theirs.
-        float4 *wave_rows = lds_rows + (threadIdx.x & ~63u);
+        // The base is made wave-uniform once (a scalar register), so that
+        // each column's base is a scalar add, not a vector add and a
+        // readfirstlane per load.
+        typedef __attribute__((address_space(3))) char lds_char;
+        const uint32_t wave_base = (uint32_t)__builtin_amdgcn_readfirstlane(
+            (int)(uint32_t)(uintptr_t)(lds_char *)(lds_rows + (threadIdx.x & ~63u)));
 #pragma unroll
         for (int j = 0; j < LDS_VEC; j++)
             __builtin_amdgcn_global_load_lds(static_cast<const void *>(row + j),
-                                             (__attribute__((address_space(3))) void *)(wave_rows + j * BLOCK),
+                                             (__attribute__((address_space(3))) void *)(uintptr_t)(
+                                                 wave_base + (uint32_t)(j * BLOCK * 16)),
                                              16, 0, 0);
 #pragma unroll
         for (int j = LDS_VEC; j < VEC; j++) {
