@@ -246,13 +246,15 @@ def test_tod_rejects_invalid_grids_and_domains(runtime):
     lib = _native.lib()
     out = np.zeros(45, np.float32)
 
-    def call(dims, inverse, dom=handle):
+    def call(dims, inverse, dom=handle, path=_native.PATH_PLAIN):
         dims = np.ascontiguousarray(dims, dtype=np.uint32)
         inverse = np.ascontiguousarray(inverse, dtype=np.float32).reshape(16)
         return lib.tgk_tod(runtime.handle, dom.handle, dom.handle, field.upload(runtime).handle,
-                           algorithm.descriptor(), _native.PATH_PLAIN, 1, 10, 10, _native.ptr(dims),
+                           algorithm.descriptor(), path, 1, 10, 10, _native.ptr(dims),
                            _native.ptr(inverse), 0, _native.ptr(out))
 
+    assert call([1, 1, 1], np.eye(4), path=7) == _native.ERR_INVALID
+    assert b"unknown path" in lib.tgk_last_error()
     assert call([2**31, 2**31, 2**31], np.eye(4)) == _native.ERR_INVALID
     assert b"too large" in lib.tgk_last_error()
     bad = np.eye(4)
@@ -362,6 +364,17 @@ def test_fast_tod_at_oblique_orientations(runtime):
         d = float(np.abs(fast[..., idx].astype(np.float64) - ref[..., idx]).sum()) / part
         dp = float(np.abs(plain[..., idx].astype(np.float64) - ref[..., idx]).sum()) / part
         assert d <= max(2e-3, 2.0 * dp), f"order {m}: {d:.2e} (plain {dp:.2e})"
+
+
+def test_fast_tod_zero_seeds_and_steps(runtime):
+    # Nothing to track: the output is cleared (and normalised to zeros).
+    domain, field, algorithm, shape = _scene("sh45-diffusion")
+    dims = np.array(shape)
+    inverse = np.eye(4, dtype=np.float32)
+    for n, steps in ((0, 50), (100, 0)):
+        got = _device_tod(runtime, domain, field, algorithm, 3, n, steps, dims, inverse, normalize=1,
+                          path=_native.PATH_FAST)
+        assert np.all(got == 0.0)
 
 
 def test_fast_tod_through_the_public_api():
