@@ -26,6 +26,7 @@
 #include <atomic>
 #include <cmath>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "tgk_fast_model.h"
@@ -234,6 +235,19 @@ template <int MODEL, int ALGO, int LMAX, int S> constexpr int shape_lds_vectors(
     return shaped<MODEL, ALGO, LMAX>() ? FAST_SHAPES[S].lds_vectors : lds_vectors<MODEL, LMAX>();
 }
 
+// Column j of a row from memory into LDS for every lane of the wavefront
+// that executes it. The instruction's immediate offset is added to both the
+// memory address and the LDS base, so all columns share one vector address
+// (the row) and the LDS base is set 16 * j below column j's slot.
+template <int BLOCK, int... J>
+__device__ __forceinline__ void dma_columns(const void *row, uint32_t wave_base, std::integer_sequence<int, J...>)
+{
+    (__builtin_amdgcn_global_load_lds(
+         row, (__attribute__((address_space(3))) void *)(uintptr_t)(wave_base + (uint32_t)(J * BLOCK * 16 - J * 16)),
+         16, J * 16, 0),
+     ...);
+}
+
 // A cell's row into the lane's cache on a cell change: the first LDS_VEC
 // float4 columns of an SH row to LDS (column j of a lane at j * BLOCK +
 // lane), the rest to registers; DTI rows to registers.
@@ -249,18 +263,13 @@ __device__ __forceinline__ void load_row(const float *__restrict__ coeffs, int64
         // are in flight): each lane's 16 bytes land at the instruction's
         // wave-uniform LDS base + 16 * lane, i.e. column j of the lane's
         // slot. Lanes that do not reload are masked off and keep theirs.
-        // The base is made wave-uniform once (a scalar register), so that
-        // each column's base is a scalar add, not a vector add and a
-        // readfirstlane per load.
+        // The wavefront's slot base is a scalar: its first lane's byte
+        // offset in the block's rows plus the rows' LDS address.
         typedef __attribute__((address_space(3))) char lds_char;
-        const uint32_t wave_base = (uint32_t)__builtin_amdgcn_readfirstlane(
-            (int)(uint32_t)(uintptr_t)(lds_char *)(lds_rows + (threadIdx.x & ~63u)));
-#pragma unroll
-        for (int j = 0; j < LDS_VEC; j++)
-            __builtin_amdgcn_global_load_lds(static_cast<const void *>(row + j),
-                                             (__attribute__((address_space(3))) void *)(uintptr_t)(
-                                                 wave_base + (uint32_t)(j * BLOCK * 16)),
-                                             16, 0, 0);
+        const uint32_t wave_base =
+            (uint32_t)__builtin_amdgcn_readfirstlane((int)((threadIdx.x & ~63u) * 16u)) +
+            (uint32_t)(uintptr_t)(lds_char *)lds_rows;
+        dma_columns<BLOCK>(static_cast<const void *>(row), wave_base, std::make_integer_sequence<int, LDS_VEC>{});
 #pragma unroll
         for (int j = LDS_VEC; j < VEC; j++) {
             const float4 v = row[j];
