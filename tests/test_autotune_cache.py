@@ -14,14 +14,18 @@ def test_cache_keys_separate_what_changes_the_kernel():
     sh45 = tg.local.RealSymmetricSphericalHarmonics(45)
     sh91 = tg.local.RealSymmetricSphericalHarmonics(91)
     mask = tg.domain.Image(tg.Nifti1Image(np.ones((2, 2, 2), dtype=np.uint8), np.eye(4)))
+    vertices = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float32)
+    faces = np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]], dtype=np.int64)
+    mesh = tg.domain.Mesh(vertices, faces)
     keys = {
+        autotune.cache_key("gpu-a", tg.algorithm.Diffusion(sh45, step_size=0.1), mesh),
         autotune.cache_key("gpu-a", tg.algorithm.Diffusion(sh45, step_size=0.1), mask),
         autotune.cache_key("gpu-b", tg.algorithm.Diffusion(sh45, step_size=0.1), mask),
         autotune.cache_key("gpu-a", tg.algorithm.Diffusion(sh91, step_size=0.1), mask),
         autotune.cache_key("gpu-a", tg.algorithm.Transport(sh45, step_size=0.1), mask),
         autotune.cache_key("gpu-a", tg.algorithm.Transport(tg.local.DiffusionTensorImaging(), step_size=0.1), mask),
     }
-    assert len(keys) == 5
+    assert len(keys) == 6
     # Step size and noise variance are not part of the key.
     assert autotune.cache_key("gpu-a", tg.algorithm.Diffusion(sh45, step_size=0.2, noise_variance=0.3), mask) in keys
 

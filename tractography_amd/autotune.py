@@ -34,6 +34,7 @@ from pathlib import Path
 import numpy as np
 
 from . import _native
+from .domain import Mesh
 from .runtime import HipRuntime, fast_shapes
 from .tracking import point_capacity
 
@@ -53,7 +54,7 @@ def cache_path() -> Path:
 
 def cache_key(device_name: str, algorithm, domain) -> str:
     descriptor = algorithm.descriptor()
-    kind = "mesh" if getattr(domain, "is_mesh", False) or type(domain).__name__ == "Mesh" else "mask"
+    kind = "mesh" if isinstance(domain, Mesh) else "mask"
     return (f"{device_name}|model={descriptor.model}|K={descriptor.n_parameters}|"
             f"algorithm={descriptor.algorithm}|domain={kind}")
 
