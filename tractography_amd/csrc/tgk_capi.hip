@@ -1200,8 +1200,12 @@ int run_track(tgk_runtime *rt, const tgk_domain *domain, tgk_field *field, const
 {
     if (n_seeds == 0)
         return TGK_OK;
-    const tgk::TrackLaunch p = make_launch(domain, field, algorithm, d_seeds, first_seed, n_seeds, d_rng_seeds,
-                                           batch_size, n_points, save_at, d_lengths, d_steps, d_hits, d_dense);
+    tgk::TrackLaunch p = make_launch(domain, field, algorithm, d_seeds, first_seed, n_seeds, d_rng_seeds, batch_size,
+                                     n_points, save_at, d_lengths, d_steps, d_hits, d_dense);
+    // The plain kernel's seed counter: the queue buffer's first word, which
+    // the connectome and TOD kernels also use on this stream, one launch at a
+    // time.
+    p.queue = rt->queue.as<uint32_t>();
     if (path == TGK_PATH_PLAIN) {
         TGK_HIP(tgk::launch_track_plain(p, rt->stream), "plain tracking kernel");
         return TGK_OK;

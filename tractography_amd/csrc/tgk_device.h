@@ -94,6 +94,7 @@ struct TrackLaunch {
     uint32_t *steps;
     uint32_t *hits;
     float *points;               // dense output, layout chosen by the kernel
+    uint32_t *queue;             // seed counter of the plain kernel's persistent lanes (zeroed by its launch)
 };
 
 struct V3 {

@@ -46,99 +46,6 @@ template <int MODEL, int ALGO> constexpr int plain_min_waves()
         return ALGO == ALGO_DETERMINISTIC ? 2 : 4;
 }
 
-template <int MODEL, int ALGO, int DOMAIN, int KT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_min_waves<MODEL, ALGO>())))
-void track_plain_kernel(TrackLaunch p)
-{
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= p.n_seeds)
-        return;
-    const float4 s0 = reinterpret_cast<const float4 *>(p.seeds)[2 * g];
-    const float4 s1 = reinterpret_cast<const float4 *>(p.seeds)[2 * g + 1];
-    V3 point = v3(s0.x, s0.y, s0.z);
-    V3 o = v3(s1.x, s1.y, s1.z);
-    uint2 rng = make_uint2(0u, 0u);
-    if constexpr (ALGO == ALGO_DIFFUSION || ALGO == ALGO_PROBABILISTIC)
-        rng = seed_stream(p, p.first_seed + g);
-
-    store_point(p.points, p.n_points, 0, g, point);
-    const float step = p.algo.step_size;
-    const float save = p.save_at;
-    const uint32_t stride = p.field.stride;
-    uint32_t n = 1, steps = 0, hit = NO_HIT;
-    float time = 0.0f;
-    V3 previous = point;
-    bool first = true;
-    while (n < p.n_points && steps < p.max_steps) {
-        if constexpr (DOMAIN == DOMAIN_MASK) {
-            if (!mask_contains(p.mask, point))
-                break;
-        } else {
-            if (!first) {
-                const float dx = point.x - previous.x, dy = point.y - previous.y, dz = point.z - previous.z;
-                if (!mesh_far(p.mesh, previous, (dx * dx + dy * dy) + dz * dz)) {
-                    uint32_t tri;
-                    float t;
-                    if (bvh_segment_hit(p.mesh, previous, point, tri, t)) {
-                        hit = tri;
-                        break;
-                    }
-                }
-            }
-        }
-        const int64_t voxel = field_voxel(p.field, point);
-        if (voxel < 0)
-            break;
-        o = plain::update<MODEL, ALGO, KT>(p.algo, p.field.data + voxel * stride, o, rng);
-        steps++;
-        previous = point;
-        point.x += step * o.x;
-        point.y += step * o.y;
-        point.z += step * o.z;
-        first = false;
-        time += step;
-        if (time >= save) {
-            time -= save;
-            store_point(p.points, p.n_points, n, g, v3(point.x - time * o.x, point.y - time * o.y, point.z - time * o.z));
-            n++;
-        }
-    }
-    p.lengths[g] = n;
-    p.steps[g] = steps;
-    p.hits[g] = hit;
-}
-
-template <int MODEL, int ALGO, int DOMAIN>
-hipError_t launch_plain_kt(const TrackLaunch &p, hipStream_t stream)
-{
-    const dim3 block(256);
-    const dim3 grid((unsigned)((p.n_seeds + 255) / 256));
-    if (MODEL == MODEL_SH && p.algo.n_parameters == 45)
-        track_plain_kernel<MODEL, ALGO, DOMAIN, 45><<<grid, block, 0, stream>>>(p);
-    else
-        track_plain_kernel<MODEL, ALGO, DOMAIN, 0><<<grid, block, 0, stream>>>(p);
-    return hipGetLastError();
-}
-
-template <int MODEL, int ALGO>
-hipError_t launch_plain_domain(const TrackLaunch &p, hipStream_t stream)
-{
-    if (p.domain_kind == DOMAIN_MESH)
-        return launch_plain_kt<MODEL, ALGO, DOMAIN_MESH>(p, stream);
-    return launch_plain_kt<MODEL, ALGO, DOMAIN_MASK>(p, stream);
-}
-
-template <int MODEL>
-hipError_t launch_plain_algo(const TrackLaunch &p, hipStream_t stream)
-{
-    switch (p.algo.algorithm) {
-    case ALGO_DIFFUSION: return launch_plain_domain<MODEL, ALGO_DIFFUSION>(p, stream);
-    case ALGO_PROBABILISTIC: return launch_plain_domain<MODEL, ALGO_PROBABILISTIC>(p, stream);
-    case ALGO_DETERMINISTIC: return launch_plain_domain<MODEL, ALGO_DETERMINISTIC>(p, stream);
-    default: return launch_plain_domain<MODEL, ALGO_TRANSPORT>(p, stream);
-    }
-}
-
 // ------------------------------------------------------ one-op drivers ---
 
 // Loads K (<= 45) coefficients from an unpadded row.
@@ -315,6 +222,101 @@ __device__ __forceinline__ int64_t claim_seed(bool idle, uint32_t *queue, int64_
         return -1;
     const uint64_t g = (uint64_t)base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
     return g < (uint64_t)n ? (int64_t)g : -1;
+}
+
+// tracking.py:112-148 (kernel tractogram), one seed per lane at a time:
+// seed g's start from seeds[g], its noise stream from seed_stream, then the
+// reference loop (domain test, field voxel, update, move, save every save_at
+// of arc length in f32) until a test breaks it or the row or the step count
+// is full; its length, step count and boundary hit at the end.
+//
+// Persistent lanes: a grid that fills the GPU once, whose lanes claim the
+// next seed when their streamline ends (claim_seed). With one lane per
+// seed for the whole launch, a wavefront issued its longest streamline's
+// steps for all 64 lanes, and on a workload of 5 to 3000 steps that left
+// about half of them masked off. Each streamline's arithmetic and noise
+// stream are those of its seed index, whichever lane runs it, so the
+// results do not change.
+template <int MODEL, int ALGO, int DOMAIN, int KT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_min_waves<MODEL, ALGO>())))
+void track_plain_kernel(TrackLaunch p)
+{
+    const float step = p.algo.step_size;
+    const float save = p.save_at;
+    const uint32_t stride = p.field.stride;
+    int64_t g = -1;
+    bool drained = false;
+    V3 point = v3(0.0f, 0.0f, 0.0f), o = point, previous = point;
+    uint2 rng = make_uint2(0u, 0u);
+    uint32_t n = 0, steps = 0, hit = NO_HIT;
+    float time = 0.0f;
+    bool first = true;
+    while (true) {
+        const int64_t claimed = claim_seed(g < 0, p.queue, p.n_seeds, drained);
+        if (claimed >= 0) {
+            g = claimed;
+            const float4 s0 = reinterpret_cast<const float4 *>(p.seeds)[2 * g];
+            const float4 s1 = reinterpret_cast<const float4 *>(p.seeds)[2 * g + 1];
+            point = v3(s0.x, s0.y, s0.z);
+            o = v3(s1.x, s1.y, s1.z);
+            if constexpr (ALGO == ALGO_DIFFUSION || ALGO == ALGO_PROBABILISTIC)
+                rng = seed_stream(p, p.first_seed + g);
+            store_point(p.points, p.n_points, 0, g, point);
+            n = 1;
+            steps = 0;
+            hit = NO_HIT;
+            time = 0.0f;
+            previous = point;
+            first = true;
+        }
+        if (__ballot(g >= 0) == 0)
+            break;
+        if (g < 0)
+            continue;
+        // One pass of the reference loop; `stop` is its break.
+        bool stop = !(n < p.n_points && steps < p.max_steps);
+        if (!stop) {
+            if constexpr (DOMAIN == DOMAIN_MASK) {
+                stop = !mask_contains(p.mask, point);
+            } else if (!first) {
+                const float dx = point.x - previous.x, dy = point.y - previous.y, dz = point.z - previous.z;
+                if (!mesh_far(p.mesh, previous, (dx * dx + dy * dy) + dz * dz)) {
+                    uint32_t tri;
+                    float t;
+                    if (bvh_segment_hit(p.mesh, previous, point, tri, t)) {
+                        hit = tri;
+                        stop = true;
+                    }
+                }
+            }
+        }
+        int64_t voxel = -1;
+        if (!stop) {
+            voxel = field_voxel(p.field, point);
+            stop = voxel < 0;
+        }
+        if (!stop) {
+            o = plain::update<MODEL, ALGO, KT>(p.algo, p.field.data + voxel * stride, o, rng);
+            steps++;
+            previous = point;
+            point.x += step * o.x;
+            point.y += step * o.y;
+            point.z += step * o.z;
+            first = false;
+            time += step;
+            if (time >= save) {
+                time -= save;
+                store_point(p.points, p.n_points, n, g,
+                            v3(point.x - time * o.x, point.y - time * o.y, point.z - time * o.z));
+                n++;
+            }
+            continue;
+        }
+        p.lengths[g] = n;
+        p.steps[g] = steps;
+        p.hits[g] = hit;
+        g = -1;
+    }
 }
 
 // tracking.py:457-558 (kernel connectome): boundary seed g from
@@ -589,15 +591,6 @@ inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + thre
 
 }  // namespace
 
-hipError_t launch_track_plain(const TrackLaunch &p, hipStream_t stream)
-{
-    if (p.n_seeds == 0)
-        return hipSuccess;
-    if (p.algo.model == MODEL_DTI)
-        return launch_plain_algo<MODEL_DTI>(p, stream);
-    return launch_plain_algo<MODEL_SH>(p, stream);
-}
-
 hipError_t launch_evaluate(int model, int K, const float *params, const float *orientations, int64_t n, float *out,
                            hipStream_t stream)
 {
@@ -703,6 +696,44 @@ hipError_t launch_persistent(Kernel kernel, const Launch &args, uint32_t *queue,
     zero_u32_kernel<<<1, 64, 0, stream>>>(queue);
     kernel<<<blocks, 256, 0, stream>>>(args);
     return hipGetLastError();
+}
+
+template <int MODEL, int ALGO, int DOMAIN>
+hipError_t launch_plain_kt(const TrackLaunch &p, hipStream_t stream)
+{
+    if (MODEL == MODEL_SH && p.algo.n_parameters == 45)
+        return launch_persistent(track_plain_kernel<MODEL, ALGO, DOMAIN, 45>, p, p.queue, p.n_seeds, stream);
+    return launch_persistent(track_plain_kernel<MODEL, ALGO, DOMAIN, 0>, p, p.queue, p.n_seeds, stream);
+}
+
+template <int MODEL, int ALGO>
+hipError_t launch_plain_domain(const TrackLaunch &p, hipStream_t stream)
+{
+    if (p.domain_kind == DOMAIN_MESH)
+        return launch_plain_kt<MODEL, ALGO, DOMAIN_MESH>(p, stream);
+    return launch_plain_kt<MODEL, ALGO, DOMAIN_MASK>(p, stream);
+}
+
+template <int MODEL>
+hipError_t launch_plain_algo(const TrackLaunch &p, hipStream_t stream)
+{
+    switch (p.algo.algorithm) {
+    case ALGO_DIFFUSION: return launch_plain_domain<MODEL, ALGO_DIFFUSION>(p, stream);
+    case ALGO_PROBABILISTIC: return launch_plain_domain<MODEL, ALGO_PROBABILISTIC>(p, stream);
+    case ALGO_DETERMINISTIC: return launch_plain_domain<MODEL, ALGO_DETERMINISTIC>(p, stream);
+    default: return launch_plain_domain<MODEL, ALGO_TRANSPORT>(p, stream);
+    }
+}
+
+hipError_t launch_track_plain(const TrackLaunch &p, hipStream_t stream)
+{
+    if (p.n_seeds == 0)
+        return hipSuccess;
+    if (!p.queue)
+        return hipErrorInvalidValue;
+    if (p.algo.model == MODEL_DTI)
+        return launch_plain_algo<MODEL_DTI>(p, stream);
+    return launch_plain_algo<MODEL_SH>(p, stream);
 }
 
 template <int MODEL, int ALGO>
