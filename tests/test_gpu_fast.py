@@ -380,7 +380,7 @@ def test_fast_generations_change_no_result(scene, case):
     # A fast-path call runs as generations of the persistent kernel: when a
     # generation's work is all claimed and half of it has ended, the
     # streamlines in flight are saved and resumed by the next generation in
-    # fuller wavefronts. A threshold of 8 makes every call here run all six
+    # fuller wavefronts. A threshold of 8 makes every call here run all four
     # generations; the results must equal a single run to completion bit
     # for bit (noise streams, saved-point counters and arc-length clocks
     # carried across).
