@@ -12,8 +12,9 @@ namespace tgk {
 // Sets the thread's tgk_last_error() message and returns `code`.
 int report_error(int code, const char *fmt, ...);
 
-// Plain FP32 path (tgk_plain.hip): one lane per seed, seed-major dense output
-// (compacted by launch_compact_seed_major, as the fast path's).
+// Plain FP32 path (tgk_plain.hip): persistent lanes claiming seeds through
+// p.queue (zeroed by the launch), seed-major dense output (compacted by
+// launch_compact_seed_major, as the fast path's).
 hipError_t launch_track_plain(const TrackLaunch &p, hipStream_t stream);
 
 // Boundary connectome (tgk_plain.hip, plain arithmetic): a mask domain, its

@@ -1,10 +1,11 @@
 // tgk_plain.hip -- the plain FP32 tracking path, the boundary-seeded
 // products (connectome, TOD) and the one-op drivers.
 //
-// One integrator per seed, one lane per seed, exactly the reference's loop
-// (tracking.py:112-148). Compiled with -ffp-contract=off: every result is bit
-// identical to the FP32 host restatement. The connectome and TOD kernels run
-// the same arithmetic on persistent lanes that claim boundary seeds.
+// One integrator per seed, exactly the reference's loop (tracking.py:112-148),
+// on persistent lanes that claim the next seed when their streamline ends.
+// Compiled with -ffp-contract=off: every result is bit identical to the FP32
+// host restatement. The connectome and TOD kernels run the same arithmetic
+// on persistent lanes that claim boundary seeds.
 //
 // Output layout: seed-major, dense[(g * n_points + n) * 3 + c], the fast
 // path's layout, so one compaction kernel serves both paths. (A point-major
