@@ -32,19 +32,28 @@ __device__ __forceinline__ void store_point(float *dense, uint32_t n_points, uin
 
 // Waves per SIMD the compiler must fit in the SH tracking kernels, which
 // otherwise take 146-148 VGPRs (Transport, Diffusion; three waves), 164-170
-// (Probabilistic; three) or 291-295 (Deterministic; one). Held to four, four
-// and two waves they spill 10-54 VGPRs to scratch and run 5%, 12% and 75%
-// faster at 145x174x145 SH45. DTI kernels keep the compiler's allocation
-// (already below 128 VGPRs, they lost 11% when held to exactly four). The
-// connectome and TOD kernels are held to two waves for Deterministic only
-// (296 and 342 VGPRs, one wave otherwise: 1.52x faster each); the SH
-// Diffusion connectome held to four waves ran 11% slower.
-template <int MODEL, int ALGO> constexpr int plain_min_waves()
+// (Probabilistic; three) or 291-295 (Deterministic; one). The Deterministic
+// kernel is held to two waves (75% faster at 145x174x145 SH45, spilling
+// some 30-50 VGPRs; 3.52 against 5.95 s at one wave on persistent lanes).
+// The others: held to four, they spill 10-54 VGPRs. With one lane per seed
+// for the whole launch that was still 5% (Transport, Diffusion) and 12%
+// (Probabilistic) faster than three waves. On persistent lanes, mask
+// domains run faster at three waves (SH45 Diffusion 74.4 against 91.5 ms
+// per launch) and mesh domains, whose boundary tests wait on dependent
+// node loads, at four (the two-mesh shell 326 against 361 ms). DTI
+// kernels keep the compiler's allocation (already below 128 VGPRs, they
+// lost 11% when held to exactly four). The connectome and TOD kernels are
+// held to two waves for Deterministic only (296 and 342 VGPRs, one wave
+// otherwise: 1.52x faster each); the SH Diffusion connectome held to four
+// waves ran 11% slower.
+template <int MODEL, int ALGO, int DOMAIN> constexpr int plain_min_waves()
 {
     if constexpr (MODEL != MODEL_SH)
         return 1;
+    else if constexpr (ALGO == ALGO_DETERMINISTIC)
+        return 2;
     else
-        return ALGO == ALGO_DETERMINISTIC ? 2 : 4;
+        return DOMAIN == DOMAIN_MESH ? 4 : 3;
 }
 
 // ------------------------------------------------------ one-op drivers ---
@@ -239,7 +248,7 @@ __device__ __forceinline__ int64_t claim_seed(bool idle, uint32_t *queue, int64_
 // stream are those of its seed index, whichever lane runs it, so the
 // results do not change.
 template <int MODEL, int ALGO, int DOMAIN, int KT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_min_waves<MODEL, ALGO>())))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(plain_min_waves<MODEL, ALGO, DOMAIN>())))
 void track_plain_kernel(TrackLaunch p)
 {
     const float step = p.algo.step_size;
