@@ -39,6 +39,7 @@ namespace {
 
 constexpr int N_DIRECTIONS = 1000;
 constexpr int TABLE_STRIDE = 48;
+constexpr int DIRECTION_BLOCK = 4;  // directions evaluated together by the VALU method
 
 // seeds.cl:12-25 (host and device: the MFMA table is built on the host at
 // the same FP32 directions).
@@ -89,8 +90,11 @@ __global__ void basis_table_kernel(float *__restrict__ table)
 }
 
 // Model value at direction i (models/*.cl evaluate_interpolated_model):
-// SH: softplus of the in-order contraction; DTI: o . (D o).
-template <int MODEL>
+// SH: softplus of the in-order contraction of the first K terms; DTI:
+// o . (D o). The contraction is unrolled so that c stays in registers (a
+// loop to a run-time K indexed it in scratch memory); KT = 45 needs no
+// test, otherwise each term is taken while k < K (a uniform branch).
+template <int MODEL, int KT>
 __device__ __forceinline__ float model_value(const float *__restrict__ row, const float c[45], int K)
 {
     if constexpr (MODEL == MODEL_DTI) {
@@ -101,14 +105,16 @@ __device__ __forceinline__ float model_value(const float *__restrict__ row, cons
         return (ox * t0 + oy * t1) + oz * t2;
     } else {
         float v = 0.0f;
-        for (int k = 0; k < K; k++)
-            v += c[k] * row[k];
+#pragma unroll
+        for (int k = 0; k < 45; k++)
+            if (KT == 45 || k < K)
+                v += c[k] * row[k];
         return plain::softplus(v);
     }
 }
 
 // seeds.cl:68-118
-template <int MODEL>
+template <int MODEL, int KT>
 __global__ __launch_bounds__(256) void seeds_from_fod_kernel(const float *__restrict__ fod, int K,
                                                              const float4 *__restrict__ voxels, uint32_t nnz,
                                                              const float *__restrict__ affine,
@@ -128,20 +134,40 @@ __global__ __launch_bounds__(256) void seeds_from_fod_kernel(const float *__rest
     voxel.y += jy;
     voxel.z += jz;
     float c[45];
+#pragma unroll
     for (int k = 0; k < 45; k++)
         c[k] = k < K ? fod[(int64_t)index * K + k] : 0.0f;
 
-    // seeds.cl:28-62
+    // seeds.cl:28-62. Directions are evaluated DIRECTION_BLOCK at a time: their
+    // contractions are independent chains the wavefront can interleave (one
+    // at a time, each waited on its own dependent adds and table loads),
+    // and their weights are then added to the sums in direction order, so
+    // every rounding is the reference's.
+    static_assert(N_DIRECTIONS % DIRECTION_BLOCK == 0, "directions come in whole blocks");
     float sum = 0.0f;
-    for (int i = 0; i < N_DIRECTIONS; i++)
-        sum += fmaxf(model_value<MODEL>(table + i * TABLE_STRIDE, c, K), 0.0f);
+    for (int i = 0; i < N_DIRECTIONS; i += DIRECTION_BLOCK) {
+        float w[DIRECTION_BLOCK];
+#pragma unroll
+        for (int d = 0; d < DIRECTION_BLOCK; d++)
+            w[d] = fmaxf(model_value<MODEL, KT>(table + (i + d) * TABLE_STRIDE, c, K), 0.0f);
+#pragma unroll
+        for (int d = 0; d < DIRECTION_BLOCK; d++)
+            sum += w[d];
+    }
     const float threshold = randu(state) * sum;
     float cs = 0.0f;
     int selected = N_DIRECTIONS;
-    for (int i = 0; i < N_DIRECTIONS; i++) {
-        cs += fmaxf(model_value<MODEL>(table + i * TABLE_STRIDE, c, K), 0.0f);
-        if (selected == N_DIRECTIONS && cs > threshold)
-            selected = i;
+    for (int i = 0; i < N_DIRECTIONS; i += DIRECTION_BLOCK) {
+        float w[DIRECTION_BLOCK];
+#pragma unroll
+        for (int d = 0; d < DIRECTION_BLOCK; d++)
+            w[d] = fmaxf(model_value<MODEL, KT>(table + (i + d) * TABLE_STRIDE, c, K), 0.0f);
+#pragma unroll
+        for (int d = 0; d < DIRECTION_BLOCK; d++) {
+            cs += w[d];
+            if (selected == N_DIRECTIONS && cs > threshold)
+                selected = i + d;
+        }
     }
     float ox = 0.0f, oy = 0.0f, oz = 1.0f;
     if (selected < N_DIRECTIONS) {
@@ -460,11 +486,15 @@ hipError_t launch_seeds_from_fod_prepared(int model, int K, const float *fod, co
     }
     const float *t = static_cast<const float *>(table);
     if (model == MODEL_DTI)
-        seeds_from_fod_kernel<MODEL_DTI><<<blocks, 256, 0, stream>>>(
+        seeds_from_fod_kernel<MODEL_DTI, 0><<<blocks, 256, 0, stream>>>(
+            fod, K, reinterpret_cast<const float4 *>(voxels), nnz, affine, t, reinterpret_cast<uint2 *>(states), n,
+            out);
+    else if (K == 45)
+        seeds_from_fod_kernel<MODEL_SH, 45><<<blocks, 256, 0, stream>>>(
             fod, K, reinterpret_cast<const float4 *>(voxels), nnz, affine, t, reinterpret_cast<uint2 *>(states), n,
             out);
     else
-        seeds_from_fod_kernel<MODEL_SH><<<blocks, 256, 0, stream>>>(
+        seeds_from_fod_kernel<MODEL_SH, 0><<<blocks, 256, 0, stream>>>(
             fod, K, reinterpret_cast<const float4 *>(voxels), nnz, affine, t, reinterpret_cast<uint2 *>(states), n,
             out);
     return hipGetLastError();
