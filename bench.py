@@ -403,8 +403,11 @@ def brain145_sh91_fod_seeds(seed: int = 0, n_seeds: int = 10_000_000, shape=SHAP
 # --------------------------------------------------------------------------
 
 MI355X_HBM_PEAK_GBS = 8000.0
-# Wavefront VALU instructions per second, one per SIMD every 4 cycles (a
-# wave64 FP32 instruction on a 16-lane SIMD): 256 CUs x 4 SIMDs x 2.4 GHz / 4.
+# SIMD quad-cycles per second: 256 CUs x 4 SIMDs x 2.4 GHz / 4. A wave64 VALU
+# instruction takes one quad-cycle of its SIMD; gfx950 issues two in one
+# quad-cycle when their operands are all VGPRs (FP32 add, mul and fma measured
+# at 2.4-2.6 cycles each, one per 4.2 cycles with an SGPR operand, FP64 fma
+# one per 4.6), which SQ_ACTIVE_INST_VALU2 counts.
 MI355X_VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 
@@ -1055,17 +1058,27 @@ def main():
         line["roofline"]["reads"] = h["reads"]
         line["roofline"]["steps_per_row_read"] = h["steps"] / max(1, h["reads"]["rows"])
     if traffic and headline == "fast":
-        line["roofline"]["traffic_detail"] = {k: v for k, v in traffic.items() if k not in ("valu_instructions", "l2")}
+        line["roofline"]["traffic_detail"] = {k: v for k, v in traffic.items()
+                                              if k not in ("valu_instructions", "valu_dual_quad_cycles", "l2")}
         if traffic.get("l2"):
             l2 = traffic["l2"]
             line["roofline"]["l2"] = {**l2, "hit_rate": l2["hits"] / max(1.0, l2["hits"] + l2["misses"]),
                                       "traffic_gbs": traffic["bytes_per_launch"] / (h["kernel_ms"] * 1e-3) / 1e9}
-        # The binding resource: wavefront VALU instructions issued per second
-        # of the kernel against the issue rate of all SIMDs.
-        issued = traffic["valu_instructions"] / (h["kernel_ms"] * 1e-3)
-        line["roofline"]["issue"] = {"bound": "valu", "achieved": issued, "peak": MI355X_VALU_ISSUE_PEAK,
-                                     "unit": "wave instructions/s", "frac": issued / MI355X_VALU_ISSUE_PEAK,
-                                     "wave_instructions_per_64_steps": traffic["valu_instructions"] * 64.0 / h["steps"]}
+        # VALU occupancy: the quad-cycles the kernel's VALU instructions took
+        # (instructions less the quad-cycles in which two issued together)
+        # per second of the kernel, against all SIMDs' quad-cycles.
+        seconds = h["kernel_ms"] * 1e-3
+        instructions = traffic["valu_instructions"]
+        dual = traffic.get("valu_dual_quad_cycles") or 0.0
+        busy = (instructions - dual) / seconds
+        line["roofline"]["issue"] = {
+            "bound": "valu", "achieved": busy, "peak": MI355X_VALU_ISSUE_PEAK,
+            "unit": "VALU-busy SIMD quad-cycles/s", "frac": busy / MI355X_VALU_ISSUE_PEAK,
+            "basis": "SQ_INSTS_VALU less SQ_ACTIVE_INST_VALU2 (quad-cycles that issued two VALU instructions) "
+                     "over the SIMDs' quad-cycles at 2.4 GHz",
+            "wave_instructions_per_64_steps": instructions * 64.0 / h["steps"],
+            "dual_issued_fraction": 2.0 * dual / max(1.0, instructions),
+            "single_issue_frac": instructions / seconds / MI355X_VALU_ISSUE_PEAK}
     if "fast" in results and "plain" in results:
         line["speedup_fast_over_plain"] = (results["fast"]["steps"] / results["fast"]["ms_per_step"]) / (
             results["plain"]["steps"] / results["plain"]["ms_per_step"])
@@ -1126,8 +1139,9 @@ def traffic_leg(args):
     counter runs of this script (FETCH_SIZE, then WRITE_SIZE, KiB per
     dispatch), each scaled by what the counter reads for an elementwise copy
     that reads and writes 1 GiB in the same run, and the kernel's wavefront
-    VALU instructions per launch from a third run (SQ_INSTS_VALU). Returns
-    None if the profiler is unavailable or a run fails."""
+    VALU instructions per launch and the quad-cycles that issued two of them
+    from a third run (SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU2). Returns None if
+    the profiler is unavailable or a run fails."""
     import shutil
     import subprocess
     import tempfile
@@ -1137,7 +1151,7 @@ def traffic_leg(args):
         return None
     measured = {}
     with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
-        for counter in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU", "TCC_HIT_sum TCC_MISS_sum"):
+        for counter in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU SQ_ACTIVE_INST_VALU2", "TCC_HIT_sum TCC_MISS_sum"):
             out = Path(tmp) / counter.split()[0]
             cmd = [profiler, "--pmc", *counter.split(), "--output-format", "csv", "-d", str(out), "-o", "run", "--",
                    sys.executable, str(ROOT / "bench.py"), "--workload", args.workload, "--paths", "fast",
@@ -1169,15 +1183,20 @@ def traffic_leg(args):
                 if hits and misses:
                     measured["l2"] = {"hits": float(np.sum(hits[0])), "misses": float(np.sum(misses[0]))}
                 continue
-            values = _counter_values(files[0], counter)
-            kernel = [v for k, v in values.items() if "track_fast_kernel" in k]
             # One tracking launch is several dispatches of the kernel (its
             # generations): per-launch figures are their sums.
-            if counter == "SQ_INSTS_VALU":
-                if not kernel:
+            if counter.startswith("SQ_INSTS_VALU"):
+                instructions = [v for k, v in _counter_values(files[0], "SQ_INSTS_VALU").items()
+                                if "track_fast_kernel" in k]
+                dual = [v for k, v in _counter_values(files[0], "SQ_ACTIVE_INST_VALU2").items()
+                        if "track_fast_kernel" in k]
+                if not instructions:
                     return None
-                measured[counter] = float(np.sum(kernel[0]))
+                measured["SQ_INSTS_VALU"] = float(np.sum(instructions[0]))
+                measured["SQ_ACTIVE_INST_VALU2"] = float(np.sum(dual[0])) if dual else None
                 continue
+            values = _counter_values(files[0], counter)
+            kernel = [v for k, v in values.items() if "track_fast_kernel" in k]
             copy = [max(v) for k, v in values.items() if "CUDAFunctorOnSelf_add" in k]
             if not kernel or not copy or copy[0] <= 0:
                 return None
@@ -1187,7 +1206,8 @@ def traffic_leg(args):
     return {"bytes_per_launch": measured["FETCH_SIZE"]["bytes"] + measured["WRITE_SIZE"]["bytes"],
             "fetch_bytes": measured["FETCH_SIZE"]["bytes"], "write_bytes": measured["WRITE_SIZE"]["bytes"],
             "fetch_scale": measured["FETCH_SIZE"]["scale"], "write_scale": measured["WRITE_SIZE"]["scale"],
-            "valu_instructions": measured["SQ_INSTS_VALU"], "l2": measured.get("l2")}
+            "valu_instructions": measured["SQ_INSTS_VALU"], "valu_dual_quad_cycles": measured["SQ_ACTIVE_INST_VALU2"],
+            "l2": measured.get("l2")}
 
 
 def calibration_copy(torch, n_bytes: int):

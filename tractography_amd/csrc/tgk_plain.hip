@@ -11,7 +11,7 @@
 // path's layout, so one compaction kernel serves both paths. (A point-major
 // layout made each save of a wavefront one contiguous store, but its
 // compaction had to gather rows of every point index: 9.2 ms at 1M seeds
-// against 2.5 ms, while the saves cost this VALU-bound kernel nothing
+// against 2.5 ms, while the saves cost this arithmetic-heavy kernel nothing
 // measurable either way.)
 #include <hip/hip_runtime.h>
 
