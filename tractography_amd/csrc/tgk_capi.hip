@@ -713,6 +713,9 @@ int tgk_domain_set_segment_grid(tgk_domain *domain, const uint32_t *cell_start, 
     if (n_cells != cells)
         return fail(TGK_ERR_INVALID, "candidate lists for %llu cells, the culling grid has %llu",
                     (unsigned long long)n_cells, (unsigned long long)cells);
+    // Tracking lanes keep a 32-bit cell id with two values reserved.
+    if (n_cells >= 0xfffffffeull)
+        return fail(TGK_ERR_INVALID, "candidate lists need a culling grid of fewer than 2^32 - 2 cells");
     if (cell_start[0] != 0 || cell_start[n_cells] != n_entries || n_entries > 0xffffffffull)
         return fail(TGK_ERR_INVALID, "candidate offsets do not span the entries");
     for (uint64_t i = 0; i < n_cells; i++)

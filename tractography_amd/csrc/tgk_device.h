@@ -472,14 +472,41 @@ __device__ __forceinline__ bool grid_triangle_hit(float4 a4, float4 e14, float4 
     return true;
 }
 
+// Closest hit of the segment start + t dir (t in (0, 1]) among candidates
+// [first, last) of the culling grid's lists, tested in list order with the
+// traversal's replacement rule (t <= best replaces).
+__device__ __forceinline__ bool grid_candidates_hit(const MeshView &m, uint32_t first, uint32_t last,
+                                                    const float st[3], const float dir[3], uint32_t &hit,
+                                                    float &hit_t)
+{
+    bool found = false;
+    for (uint32_t i = first; i < last; i++) {
+        const float4 a = m.cell_entries[3 * (size_t)i];
+        const float4 e1 = m.cell_entries[3 * (size_t)i + 1];
+        const float4 e2 = m.cell_entries[3 * (size_t)i + 2];
+        float t;
+        if (grid_triangle_hit(a, e1, e2, st, dir, hit_t, t)) {
+            hit_t = t;
+            hit = __float_as_uint(a.w);
+            found = true;
+        }
+    }
+    return found;
+}
+
 // Closest hit of a segment no longer than `reach` through the candidate
 // lists: the candidates of the start cell hold every triangle whose bounding
 // box the segment can touch, in the traversal's leaf order, so the same
 // replacement rule (t <= best replaces) gives bvh_segment_hit's answer.
 // Returns false with `decided` unset when the lists cannot decide (no lists,
 // segment too long, start outside the grid).
+// `cached_cell`, `first` and `last` keep the start cell's candidate range
+// between calls: a lane's segments, 0.1 voxel long, start in one culling cell
+// for several steps, and a hit on the cached range reads no offsets. The
+// caller sets `cached_cell` to a value no cell has (the grid holds fewer
+// than 2^32 - 2 cells) to start.
 __device__ inline bool grid_segment_hit(const MeshView &m, V3 start, V3 end, bool &decided, uint32_t &hit,
-                                        float &hit_t)
+                                        float &hit_t, uint32_t &cached_cell, uint32_t &first, uint32_t &last)
 {
     decided = false;
     hit_t = 1.0f;
@@ -495,20 +522,19 @@ __device__ inline bool grid_segment_hit(const MeshView &m, V3 start, V3 end, boo
     if (cell < 0)
         return false;
     decided = true;
-    bool found = false;
-    const uint32_t end_i = m.cell_start[cell + 1];
-    for (uint32_t i = m.cell_start[cell]; i < end_i; i++) {
-        const float4 a = m.cell_entries[3 * (size_t)i];
-        const float4 e1 = m.cell_entries[3 * (size_t)i + 1];
-        const float4 e2 = m.cell_entries[3 * (size_t)i + 2];
-        float t;
-        if (grid_triangle_hit(a, e1, e2, st, dir, hit_t, t)) {
-            hit_t = t;
-            hit = __float_as_uint(a.w);
-            found = true;
-        }
+    if ((uint32_t)cell != cached_cell) {
+        cached_cell = (uint32_t)cell;
+        first = m.cell_start[cell];
+        last = m.cell_start[cell + 1];
     }
-    return found;
+    return grid_candidates_hit(m, first, last, st, dir, hit, hit_t);
+}
+
+__device__ inline bool grid_segment_hit(const MeshView &m, V3 start, V3 end, bool &decided, uint32_t &hit,
+                                        float &hit_t)
+{
+    uint32_t cached_cell = 0xffffffffu, first = 0, last = 0;
+    return grid_segment_hit(m, start, end, decided, hit, hit_t, cached_cell, first, last);
 }
 
 // True when the culling grid proves that a segment from `start` of squared

@@ -102,6 +102,9 @@ template <int G> struct Lane {
     uint32_t field_cell; // cell ids of `point` (UNSET_CELL: none yet); their data is requested by request_cells
     uint32_t mask_cell;
     uint32_t mask_value; // the mask at mask_cell (0 outside the mask image)
+    // Mesh domains: culling-grid cell of the last tested segment's start
+    // (UNSET_CELL: none yet) and its candidate range [cand_first, cand_last).
+    uint32_t mesh_cell, cand_first, cand_last;
     bool field_ok;       // `point` is inside the field image
     bool row_read;       // request_cells read a row / a mask value this iteration
     bool mask_read;
@@ -405,7 +408,8 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, bool shared_affine
             bool decided;
             uint32_t tri;
             float t;
-            if (grid_segment_hit(p.mesh, L.previous, L.point, decided, tri, t)) {
+            if (grid_segment_hit(p.mesh, L.previous, L.point, decided, tri, t, L.mesh_cell, L.cand_first,
+                                 L.cand_last)) {
                 hit = tri;
                 stop = true;
             } else if (!decided) {
@@ -568,6 +572,7 @@ void track_fast_kernel(FastLaunch f)
                     L.field_cell = UNSET_CELL;
                     L.mask_cell = UNSET_CELL;
                     L.mask_value = 0;
+                    L.mesh_cell = UNSET_CELL;
                     request_cells<MODEL, DOMAIN, LMAX, G, LDS_VEC, BLOCK>(p, f.shared_affine != 0, coeffs, cstride,
                                                                           L, q, lds_rows);
                 }
