@@ -268,6 +268,35 @@ def test_fast_mesh_domain_tracks_fp64_oracle(fast_runtime, scene, oracle_threads
     assert hits_equal > 0.99
 
 
+def test_fast_mesh_candidate_lists_equal_the_bvh_path(fast_runtime, scene):
+    # The fast kernel answers a step's boundary test from its start cell's
+    # candidate list, the lanes' lists spread over the wavefront; with no
+    # reach (reach 0) every test traverses the BVH. Both must give the same
+    # hits, hence the same tracks, bit for bit.
+    import bench
+
+    cfg, field, _ = scene
+    outer, inner, amplitude = (13.0, 14.0, 12.0), (5.0, 6.0, 5.0), 1.0
+    geometry = np.random.default_rng(13)
+    vertices, faces, centre, radii = bench.shell_domain(geometry, (32, 32, 32), outer, inner, 30, 36, amplitude)
+    seeds = bench.seeds_in_shell(geometry, centre, radii, np.asarray(outer) + amplitude + 1.0, 4000)
+    algorithm = tg.algorithm.Diffusion(tg.local.RealSymmetricSphericalHarmonics(45), step_size=0.1, gamma=1.0,
+                                       noise_variance=0.1, seed=23)
+    rng = [algorithm.get_rng_seed()]
+    results = []
+    for cell_size, reach in [(0.5, 0.15), (1.0, 0.5), (1.0, 0.0)]:
+        domain = tg.domain.Mesh(vertices, faces, cell_size=cell_size, reach=reach)
+        results.append(tg.tracking.track(algorithm, domain, field, seeds, rng, 4000, 0.25, 80.0, fast_runtime))
+    hits = np.mean(results[-1].hits != np.iinfo(np.uint32).max)
+    print(f"\n[mesh lists] {hits:.3f} of streamlines end on a boundary hit")
+    assert hits > 0.3
+    for r in results[:-1]:
+        np.testing.assert_array_equal(r.lengths, results[-1].lengths)
+        np.testing.assert_array_equal(r.steps, results[-1].steps)
+        np.testing.assert_array_equal(r.hits, results[-1].hits)
+        np.testing.assert_array_equal(r.points, results[-1].points)
+
+
 def _order12_scene(n_seeds=1500, seed=30):
     rng = np.random.default_rng(seed)
     shape = (24, 24, 24)

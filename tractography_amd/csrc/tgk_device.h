@@ -500,13 +500,8 @@ __device__ __forceinline__ bool grid_candidates_hit(const MeshView &m, uint32_t 
 // replacement rule (t <= best replaces) gives bvh_segment_hit's answer.
 // Returns false with `decided` unset when the lists cannot decide (no lists,
 // segment too long, start outside the grid).
-// `cached_cell`, `first` and `last` keep the start cell's candidate range
-// between calls: a lane's segments, 0.1 voxel long, start in one culling cell
-// for several steps, and a hit on the cached range reads no offsets. The
-// caller sets `cached_cell` to a value no cell has (the grid holds fewer
-// than 2^32 - 2 cells) to start.
 __device__ inline bool grid_segment_hit(const MeshView &m, V3 start, V3 end, bool &decided, uint32_t &hit,
-                                        float &hit_t, uint32_t &cached_cell, uint32_t &first, uint32_t &last)
+                                        float &hit_t)
 {
     decided = false;
     hit_t = 1.0f;
@@ -522,19 +517,7 @@ __device__ inline bool grid_segment_hit(const MeshView &m, V3 start, V3 end, boo
     if (cell < 0)
         return false;
     decided = true;
-    if ((uint32_t)cell != cached_cell) {
-        cached_cell = (uint32_t)cell;
-        first = m.cell_start[cell];
-        last = m.cell_start[cell + 1];
-    }
-    return grid_candidates_hit(m, first, last, st, dir, hit, hit_t);
-}
-
-__device__ inline bool grid_segment_hit(const MeshView &m, V3 start, V3 end, bool &decided, uint32_t &hit,
-                                        float &hit_t)
-{
-    uint32_t cached_cell = 0xffffffffu, first = 0, last = 0;
-    return grid_segment_hit(m, start, end, decided, hit, hit_t, cached_cell, first, last);
+    return grid_candidates_hit(m, m.cell_start[cell], m.cell_start[cell + 1], st, dir, hit, hit_t);
 }
 
 // True when the culling grid proves that a segment from `start` of squared

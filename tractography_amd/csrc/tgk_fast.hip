@@ -388,13 +388,103 @@ __device__ __forceinline__ void request_cells(const TrackLaunch &p, bool shared_
     }
 }
 
+// Boundary test of the wavefront's last segments against their start cells'
+// candidate lists, the (lane, candidate) pairs spread over the wavefront's
+// lanes. Lanes near a surface are few (about one in twenty per step), so
+// looping per lane would run the longest list for all 64 lanes; here a step
+// costs one round of 64 pairs while the lists sum to 64 or fewer. Each pair
+// is tested with the bound 1 (grid_triangle_hit), and a lane keeps its
+// smallest t, the last in list order among equal t: the candidate that
+// testing its list in order with "t <= best replaces" ends on
+// (grid_segment_hit), so the hit is the same. Called by every lane of the
+// wavefront; `decided` is false where the lists cannot decide (the caller
+// then asks the BVH), `found`/`tri` give the hit.
+template <int G>
+__device__ __forceinline__ void wave_segment_test(const MeshView &m, Lane<G> &L, bool live, bool &decided,
+                                                  bool &found, uint32_t &tri)
+{
+    decided = false;
+    found = false;
+    tri = 0;
+    const int lane = __lane_id();
+    const float sx = L.previous.x, sy = L.previous.y, sz = L.previous.z;
+    const float dx = L.point.x - sx, dy = L.point.y - sy, dz = L.point.z - sz;
+    uint32_t first = 0, count = 0;
+    if (live && L.steps > 0 && m.cell_start != nullptr && m.reach > 0.0f &&
+        (dx * dx + dy * dy) + dz * dz <= m.reach * m.reach) {
+        const int64_t cell = mesh_grid_cell(m, L.previous);
+        if (cell >= 0) {
+            decided = true;
+            if ((uint32_t)cell != L.mesh_cell) {
+                L.mesh_cell = (uint32_t)cell;
+                L.cand_first = m.cell_start[cell];
+                L.cand_last = m.cell_start[cell + 1];
+            }
+            first = L.cand_first;
+            count = L.cand_last - L.cand_first;
+        }
+    }
+    // Pairs [begin, end) of the flattened list belong to this lane.
+    uint32_t end = count;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(end, d);
+        if (lane >= d)
+            end += v;
+    }
+    const uint32_t total = __shfl(end, 63);
+    const uint32_t begin = end - count;
+    float best_t = 1.0f;
+    uint32_t best_index = 0;
+    for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t k = base + (uint32_t)lane;
+        // Owner of pair k: the last lane whose begin is at most k (begins
+        // never decrease along the lanes; lanes after the owner begin past k).
+        int owner = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1) {
+            if (__shfl(begin, owner + step) <= k)
+                owner += step;
+        }
+        const uint32_t index = __shfl(first, owner) + (k - __shfl(begin, owner));
+        const float st[3] = {__shfl(sx, owner), __shfl(sy, owner), __shfl(sz, owner)};
+        const float dir[3] = {__shfl(dx, owner), __shfl(dy, owner), __shfl(dz, owner)};
+        bool h = false;
+        float t = 0.0f;
+        uint32_t slot = 0;
+        if (k < total) {
+            const float4 a = m.cell_entries[3 * (size_t)index];
+            const float4 e1 = m.cell_entries[3 * (size_t)index + 1];
+            const float4 e2 = m.cell_entries[3 * (size_t)index + 2];
+            h = grid_triangle_hit(a, e1, e2, st, dir, 1.0f, t);
+            slot = __float_as_uint(a.w);
+        }
+        // Hits are rare: hand each to its owner in turn.
+        uint64_t hits = __ballot(h);
+        while (hits != 0) {
+            const int j = __ffsll((unsigned long long)hits) - 1;
+            hits &= hits - 1;
+            const int oj = __shfl(owner, j);
+            const float tj = __shfl(t, j);
+            const uint32_t ij = __shfl(index, j);
+            const uint32_t sj = __shfl(slot, j);
+            if (lane == oj && (!found || tj < best_t || (tj == best_t && ij > best_index))) {
+                found = true;
+                best_t = tj;
+                best_index = ij;
+                tri = sj;
+            }
+        }
+    }
+}
+
 // nz: this step's Diffusion draws, made by the caller before anything that
 // waits on memory, so they cover the latency of the cell data
 // request_cells asked for.
 template <int MODEL, int ALGO, int DOMAIN, int LMAX, int G, int LDS_VEC, int BLOCK>
 __device__ __forceinline__ void advance(const TrackLaunch &p, bool shared_affine, const float *__restrict__ coeffs,
                                         uint32_t cstride, Lane<G> &L, float *q, float4 *__restrict__ lds_rows,
-                                        const fast::Noise &nz)
+                                        const fast::Noise &nz, bool seg_decided, bool seg_found, uint32_t seg_tri)
 {
     bool stop = !(L.n < p.n_points && L.steps < p.max_steps);
     uint32_t hit = NO_HIT;
@@ -402,17 +492,15 @@ __device__ __forceinline__ void advance(const TrackLaunch &p, bool shared_affine
         if constexpr (DOMAIN == DOMAIN_MASK) {
             stop = L.mask_value == 0u;
         } else if (L.steps > 0) {
-            // The start cell's candidate list answers segments up to `reach`
-            // long with a few independent loads; longer segments, or starts
-            // off the grid, traverse the BVH.
-            bool decided;
-            uint32_t tri;
-            float t;
-            if (grid_segment_hit(p.mesh, L.previous, L.point, decided, tri, t, L.mesh_cell, L.cand_first,
-                                 L.cand_last)) {
-                hit = tri;
+            // The start cell's candidate list (wave_segment_test) answers
+            // segments up to `reach` long; longer segments, or starts off the
+            // grid, traverse the BVH.
+            if (seg_found) {
+                hit = seg_tri;
                 stop = true;
-            } else if (!decided) {
+            } else if (!seg_decided) {
+                uint32_t tri;
+                float t;
                 const float dx = L.point.x - L.previous.x, dy = L.point.y - L.previous.y,
                             dz = L.point.z - L.previous.z;
                 if (!mesh_far(p.mesh, L.previous, dx * dx + dy * dy + dz * dz) &&
@@ -622,9 +710,13 @@ void track_fast_kernel(FastLaunch f)
             ended_pending = 0;
             break;
         }
+        bool seg_decided = false, seg_found = false;
+        uint32_t seg_tri = 0;
+        if constexpr (DOMAIN == DOMAIN_MESH)
+            wave_segment_test<G>(p.mesh, L, L.g >= 0, seg_decided, seg_found, seg_tri);
         if (L.g >= 0)
             advance<MODEL, ALGO, DOMAIN, LMAX, G, LDS_VEC, BLOCK>(p, f.shared_affine != 0, coeffs, cstride, L, q,
-                                                                 lds_rows, nz);
+                                                                 lds_rows, nz, seg_decided, seg_found, seg_tri);
         rows_read += (uint64_t)__popcll(__ballot(L.row_read));
         masks_read += (uint64_t)__popcll(__ballot(L.mask_read));
         L.row_read = false;
