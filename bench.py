@@ -913,7 +913,7 @@ def main():
                         help="launch shape of the order-8 SH Transport / Diffusion fast kernels "
                              "(tractography_amd.runtime.fast_shapes(); default 0)")
     parser.add_argument("--autotune", action="store_true",
-                        help="choose the fast kernel's launch shape, grid and repack threshold on 100k seeds of "
+                        help="choose the fast kernel's launch shape, grid and repack threshold on up to 2M seeds of "
                              "the workload before timing")
     parser.add_argument("--repack", type=int, default=0,
                         help="fast path: streamlines below which a generation is no longer cut short "
@@ -975,10 +975,13 @@ def main():
     runtime = tg.HipRuntime()
     tuned = None
     if args.autotune:
-        # Kernel-only timing of a 100k-seed sample for each grid and repack
-        # threshold (not cached: the bench measures what it tunes).
+        # Kernel-only timing of the workload's seeds (up to 2M) for each
+        # launch shape, grid and repack threshold (not cached: the bench
+        # measures what it tunes). A sample smaller than a few grid fills
+        # (about 262k lanes resident) would time a launch that never drains a
+        # queue, and the grid and repack choices would not carry over.
         tuner = tg.HipRuntime(path="fast")
-        sample = cfg["seeds"][:100_000]
+        sample = cfg["seeds"][:2_000_000]
         tuned = tg.autotune.tune_fast(make_algorithm(tg, cfg), make_domain(tg, cfg),
                                       tg.field.Image(tg.Nifti1Image(cfg["fod"], np.eye(4))), sample,
                                       runtime=tuner, save_at=cfg["save_at"],

@@ -112,8 +112,9 @@ def time_fast(runtime: HipRuntime, algorithm, domain, field, seeds, *, save_at: 
 def tune_fast(algorithm, domain, field, seeds, *, runtime: HipRuntime, save_at: float,
               maximum_streamline_length: float, grid_factors=DEFAULT_GRID_FACTORS, repacks=DEFAULT_REPACKS,
               shapes=DEFAULT_SHAPES, repeats: int = 2, cache: bool | str | Path = True, force: bool = False) -> dict:
-    """Times `seeds` (a sample of the workload, e.g. 100k rows of (N, 8)) on
-    the fast path: every launch shape at the occupancy-sized grid and the
+    """Times `seeds` (a sample of the workload, rows of (N, 8), several times
+    the lanes the grid keeps resident: about 262k on a 256-CU device, so that
+    the queue outlasts the first fill as in the full run) on the fast path: every launch shape at the occupancy-sized grid and the
     default repack threshold, then, with the fastest shape, every grid (0 =
     occupancy-sized, or factor x compute units x 1024 lanes) and repack
     threshold; sets the fastest on `runtime`. With `cache`, a stored entry

@@ -1184,13 +1184,17 @@ hipError_t launch_fast(const FastLaunch &f, int grid_blocks, bool query, int64_t
     // Generation g resumes what g - 1 saved in cont[(g - 1) % 2] and saves
     // into cont[g % 2], whose count (generation g - 2's, consumed by g - 1)
     // it clears first.
+    // A first generation that is never cut short (half its resident work
+    // below the repack threshold) is the only one.
+    const uint64_t first_work = std::min<uint64_t>((uint64_t)f.track.n_seeds, (uint64_t)grid_blocks * BLOCK);
+    const int generations = first_work / 2 >= f.min_repack ? FAST_GENERATIONS : 1;
     FastLaunch gen = f;
-    for (int g = 0; g < FAST_GENERATIONS; g++) {
+    for (int g = 0; g < generations; g++) {
         gen.cont_in = g == 0 ? nullptr : f.cont[(g - 1) % 2];
         gen.cont_in_count = g == 0 ? nullptr : f.cont_count[(g - 1) % 2];
         gen.cont_out = f.cont[g % 2];
         gen.cont_out_count = f.cont_count[g % 2];
-        gen.last = g == FAST_GENERATIONS - 1;
+        gen.last = g == generations - 1;
         zero_generation<<<1, 64, 0, stream>>>(f.claims, f.ended, gen.cont_out_count, g == 0 ? f.stats : nullptr);
         const int blocks_g = std::max(1, grid_blocks >> g);
         track_fast_kernel<MODEL, ALGO, DOMAIN, LMAX, S><<<blocks_g, BLOCK, 0, stream>>>(gen);
