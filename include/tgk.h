@@ -193,8 +193,8 @@ int tgk_domain_set_boundary(tgk_domain *domain, const float *vertices, uint32_t 
  * entries (n_entries, 12) f32 rows (a.xyz, slot bits, (b - a).xyz, 0,
  * (c - a).xyz, 0) per candidate, in BVH leaf order within a cell. The FAST
  * path answers segments up to `reach` long from these lists instead of a
- * traversal. The grid must hold fewer than 2^32 - 2 cells (TGK_ERR_INVALID
- * otherwise). No counterpart in the reference, whose kernel traverses the
+ * traversal. The grid must hold fewer than 2^32 - 2 cells, each listing
+ * fewer than 2^26 candidates (TGK_ERR_INVALID otherwise). No counterpart in the reference, whose kernel traverses the
  * BVH for every step (bvh.cl:83-131). */
 int tgk_domain_set_segment_grid(tgk_domain *domain, const uint32_t *cell_start, uint64_t n_cells,
                                 const float *entries, uint64_t n_entries);

@@ -352,3 +352,28 @@ def test_candidate_lists_answer_as_the_bvh(runtime):
     np.testing.assert_array_equal(grid[0], bvh[0])
     np.testing.assert_array_equal(grid[1][bvh[0]], bvh[1][bvh[0]])
     np.testing.assert_array_equal(grid[2][bvh[0]], bvh[2][bvh[0]])
+
+
+def test_candidate_lists_refused_beyond_their_limits(runtime):
+    # Offsets that decrease, or a cell listing 2^26 candidates or more (a
+    # wavefront sums 64 lists in 32 bits), are refused before any entry is read.
+    from tractography_amd import _native
+
+    mesh, _, _, _ = fixtures.shell_scene(n_seeds=10, seed=21)
+    device = mesh.upload(runtime)
+    near, _, _ = mesh.culling_grid()
+    n_cells = int(np.prod(near.shape))
+    entries = np.zeros(12, dtype=np.float32)
+    lib = _native.lib()
+    for big in [1 << 26, (1 << 26) + 5]:
+        cell_start = np.zeros(n_cells + 1, dtype=np.uint32)
+        cell_start[n_cells // 2 + 1:] = big
+        with pytest.raises(ValueError, match="2\\^26"):
+            _native.check(lib.tgk_domain_set_segment_grid(device.handle, _native.ptr(cell_start), n_cells,
+                                                           _native.ptr(entries), big))
+    cell_start = np.zeros(n_cells + 1, dtype=np.uint32)
+    cell_start[1] = 3
+    cell_start[-1] = 1
+    with pytest.raises(ValueError, match="decrease"):
+        _native.check(lib.tgk_domain_set_segment_grid(device.handle, _native.ptr(cell_start), n_cells,
+                                                       _native.ptr(entries), 1))

@@ -718,9 +718,14 @@ int tgk_domain_set_segment_grid(tgk_domain *domain, const uint32_t *cell_start, 
         return fail(TGK_ERR_INVALID, "candidate lists need a culling grid of fewer than 2^32 - 2 cells");
     if (cell_start[0] != 0 || cell_start[n_cells] != n_entries || n_entries > 0xffffffffull)
         return fail(TGK_ERR_INVALID, "candidate offsets do not span the entries");
-    for (uint64_t i = 0; i < n_cells; i++)
+    for (uint64_t i = 0; i < n_cells; i++) {
         if (cell_start[i + 1] < cell_start[i])
             return fail(TGK_ERR_INVALID, "candidate offsets decrease at cell %llu", (unsigned long long)i);
+        // A wavefront sums its 64 lanes' list lengths in 32 bits.
+        if (cell_start[i + 1] - cell_start[i] >= (1u << 26))
+            return fail(TGK_ERR_INVALID, "cell %llu lists %u candidates; a cell takes fewer than 2^26",
+                        (unsigned long long)i, cell_start[i + 1] - cell_start[i]);
+    }
     for (uint64_t i = 0; i < n_entries; i++) {
         uint32_t slot;
         memcpy(&slot, entries + 12 * i + 3, sizeof slot);
