@@ -109,9 +109,9 @@ int tgk_fast_shape_info(int32_t shape, int32_t *lanes_per_block, int32_t *lds_co
  * started and fewer than half of them are still in flight, it saves those
  * and the next generation resumes them packed into full wavefronts. A
  * generation is cut short only while half its streamlines number at least
- * min_streamlines (default 16384 for SH models; DTI calls run in one
- * generation); a large value runs every call in one launch. Results do not
- * depend on it. */
+ * min_streamlines (default 16384 for SH models, 65536 on mesh domains; DTI
+ * calls run in one generation); a large value runs every call in one
+ * launch. Results do not depend on it. */
 int tgk_runtime_set_fast_repack(tgk_runtime *rt, int32_t min_streamlines);
 /* No counterpart in the reference (its kernel reads a voxel row every step):
  * how many coefficient rows and mask values the last FAST-path tracking

@@ -92,6 +92,8 @@ constexpr int FAST_GENERATIONS = 4;
 // By default a generation stops early only while it has at least twice
 // this many streamlines; below that, repacking them gains too little.
 constexpr uint32_t FAST_MIN_REPACK = 16384;
+// The same on mesh domains, whose tail generations gained from fuller ones.
+constexpr uint32_t FAST_MIN_REPACK_MESH = 65536;
 // float4s per saved streamline.
 constexpr int FAST_CONTINUATION_VEC4 = 4;
 // Counters of streamlines that left a generation: FAST_END_SHARDS u32, one
