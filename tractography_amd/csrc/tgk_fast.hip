@@ -611,6 +611,10 @@ void track_fast_kernel(FastLaunch f)
     constexpr int G = point_group<MODEL, ALGO, LMAX>();
     Lane<G> L;
     L.g = -1;
+    // An idle lane's position is read (never used) by the wavefront-wide
+    // boundary test before its first seed.
+    L.point = L.previous = v3(0.0f, 0.0f, 0.0f);
+    L.steps = 0;
     L.held = 0;
     L.store_due = false;
     L.row_read = false;
