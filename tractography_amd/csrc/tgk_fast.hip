@@ -424,6 +424,8 @@ __device__ __forceinline__ void wave_segment_test(const MeshView &m, Lane<G> &L,
             count = L.cand_last - L.cand_first;
         }
     }
+    if (__ballot(count != 0) == 0)
+        return;
     // Pairs [begin, end) of the flattened list belong to this lane. A list
     // holds fewer than 2^26 candidates (tgk_domain_set_segment_grid), so the
     // sum of 64 fits 32 bits with room for the last round's base + 64.
